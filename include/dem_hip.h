@@ -1,0 +1,176 @@
+/*
+ * dem_hip.h — C ABI of libdemhip.so, the MI355X (gfx950) implementation of the
+ * DEM droplet hot path: dipole force + Oseen convection (direct O(N^2)),
+ * spatial-hash contact detection, the explicit Euler step and the flat
+ * Barnes-Hut octree.
+ *
+ * Conventions (all entry points):
+ *   - Buffers are device pointers owned by the caller (PyTorch ROCm tensors in
+ *     the Python host layer).  The library never frees caller memory and never
+ *     allocates device memory: every scratch buffer is a caller-provided
+ *     workspace whose size is returned by the matching *_workspace_bytes().
+ *   - Positions are (N,3) row-major float64 in metres, radii (N,) float64,
+ *     forces (N,3) float64 in newtons, velocities (N,3) float64 in m/s — the
+ *     same arrays the reference passes between its Python operators.
+ *   - Every call is asynchronous on `stream` (a hipStream_t; NULL = default
+ *     stream) and returns DEM_OK or a negative status; dem_last_error() gives a
+ *     thread-local message for the last failure on the calling thread.
+ *   - There is no global mutable state: calls with distinct workspaces may run
+ *     concurrently on distinct streams.
+ */
+#ifndef DEM_HIP_H
+#define DEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* dem_stream_t; /* hipStream_t */
+
+enum dem_status {
+  DEM_OK = 0,
+  DEM_ERR_INVALID = -1,   /* bad argument (n < 0, null pointer, L <= 0 ...)        */
+  DEM_ERR_CAPACITY = -2,  /* output / workspace capacity exceeded                    */
+  DEM_ERR_HIP = -3,       /* a HIP runtime call failed                               */
+  DEM_ERR_WORKSPACE = -4  /* workspace smaller than *_workspace_bytes() demands       */
+};
+
+/* Physical constants of one force evaluation.
+ * Reference: dem/force_calculator/direct_droplet_force_calculator.py:23-54
+ * (m_const = 12*pi*eps0*eps_oil*E^2, eta_const = 1/(8*pi*eta_oil), L,
+ *  boundary_mode_int) and dem/octree/force_tree.py:54-57. */
+typedef struct dem_phys {
+  double m_const;   /* dipole force constant (N/m^2)                 */
+  double eta_const; /* Oseen prefactor 1/(8 pi eta_oil) (1/(Pa s))    */
+  double L;         /* periodic box edge (m)                           */
+  int32_t periodic; /* 1: minimum-image convention, 0: open boundary   */
+  int32_t reserved;
+} dem_phys;
+
+/* COMSOL periodic-image correction of the Oseen sum (trilinear lookups on a
+ * regular n^3 grid over [grid_min, -grid_min]^3).
+ * Reference: direct_droplet_force_calculator.py:74-181, :413-464 and
+ * dem/octree/flat_tree.py:141-167.  `grid` is float32 (u,v,w,0) per node,
+ * node (ix,iy,iz) at ((ix*n)+iy)*n+iz; `grid64` the same in float64 (used only
+ * by the FP64 oracle kernel; may be NULL otherwise). */
+typedef struct dem_correction {
+  const float* grid;
+  const double* grid64;
+  int32_t n;
+  int32_t enabled;
+  double grid_min;
+  double inv_dx;
+  double Fz_inv;
+  double L_ratio;   /* L_comsol / L_sim                                   */
+  double eta_ratio; /* eta_comsol / eta_oil                               */
+  double self_coeff;/* w(0,0,0)/Fz * eta_ratio * L_ratio                  */
+} dem_correction;
+
+const char* dem_last_error(void);
+int dem_version(void);
+/* Number of HIP devices visible to the library's runtime (or a negative
+ * status).  Used by the host layer to fail loudly without a GPU. */
+int dem_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Direct O(N^2) dipole force + Oseen convection (FP32 compute, FP64 I/O)    */
+/* ------------------------------------------------------------------------ */
+
+size_t dem_direct_workspace_bytes(int n);
+
+/* Forces only.  Replaces DirectDropletForceCalculator._calculate_forces
+ * (direct_droplet_force_calculator.py:202-253) behind calculate() (:260-305). */
+int dem_direct_forces(const double* pos, const double* radii, int n,
+                      const dem_phys* phys, double* forces,
+                      void* ws, size_t ws_bytes, dem_stream_t stream);
+
+/* Convection from caller-supplied forces.  Replaces
+ * _calculate_convection_velocities (:354-468) behind calculate_convection
+ * (:307-352).  `corr` may be NULL (no periodic correction). */
+int dem_direct_convection(const double* pos, const double* radii,
+                          const double* forces, int n, const dem_phys* phys,
+                          const dem_correction* corr, double* velocities,
+                          void* ws, size_t ws_bytes, dem_stream_t stream);
+
+/* Fused force + convection.  Replaces calculate_forces_and_convection
+ * (:470-527): phase 1 (forces), then phase 2 (Oseen sum of those forces). */
+int dem_direct_forces_and_convection(const double* pos, const double* radii,
+                                     int n, const dem_phys* phys,
+                                     const dem_correction* corr,
+                                     double* forces, double* velocities,
+                                     void* ws, size_t ws_bytes,
+                                     dem_stream_t stream);
+
+/* FP64 oracle twin of dem_direct_forces_and_convection: the reference's
+ * arithmetic in float64 (same branches, masks and minimum-image rule), one
+ * target per thread.  Used as the checker at sizes the CPU oracle cannot
+ * reach in seconds.  No workspace. */
+int dem_direct_forces_and_convection_f64(const double* pos, const double* radii,
+                                         int n, const dem_phys* phys,
+                                         const dem_correction* corr,
+                                         double* forces, double* velocities,
+                                         dem_stream_t stream);
+
+/* Tuning knob of the direct kernels (j-split factor and FP32 tile length);
+ * 0 keeps the built-in heuristic.  Returns the previous values packed as
+ * (split << 16) | tile. */
+int dem_direct_set_tuning(int split, int tile);
+
+/* ------------------------------------------------------------------------ */
+/* Spatial-hash contact detection                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace for n particles and a hash table of `table_size` buckets
+ * (table_size = next_pow2(2 * capacity) in the reference,
+ * spatial_hash_collision_detector.py:42-43). */
+size_t dem_hash_workspace_bytes(int n, int table_size);
+
+/* Contact pairs (i < j, dist <= r_i + r_j) found through the reference's
+ * cell hash and 27-cell stencil (spatial_hash_collision_detector.py:73-229):
+ * cell = 2*max(r), g = max(1, ceil(L/cell)), periodic wrap or open clamp,
+ * hash (ix*73856093 ^ iy*19349663 ^ iz*83492791) & (table_size-1).
+ * Pairs are written unique and lexicographically sorted to
+ * pairs[0 .. 2*min(count, pair_capacity)); *count_out (device int32) receives
+ * the number of distinct pairs (may exceed pair_capacity: the caller then
+ * retries with a larger buffer).  periodic: 1 periodic, 0 open. */
+int dem_hash_pairs(const double* pos, const double* radii, int n, double L,
+                   int periodic, int table_size, int32_t* pairs,
+                   int pair_capacity, int32_t* count_out, void* ws,
+                   size_t ws_bytes, dem_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Explicit Euler step (float64, bit-exact with the reference's numpy)        */
+/* ------------------------------------------------------------------------ */
+
+/* pos += (forces / (stokes_factor * r) + conv) * dt, then pos %= L if
+ * periodic (Python float-mod semantics).  euler_droplet_solver.py:124-131.
+ * conv may be NULL (treated as zero). */
+int dem_euler_step(double* pos, const double* forces, const double* conv,
+                   const double* radii, int n, double stokes_factor, double dt,
+                   double L, int periodic, dem_stream_t stream);
+
+/* pos += vel * dt, then pos %= L if periodic (tree path,
+ * euler_droplet_solver.py:119, :128-131). */
+int dem_euler_step_total(double* pos, const double* vel, int n, double dt,
+                         double L, int periodic, dem_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing (measurement support; no counterpart in the reference)       */
+/* ------------------------------------------------------------------------ */
+
+/* While started, every launch of a hot kernel is bracketed by a HIP event
+ * pair recorded on the launch stream.  dem_profile_read() synchronises those
+ * events and returns the summed duration (ms) and launch count of `kernel`
+ * (e.g. "k_force_tile", "k_conv_tile").  Process-wide, single-threaded use. */
+int dem_profile_start(void);
+int dem_profile_stop(void);
+int dem_profile_read(const char* kernel, double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEM_HIP_H */

@@ -1,0 +1,275 @@
+"""FP64 CPU oracle of the reference DEM hot path — TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker (or the
+timed CPU baseline).  The product package ``dem`` never imports it; the HIP
+path fails loudly when its extension is missing instead of falling back here.
+
+What each function restates (reference file:line) and what pins it
+(tests/test_oracle_*.py):
+
+* ``direct_forces`` — ``_calculate_forces`` + ``_calculate_one_force``
+  (dem/force_calculator/direct_droplet_force_calculator.py:183-253).  Pinned by
+  the two-body closed forms along z and x, Newton's third law, F = -grad U of
+  U = (m/3) r_i^3 r_j^3 (r^2 - 3 dz^2)/r^5 (central finite differences), the
+  minimum-image sign flip (reference test_direct...py:77), zero force on a
+  symmetric cubic lattice, E^2 / length^2 scaling and a pure-Python loop.
+* ``direct_convection`` — ``_calculate_convection_velocities`` (:354-468).
+  Pinned by the two-body Oseen closed form v = eta/r (F + (F.r^)r^), symmetry
+  of the Oseen tensor, v proportional to 1/eta, and a pure-Python loop.  The
+  COMSOL term is pinned by trilinear interpolation reproducing grid nodes and
+  the reference's own symmetry / convolution identities
+  (tests/periodic_correction/test_comsol_lattice_correction.py:62-142).
+* ``hash_pairs`` — ``SpatialHashCollisionDetector.detect``
+  (dem/collision_detector/spatial_hash_collision_detector.py:73-316).  Pinned
+  by the reference's exact small cases (test_spatial_hash...py:24-113) and by
+  brute force: every hash pair is a true contact, and the hash set equals the
+  brute-force set filtered by the 27-cell stencil (the reference's periodic
+  miss included).
+* ``merge`` — ``DropletSolution.generate_next_solution``
+  (dem/solution/droplet_solution.py:208-287).  Pinned by the reference's
+  test_collision_merges_two_drops (3 -> 2 drops, r^3 conserved, merged drop
+  appended last) plus the greedy-skip rule on chains.
+* ``euler_step`` — ``EulerDropletSolver.solve`` update
+  (dem/solver/euler_droplet_solver.py:119-131).  Pinned by hand-computed
+  single steps and the periodic wrap.
+* Octree functions (``octree_*``) — ``FlatOctree`` (dem/octree/flat_tree.py).
+  Pinned by the build invariants of the reference tests, theta = 0 equalling
+  the direct sum, and the reference's tree-vs-direct error bands.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "liboracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile the C restatement (gcc, OpenMP) into oracle/build/."""
+    if force or not LIB_PATH.exists() or any(
+        p.stat().st_mtime > LIB_PATH.stat().st_mtime for p in HERE.glob("*.c")
+    ):
+        subprocess.run(["make", "-C", str(HERE), "-B" if force else "all"], check=True,
+                       capture_output=True)
+    return LIB_PATH
+
+
+class _Corr(ctypes.Structure):
+    _fields_ = [
+        ("u", ctypes.c_void_p),
+        ("v", ctypes.c_void_p),
+        ("w", ctypes.c_void_p),
+        ("n", ctypes.c_int),
+        ("enabled", ctypes.c_int),
+        ("gmin", ctypes.c_double),
+        ("inv_dx", ctypes.c_double),
+        ("Fz_inv", ctypes.c_double),
+        ("L_ratio", ctypes.c_double),
+        ("eta_ratio", ctypes.c_double),
+        ("self_coeff", ctypes.c_double),
+    ]
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        _lib.oracle_direct_forces_range.argtypes = [i, i, i, p, p, d, d, i, p]
+        _lib.oracle_direct_convection_range.argtypes = [i, i, i, p, p, d, d, i, p, p]
+        _lib.oracle_corr_eval.argtypes = [p, i, p, p]
+        _lib.oracle_hash_pairs.argtypes = [i, p, p, d, i, i, p, i]
+        _lib.oracle_hash_pairs.restype = i
+        _lib.oracle_brute_pairs.argtypes = [i, p, p, d, i, p, i]
+        _lib.oracle_brute_pairs.restype = i
+        _lib.oracle_num_threads.restype = i
+        _lib.oracle_set_threads.argtypes = [i]
+    return _lib
+
+
+def _c(a: np.ndarray, dtype=np.float64) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def set_threads(t: int) -> None:
+    lib().oracle_set_threads(int(t))
+
+
+# ---------------------------------------------------------------- direct
+
+
+def direct_forces(pos, radii, m_const, L, periodic, targets=None) -> np.ndarray:
+    """Float64 dipole forces; ``targets`` = (i0, i1) limits the computed rows."""
+    pos = _c(pos).reshape(-1, 3)
+    radii = _c(radii)
+    n = pos.shape[0]
+    i0, i1 = targets if targets is not None else (0, n)
+    F = np.zeros((n, 3))
+    lib().oracle_direct_forces_range(n, i0, i1, _ptr(pos), _ptr(radii), float(m_const), float(L),
+                                     int(bool(periodic)), _ptr(F))
+    return F
+
+
+class Correction:
+    """Float64 grids + scalars of the COMSOL correction, in the reference's
+    layout (grid_u/v/w of shape (n,n,n), direct...py:87-108)."""
+
+    def __init__(self, grid_u, grid_v, grid_w, grid_min, grid_dx, Fz_comsol, L_ratio, eta_ratio,
+                 self_coeff):
+        self.u, self.v, self.w = _c(grid_u), _c(grid_v), _c(grid_w)
+        self.st = _Corr(_ptr(self.u), _ptr(self.v), _ptr(self.w), self.u.shape[0], 1,
+                        float(grid_min), 1.0 / float(grid_dx), 1.0 / float(Fz_comsol),
+                        float(L_ratio), float(eta_ratio), float(self_coeff))
+
+    def eval(self, pts) -> np.ndarray:
+        pts = _c(pts).reshape(-1, 3)
+        out = np.zeros_like(pts)
+        lib().oracle_corr_eval(ctypes.byref(self.st), pts.shape[0], _ptr(pts), _ptr(out))
+        return out
+
+
+def direct_convection(pos, forces, eta_const, L, periodic, corr: Correction | None = None,
+                      targets=None) -> np.ndarray:
+    pos = _c(pos).reshape(-1, 3)
+    forces = _c(forces).reshape(-1, 3)
+    n = pos.shape[0]
+    i0, i1 = targets if targets is not None else (0, n)
+    V = np.zeros((n, 3))
+    cp = ctypes.byref(corr.st) if corr is not None else None
+    lib().oracle_direct_convection_range(n, i0, i1, _ptr(pos), _ptr(forces), float(eta_const),
+                                         float(L), int(bool(periodic)), cp, _ptr(V))
+    return V
+
+
+# ---------------------------------------------------------------- hash
+
+
+def hash_pairs(pos, radii, L, periodic, table_size) -> tuple[bool, np.ndarray]:
+    """(is_collision, unique sorted (K,2) int32 pairs) as detect() returns."""
+    pos = _c(pos).reshape(-1, 3)
+    radii = _c(radii)
+    n = pos.shape[0]
+    cap = max(16, 4 * n)
+    while True:
+        out = np.zeros((cap, 2), dtype=np.int32)
+        k = lib().oracle_hash_pairs(n, _ptr(pos), _ptr(radii), float(L), int(bool(periodic)),
+                                    int(table_size), _ptr(out), cap)
+        if k < 0:
+            return False, np.empty((0, 2), dtype=np.int32)
+        if k <= cap:
+            return k > 0, out[:k].copy()
+        cap = k
+
+
+def brute_pairs(pos, radii, L, periodic) -> np.ndarray:
+    pos = _c(pos).reshape(-1, 3)
+    radii = _c(radii)
+    n = pos.shape[0]
+    cap = max(16, 4 * n)
+    while True:
+        out = np.zeros((cap, 2), dtype=np.int32)
+        k = lib().oracle_brute_pairs(n, _ptr(pos), _ptr(radii), float(L), int(bool(periodic)),
+                                     _ptr(out), cap)
+        if k <= cap:
+            return out[:k].copy()
+        cap = k
+
+
+# ---------------------------------------------------------------- merge / Euler
+
+
+def merge(positions, radii, pairs):
+    """Greedy volume-conserving merge (droplet_solution.py:208-287).
+
+    Returns (new_positions, new_radii, resulting_indices).  Pairs are ordered
+    by their first index; ties keep lexicographic order (the reference's
+    non-stable argsort leaves tie order to numpy's sort)."""
+    pairs = np.asarray(pairs).reshape(-1, 2)
+    order = np.argsort(pairs[:, 0], kind="stable")
+    pairs = pairs[order]
+    new_r = np.array(radii, dtype=np.float64, copy=True)
+    new_p = np.array(positions, dtype=np.float64, copy=True)
+    used: set[int] = set()
+    add_r, add_p = [], []
+    for a, b in pairs:
+        a, b = int(a), int(b)
+        if a in used or b in used:
+            continue
+        r1c = new_r[a] ** 3
+        r2c = new_r[b] ** 3
+        add_r.append((r1c + r2c) ** (1 / 3))
+        add_p.append((new_p[a] * r1c + new_p[b] * r2c) / (r1c + r2c))
+        used.add(a)
+        used.add(b)
+    drop = sorted(used, reverse=True)
+    new_r = np.delete(new_r, drop)
+    new_p = np.delete(new_p, drop, axis=0)
+    new_r = np.append(new_r, add_r)
+    new_p = np.vstack([new_p, add_p]) if add_p else new_p
+    res = list(range(len(new_r) - len(add_r), len(new_r)))
+    return new_p, new_r, res
+
+
+def euler_step(positions, forces, conv, radii, stokes_factor, dt, L, periodic):
+    """One direct-path Euler step (euler_droplet_solver.py:124-131)."""
+    x = np.array(positions, dtype=np.float64, copy=True)
+    mig = forces / (stokes_factor * radii[:, np.newaxis])
+    x += (mig + conv) * dt
+    if periodic:
+        x %= L
+    return x
+
+
+def euler_step_total(positions, vel, dt, L, periodic):
+    """Tree-path Euler step (euler_droplet_solver.py:119, :128-131)."""
+    x = np.array(positions, dtype=np.float64, copy=True)
+    x += vel * dt
+    if periodic:
+        x %= L
+    return x
+
+
+def stokes_factor(eta_oil, eta_water) -> float:
+    """euler_droplet_solver.py:37-41."""
+    return 2 * np.pi * eta_oil * (2 * eta_oil + 3 * eta_water) / (eta_oil + eta_water)
+
+
+EPS0 = 8.85418781762039e-12
+
+
+def m_const(eps_oil, E) -> float:
+    """direct_droplet_force_calculator.py:30."""
+    return 12 * np.pi * EPS0 * eps_oil * E ** 2
+
+
+def eta_const(eta_oil) -> float:
+    """direct_droplet_force_calculator.py:31."""
+    return 1 / (8 * np.pi * eta_oil)
+
+
+__all__ = [
+    "build", "lib", "num_threads", "set_threads", "direct_forces", "direct_convection",
+    "Correction", "hash_pairs", "brute_pairs", "merge", "euler_step", "euler_step_total",
+    "stokes_factor", "m_const", "eta_const", "EPS0",
+]
+
+if os.environ.get("DEM_ORACLE_EAGER_BUILD"):
+    build()

@@ -1,0 +1,730 @@
+// Direct O(N^2) dipole force + Oseen convection for gfx950.
+//
+// Reference semantics (dem/force_calculator/direct_droplet_force_calculator.py):
+//   phase 1 (_calculate_forces, :202-253): for every target i, sum over j != i
+//     with minimum-image d = x_j - x_i (:233-240) and contact mask
+//     |d| > r_i + r_j (:244-249) of
+//       F_i += m r_i^3 r_j^3 / |d|^7 * (dx(4dz^2-dx^2-dy^2), dy(..), dz(2dz^2-3dx^2-3dy^2))
+//   phase 2 (_calculate_convection_velocities, :354-468): for every i, sum over
+//     j != i (no contact mask) of the Oseen tensor applied to F_j,
+//       v_i += eta (F_j/|x| + x (x.F_j)/|x|^3),  x = x_i - x_j (minimum image),
+//     plus, when enabled, the COMSOL periodic-image correction (three permuted
+//     trilinear lookups, :413-457) and the self term g_self F_i (:459-464).
+//
+// MI355X design:
+//   * Positions are quantised to 32-bit fixed point.  Periodic boxes map [0,L)
+//     onto the full uint32 ring, so (int32)(q_j - q_i) IS the minimum-image
+//     difference (exact, one integer subtract); open boxes map the bounding box
+//     onto [0, 2^31) so the same subtract never wraps.  Quantisation error is
+//     L/2^33 (~1.6e-13 m for a 1.4 mm box).
+//   * Arithmetic is FP32 VALU (the pair kernels are not contractions: no MFMA).
+//     Work units are 2^22 fixed units (L/1024 for periodic boxes), which keeps
+//     r^-7 and the cubic numerators inside the FP32 range for any droplet
+//     size / box ratio up to ~1e5.
+//   * One target per lane, 256 lanes per workgroup; the j (source) loop reads a
+//     wave-uniform 32-byte record per source, which the compiler serves from
+//     the scalar data cache (s_load) into SGPRs, so sources cost no VALU or LDS
+//     traffic.  The j range is split over blockIdx.y to fill the 256 CUs; each
+//     split writes a float64 partial that a deterministic reduction kernel sums.
+//   * Inside a split the FP32 sum runs over `tile` sources and is then added to
+//     a float64 accumulator (blocked summation: the FP32 rounding error grows
+//     with the tile length, not with N).
+#include <math.h>
+
+#include "dem_common.h"
+#include "profile.h"
+
+namespace {
+
+constexpr int WG = 256;
+constexpr double kTwo32 = 4294967296.0;
+constexpr double kSigma = 0x1p-22;   // work unit / fixed unit
+constexpr float kSigma2f = 0x1p-44f; // kSigma^2
+constexpr double kSigma3 = 0x1p-66;
+
+// Source / target record: 32 bytes, read whole by the pair kernels.
+struct alignas(16) Rec {
+  uint32_t x, y, z;  // fixed-point position
+  float r;           // radius in work units
+  float r3;          // radius^3 in work units
+  float fx, fy, fz;  // force / Fs (phase-2 sources)
+};
+static_assert(sizeof(Rec) == 32, "record layout");
+
+// Per-call scales, computed on the device so open-boundary boxes need no host sync.
+struct DevScales {
+  double origin[3];  // subtracted before quantisation (open boxes)
+  double inv_U;      // fixed units per metre
+  double U;          // metres per fixed unit
+  double lambda;     // metres per work unit (= U * 2^22)
+  double Fs;         // newtons per force unit of Rec::f
+  int periodic;
+  int pad;
+};
+
+int g_split = 0;  // 0: heuristic
+int g_tile = 0;   // 0: heuristic
+
+inline int split_max(int n) {
+  long long cap = (1ll << 21) / (n > 0 ? n : 1);
+  long long s = (n + 63) / 64;
+  if (cap < s) s = cap;
+  if (s > 64) s = 64;
+  if (s < 1) s = 1;
+  return (int)s;
+}
+
+inline int choose_split(int n) {
+  int iblocks = (n + WG - 1) / WG;
+  int s = g_split > 0 ? g_split : (2048 + iblocks - 1) / iblocks;
+  int smax = split_max(n);
+  if (s > smax) s = smax;
+  if (s < 1) s = 1;
+  return s;
+}
+
+inline int choose_tile() { return g_tile > 0 ? g_tile : 128; }
+
+constexpr int kBBoxBlocks = 256;
+
+struct Layout {
+  DevScales* sc;
+  double* bbox;  // [kBBoxBlocks][6]
+  Rec* rec;
+  double* part;  // [split][6][n]
+  size_t bytes;
+};
+
+Layout carve(void* ws, int n) {
+  dem::Carver c(ws);
+  Layout l;
+  l.sc = c.take<DevScales>(1);
+  l.bbox = c.take<double>(kBBoxBlocks * 6);
+  l.rec = c.take<Rec>((size_t)(n > 0 ? n : 1));
+  l.part = c.take<double>((size_t)split_max(n) * 6 * (size_t)(n > 0 ? n : 1));
+  l.bytes = dem::align_up(c.used, 256);
+  return l;
+}
+
+// ---------------------------------------------------------------- set-up
+
+__global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__ pos, int n,
+                                                      double* __restrict__ part) {
+  __shared__ double s_lo[3][256], s_hi[3][256];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int a = 0; a < 3; ++a) {
+      double v = pos[3 * (size_t)i + a];
+      lo[a] = fmin(lo[a], v);
+      hi[a] = fmax(hi[a], v);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    s_lo[a][threadIdx.x] = lo[a];
+    s_hi[a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        s_lo[a][threadIdx.x] = fmin(s_lo[a][threadIdx.x], s_lo[a][threadIdx.x + w]);
+        s_hi[a][threadIdx.x] = fmax(s_hi[a][threadIdx.x], s_hi[a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; ++a) {
+      part[6 * blockIdx.x + a] = s_lo[a][0];
+      part[6 * blockIdx.x + 3 + a] = s_hi[a][0];
+    }
+}
+
+// Single thread: fold the bounding box and derive the per-call scales.
+__global__ void k_setup(const double* __restrict__ bbox, int nblk, double L, int periodic,
+                        double m_const, DevScales* __restrict__ sc) {
+  if (threadIdx.x != 0) return;
+  DevScales s;
+  s.periodic = periodic;
+  s.pad = 0;
+  if (periodic) {
+    s.origin[0] = s.origin[1] = s.origin[2] = 0.0;
+    s.U = L / kTwo32;
+    s.inv_U = kTwo32 / L;
+  } else {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nblk; ++b)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fmin(lo[a], bbox[6 * b + a]);
+        hi[a] = fmax(hi[a], bbox[6 * b + 3 + a]);
+      }
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) ext = fmax(ext, hi[a] - lo[a]);
+    if (!(ext > 0.0) || !isfinite(ext)) ext = 1.0;
+    for (int a = 0; a < 3; ++a) s.origin[a] = isfinite(lo[a]) ? lo[a] : 0.0;
+    // Map the box onto [0, 2^31 - 2^12]: differences never leave int32.
+    s.U = ext / (2147483648.0 - 4096.0);
+    s.inv_U = 1.0 / s.U;
+  }
+  s.lambda = s.U / kSigma;
+  double fs = m_const * s.lambda * s.lambda;
+  s.Fs = (fs > 0.0 && isfinite(fs)) ? fs : 1.0;
+  *sc = s;
+}
+
+// Quantise positions, scale radii, and (optionally) load caller forces as sources.
+__global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
+                                              const double* __restrict__ radii,
+                                              const double* __restrict__ forces, int n,
+                                              const DevScales* __restrict__ scp,
+                                              Rec* __restrict__ rec) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const DevScales sc = *scp;
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    double x = pos[3 * (size_t)i + a];
+    if (sc.periodic) {
+      double u = x * sc.inv_U * 0x1p-32;  // x / L
+      u -= floor(u);
+      q[a] = (uint32_t)(unsigned long long)rint(u * kTwo32);
+    } else {
+      q[a] = (uint32_t)(long long)rint((x - sc.origin[a]) * sc.inv_U);
+    }
+  }
+  double rl = radii[i] / sc.lambda;
+  Rec o;
+  o.x = q[0];
+  o.y = q[1];
+  o.z = q[2];
+  o.r = (float)rl;
+  o.r3 = (float)(rl * rl * rl);
+  if (forces) {
+    o.fx = (float)(forces[3 * (size_t)i + 0] / sc.Fs);
+    o.fy = (float)(forces[3 * (size_t)i + 1] / sc.Fs);
+    o.fz = (float)(forces[3 * (size_t)i + 2] / sc.Fs);
+  } else {
+    o.fx = o.fy = o.fz = 0.f;
+  }
+  rec[i] = o;
+}
+
+// ---------------------------------------------------------------- phase 1
+
+__global__ __launch_bounds__(WG) void k_force_tile(const Rec* __restrict__ rec, int n, int jchunk,
+                                                   int tile, double* __restrict__ part) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  const int s = blockIdx.y;
+  const int j0 = s * jchunk;
+  const int j1 = min(n, j0 + jchunk);
+  const uint32_t xi = rec[ii].x, yi = rec[ii].y, zi = rec[ii].z;
+  const float ri = rec[ii].r;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int jt = j0; jt < j1; jt += tile) {
+    const int je = min(j1, jt + tile);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 4
+    for (int j = jt; j < je; ++j) {
+      const Rec sj = rec[j];  // wave-uniform: scalar loads
+      const float dx = (float)(int32_t)(sj.x - xi);
+      const float dy = (float)(int32_t)(sj.y - yi);
+      const float dz = (float)(int32_t)(sj.z - zi);
+      const float dz2 = dz * dz;
+      const float dxy2 = dx * dx + dy * dy;
+      const float r2 = (dxy2 + dz2) * kSigma2f;  // work units^2
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float ir2 = ir * ir;
+      const float ir7 = (ir2 * ir2) * (ir2 * ir);
+      const float rs = ri + sj.r;
+      const float s7 = (r2 > rs * rs) ? sj.r3 * ir7 : 0.f;  // contact mask; also drops j == i
+      const float axy = 4.f * dz2 - dxy2;
+      const float azz = 2.f * dz2 - 3.f * dxy2;
+      const float txy = s7 * axy;
+      fx += txy * dx;
+      fy += txy * dy;
+      fz += (s7 * azz) * dz;
+    }
+    ax += (double)fx;
+    ay += (double)fy;
+    az += (double)fz;
+  }
+  if (i < n) {
+    part[(size_t)(6 * s + 0) * n + i] = ax;
+    part[(size_t)(6 * s + 1) * n + i] = ay;
+    part[(size_t)(6 * s + 2) * n + i] = az;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__ part, int S, int n,
+                                                      const double* __restrict__ radii,
+                                                      const DevScales* __restrict__ scp,
+                                                      double m_const, double* __restrict__ F,
+                                                      Rec* __restrict__ rec) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int s = 0; s < S; ++s) {
+    sx += part[(size_t)(6 * s + 0) * n + i];
+    sy += part[(size_t)(6 * s + 1) * n + i];
+    sz += part[(size_t)(6 * s + 2) * n + i];
+  }
+  const DevScales sc = *scp;
+  const double rl = radii[i] / sc.lambda;
+  const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
+  const double Fx = c * sx, Fy = c * sy, Fz = c * sz;
+  F[3 * (size_t)i + 0] = Fx;
+  F[3 * (size_t)i + 1] = Fy;
+  F[3 * (size_t)i + 2] = Fz;
+  rec[i].fx = (float)(Fx / sc.Fs);
+  rec[i].fy = (float)(Fy / sc.Fs);
+  rec[i].fz = (float)(Fz / sc.Fs);
+}
+
+// ---------------------------------------------------------------- phase 2
+
+struct CorrArgs {
+  const float4* grid;  // (u, v, w, 0) per node
+  int n;
+  float nmax;          // n - 1
+  double gmin, inv_dx, L_ratio;
+};
+
+// Clamped grid coordinate: index in [0, n-2] and fraction (reference :141-157).
+__device__ __forceinline__ void grid_coord(float g, float nmax, int n, int& i, float& t) {
+  g = fminf(fmaxf(g, 0.f), nmax);
+  i = min((int)floorf(g), n - 2);
+  t = g - (float)i;
+}
+
+template <bool CORR>
+__global__ __launch_bounds__(WG) void k_conv_tile(const Rec* __restrict__ rec, int n, int jchunk,
+                                                  int tile, const DevScales* __restrict__ scp,
+                                                  CorrArgs ca, double* __restrict__ part) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  const int s = blockIdx.y;
+  const int j0 = s * jchunk;
+  const int j1 = min(n, j0 + jchunk);
+  const uint32_t xi = rec[ii].x, yi = rec[ii].y, zi = rec[ii].z;
+  // Grid coordinate of x_i - x_j = -U D  is  a * D + b.
+  float ga = 0.f, gb = 0.f;
+  if (CORR) {
+    const double U = scp->U;
+    ga = (float)(-U * ca.L_ratio * ca.inv_dx);
+    gb = (float)(-ca.gmin * ca.inv_dx);
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int jt = j0; jt < j1; jt += tile) {
+    const int je = min(j1, jt + tile);
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll 4
+    for (int j = jt; j < je; ++j) {
+      const Rec sj = rec[j];
+      const float dx = (float)(int32_t)(sj.x - xi);
+      const float dy = (float)(int32_t)(sj.y - yi);
+      const float dz = (float)(int32_t)(sj.z - zi);
+      const float r2 = dx * dx + dy * dy + dz * dz;  // fixed units^2
+      const bool other = r2 > 0.f;                     // drops j == i
+      const float ir = other ? __builtin_amdgcn_rsqf(r2) : 0.f;
+      const float dot = dx * sj.fx + dy * sj.fy + dz * sj.fz;
+      const float t = dot * (ir * ir);
+      vx += ir * (sj.fx + t * dx);
+      vy += ir * (sj.fy + t * dy);
+      vz += ir * (sj.fz + t * dz);
+      if (CORR) {
+        int gx, gy, gz;
+        float tx, ty, tz;
+        grid_coord(ga * dx + gb, ca.nmax, ca.n, gx, tx);
+        grid_coord(ga * dy + gb, ca.nmax, ca.n, gy, ty);
+        grid_coord(ga * dz + gb, ca.nmax, ca.n, gz, tz);
+        // Eight trilinear weights w[a][b][c] of fractions (tx, ty, tz).
+        float w[2][2][2];
+        const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
+        w[0][0][0] = ox * oy * oz; w[0][0][1] = ox * oy * tz;
+        w[0][1][0] = ox * ty * oz; w[0][1][1] = ox * ty * tz;
+        w[1][0][0] = tx * oy * oz; w[1][0][1] = tx * oy * tz;
+        w[1][1][0] = tx * ty * oz; w[1][1][1] = tx * ty * tz;
+        const int nn = ca.n;
+        // G_z: grid at (x, y, z) -> (u, v, w)
+        float gzx = 0.f, gzy = 0.f, gzz = 0.f;
+        // G_x: grid at (z, y, x) -> (w, v, u)
+        float gxx = 0.f, gxy = 0.f, gxz = 0.f;
+        // G_y: grid at (x, z, y) -> (u, w, v)
+        float gyx = 0.f, gyy = 0.f, gyz = 0.f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              const float4 g1 = ca.grid[((gx + a) * nn + (gy + b)) * nn + (gz + c)];
+              const float w1 = w[a][b][c];
+              gzx += w1 * g1.x; gzy += w1 * g1.y; gzz += w1 * g1.z;
+              // lookup at (gz+a, gy+b, gx+c) carries fractions (tz, ty, tx)
+              const float4 g2 = ca.grid[((gz + a) * nn + (gy + b)) * nn + (gx + c)];
+              const float w2 = w[c][b][a];
+              gxx += w2 * g2.z; gxy += w2 * g2.y; gxz += w2 * g2.x;
+              // lookup at (gx+a, gz+b, gy+c) carries fractions (tx, tz, ty)
+              const float4 g3 = ca.grid[((gx + a) * nn + (gz + b)) * nn + (gy + c)];
+              const float w3 = w[a][c][b];
+              gyx += w3 * g3.x; gyy += w3 * g3.z; gyz += w3 * g3.y;
+            }
+        const float fxm = other ? sj.fx : 0.f, fym = other ? sj.fy : 0.f, fzm = other ? sj.fz : 0.f;
+        ux += gxx * fxm + gyx * fym + gzx * fzm;
+        uy += gxy * fxm + gyy * fym + gzy * fzm;
+        uz += gxz * fxm + gyz * fym + gzz * fzm;
+      }
+    }
+    ax += (double)vx;
+    ay += (double)vy;
+    az += (double)vz;
+    if (CORR) {
+      cx += (double)ux;
+      cy += (double)uy;
+      cz += (double)uz;
+    }
+  }
+  if (i < n) {
+    part[(size_t)(6 * s + 0) * n + i] = ax;
+    part[(size_t)(6 * s + 1) * n + i] = ay;
+    part[(size_t)(6 * s + 2) * n + i] = az;
+    if (CORR) {
+      part[(size_t)(6 * s + 3) * n + i] = cx;
+      part[(size_t)(6 * s + 4) * n + i] = cy;
+      part[(size_t)(6 * s + 5) * n + i] = cz;
+    }
+  }
+}
+
+struct CorrScalars {
+  int enabled;
+  double common;      // Fz_inv * eta_ratio * L_ratio
+  double self_coeff;
+};
+
+__global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ part, int S, int n,
+                                                     const DevScales* __restrict__ scp,
+                                                     double eta_const, CorrScalars cs,
+                                                     const double* __restrict__ F,
+                                                     double* __restrict__ V) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double m[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+  for (int s = 0; s < S; ++s)
+    for (int a = 0; a < 3; ++a) {
+      m[a] += part[(size_t)(6 * s + a) * n + i];
+      if (cs.enabled) c[a] += part[(size_t)(6 * s + 3 + a) * n + i];
+    }
+  const DevScales sc = *scp;
+  const double km = eta_const * sc.Fs / sc.U;
+  const double kc = sc.Fs * cs.common;
+  for (int a = 0; a < 3; ++a) {
+    double v = km * m[a];
+    if (cs.enabled) v += kc * c[a] + cs.self_coeff * F[3 * (size_t)i + a];
+    V[3 * (size_t)i + a] = v;
+  }
+}
+
+// ---------------------------------------------------------------- FP64 oracle kernels
+
+__device__ __forceinline__ double mic64(double d, double L, int periodic) {
+  if (periodic && fabs(d) > L * 0.5) d -= copysign(L, d);
+  return d;
+}
+
+__global__ __launch_bounds__(256) void k_f64_forces(const double* __restrict__ pos,
+                                                    const double* __restrict__ r, int n, double m,
+                                                    double L, int periodic,
+                                                    double* __restrict__ F) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  const double ri = r[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    double dx = mic64(pos[3 * (size_t)j] - xi, L, periodic);
+    double dy = mic64(pos[3 * (size_t)j + 1] - yi, L, periodic);
+    double dz = mic64(pos[3 * (size_t)j + 2] - zi, L, periodic);
+    double rmag = sqrt(dx * dx + dy * dy + dz * dz);
+    double rj = r[j];
+    if (rmag > ri + rj) {
+      double r7 = rmag * rmag * rmag * rmag * rmag * rmag * rmag;
+      double M = m * (ri * ri * ri) * (rj * rj * rj) / r7;
+      double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+      double fxy = M * (4.0 * dz2 - dx2 - dy2);
+      fx += fxy * dx;
+      fy += fxy * dy;
+      fz += M * (2.0 * dz2 - 3.0 * dx2 - 3.0 * dy2) * dz;
+    }
+  }
+  F[3 * (size_t)i] = fx;
+  F[3 * (size_t)i + 1] = fy;
+  F[3 * (size_t)i + 2] = fz;
+}
+
+struct Corr64Args {
+  const double* grid;  // (u, v, w, 0) per node
+  int n, enabled;
+  double gmin, inv_dx, Fz_inv, L_ratio, eta_ratio, self_coeff;
+};
+
+__device__ void grid_idx64(double x, double gmin, double inv, int n, int& i, double& t) {
+  double f = (x - gmin) * inv;
+  f = fmax(0.0, fmin(f, (double)(n - 1)));
+  i = min((int)floor(f), n - 2);
+  t = f - (double)i;
+}
+
+__device__ double interp64(const double* g, int comp, int n, int ix, int iy, int iz, double tx,
+                           double ty, double tz) {
+  auto at = [&](int a, int b, int c) { return g[4 * (((size_t)(ix + a) * n + (iy + b)) * n + (iz + c)) + comp]; };
+  return at(0, 0, 0) * (1 - tx) * (1 - ty) * (1 - tz) + at(0, 0, 1) * (1 - tx) * (1 - ty) * tz +
+         at(0, 1, 0) * (1 - tx) * ty * (1 - tz) + at(0, 1, 1) * (1 - tx) * ty * tz +
+         at(1, 0, 0) * tx * (1 - ty) * (1 - tz) + at(1, 0, 1) * tx * (1 - ty) * tz +
+         at(1, 1, 0) * tx * ty * (1 - tz) + at(1, 1, 1) * tx * ty * tz;
+}
+
+__global__ __launch_bounds__(256) void k_f64_conv(const double* __restrict__ pos,
+                                                  const double* __restrict__ F, int n, double eta,
+                                                  double L, int periodic, Corr64Args c,
+                                                  double* __restrict__ V) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    double x = mic64(xi - pos[3 * (size_t)j], L, periodic);
+    double y = mic64(yi - pos[3 * (size_t)j + 1], L, periodic);
+    double z = mic64(zi - pos[3 * (size_t)j + 2], L, periodic);
+    double r = sqrt(x * x + y * y + z * z);
+    double r3 = r * r * r;
+    double fx = F[3 * (size_t)j], fy = F[3 * (size_t)j + 1], fz = F[3 * (size_t)j + 2];
+    vx += eta * ((1.0 / r + (x * x) / r3) * fx + (x * y) / r3 * fy + (x * z) / r3 * fz);
+    vy += eta * ((x * y) / r3 * fx + (1.0 / r + (y * y) / r3) * fy + (y * z) / r3 * fz);
+    vz += eta * ((x * z) / r3 * fx + (y * z) / r3 * fy + (1.0 / r + (z * z) / r3) * fz);
+    if (c.enabled) {
+      const double cL = c.L_ratio;
+      const double xc = x * cL, yc = y * cL, zc = z * cL;
+      int i1x, i1y, i1z, i2x, i2y, i2z, i3x, i3y, i3z;
+      double t1x, t1y, t1z, t2x, t2y, t2z, t3x, t3y, t3z;
+      grid_idx64(xc, c.gmin, c.inv_dx, c.n, i1x, t1x);
+      grid_idx64(yc, c.gmin, c.inv_dx, c.n, i1y, t1y);
+      grid_idx64(zc, c.gmin, c.inv_dx, c.n, i1z, t1z);
+      grid_idx64(zc, c.gmin, c.inv_dx, c.n, i2x, t2x);
+      grid_idx64(yc, c.gmin, c.inv_dx, c.n, i2y, t2y);
+      grid_idx64(xc, c.gmin, c.inv_dx, c.n, i2z, t2z);
+      grid_idx64(xc, c.gmin, c.inv_dx, c.n, i3x, t3x);
+      grid_idx64(zc, c.gmin, c.inv_dx, c.n, i3y, t3y);
+      grid_idx64(yc, c.gmin, c.inv_dx, c.n, i3z, t3z);
+      const int N = c.n;
+      double gz_x = interp64(c.grid, 0, N, i1x, i1y, i1z, t1x, t1y, t1z);
+      double gz_y = interp64(c.grid, 1, N, i1x, i1y, i1z, t1x, t1y, t1z);
+      double gz_z = interp64(c.grid, 2, N, i1x, i1y, i1z, t1x, t1y, t1z);
+      double gx_x = interp64(c.grid, 2, N, i2x, i2y, i2z, t2x, t2y, t2z);
+      double gx_y = interp64(c.grid, 1, N, i2x, i2y, i2z, t2x, t2y, t2z);
+      double gx_z = interp64(c.grid, 0, N, i2x, i2y, i2z, t2x, t2y, t2z);
+      double gy_x = interp64(c.grid, 0, N, i3x, i3y, i3z, t3x, t3y, t3z);
+      double gy_y = interp64(c.grid, 2, N, i3x, i3y, i3z, t3x, t3y, t3z);
+      double gy_z = interp64(c.grid, 1, N, i3x, i3y, i3z, t3x, t3y, t3z);
+      const double common = c.Fz_inv * c.eta_ratio * cL;
+      const double sx = fx * common, sy = fy * common, sz = fz * common;
+      vx += gx_x * sx + gy_x * sy + gz_x * sz;
+      vy += gx_y * sx + gy_y * sy + gz_y * sz;
+      vz += gx_z * sx + gy_z * sy + gz_z * sz;
+    }
+  }
+  if (c.enabled) {
+    vx += c.self_coeff * F[3 * (size_t)i];
+    vy += c.self_coeff * F[3 * (size_t)i + 1];
+    vz += c.self_coeff * F[3 * (size_t)i + 2];
+  }
+  V[3 * (size_t)i] = vx;
+  V[3 * (size_t)i + 1] = vy;
+  V[3 * (size_t)i + 2] = vz;
+}
+
+// ---------------------------------------------------------------- host drivers
+
+int check_common(const double* pos, const double* radii, int n, const dem_phys* ph) {
+  if (n < 0) return dem::set_error(DEM_ERR_INVALID, "n must be >= 0 (got %d)", n);
+  if (!ph) return dem::set_error(DEM_ERR_INVALID, "phys is NULL");
+  if (n > 0 && (!pos || !radii)) return dem::set_error(DEM_ERR_INVALID, "null positions/radii");
+  if (ph->periodic && !(ph->L > 0.0))
+    return dem::set_error(DEM_ERR_INVALID, "periodic box needs L > 0 (got %g)", ph->L);
+  if (n > (1 << 28)) return dem::set_error(DEM_ERR_CAPACITY, "n too large (%d)", n);
+  return DEM_OK;
+}
+
+int check_corr(const dem_correction* c) {
+  if (!c || !c->enabled) return DEM_OK;
+  if (!c->grid) return dem::set_error(DEM_ERR_INVALID, "correction enabled without a float grid");
+  if (c->n < 2) return dem::set_error(DEM_ERR_INVALID, "correction grid needs n >= 2");
+  return DEM_OK;
+}
+
+// Set-up + pack (shared by all FP32 entry points).
+int prepare(const Layout& l, const double* pos, const double* radii, const double* forces, int n,
+            const dem_phys* ph, hipStream_t st) {
+  int nblk = 1;
+  if (!ph->periodic) {
+    nblk = (n + 255) / 256;
+    if (nblk > kBBoxBlocks) nblk = kBBoxBlocks;
+    if (nblk < 1) nblk = 1;
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nblk), dim3(256), 0, st, pos, n, l.bbox);
+    DEM_CHECK_LAUNCH("k_bbox_partial");
+  }
+  hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, l.bbox, nblk, ph->L, ph->periodic,
+                     ph->m_const, l.sc);
+  DEM_CHECK_LAUNCH("k_setup");
+  hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, st, pos, radii, forces, n, l.sc,
+                     l.rec);
+  DEM_CHECK_LAUNCH("k_pack");
+  return DEM_OK;
+}
+
+int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, double* F,
+               hipStream_t st) {
+  const int S = choose_split(n);
+  const int jchunk = (n + S - 1) / S;
+  dim3 grid((n + WG - 1) / WG, S);
+  DEM_PROF_LAUNCH("k_force_tile", st,
+                  hipLaunchKernelGGL(k_force_tile, grid, dim3(WG), 0, st, (const Rec*)l.rec, n,
+                                     jchunk, choose_tile(), l.part));
+  DEM_CHECK_LAUNCH("k_force_tile");
+  hipLaunchKernelGGL(k_force_reduce, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)l.part,
+                     S, n, radii, (const DevScales*)l.sc, ph->m_const, F, l.rec);
+  DEM_CHECK_LAUNCH("k_force_reduce");
+  return DEM_OK;
+}
+
+int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* corr,
+             const double* F, double* V, hipStream_t st) {
+  const int S = choose_split(n);
+  const int jchunk = (n + S - 1) / S;
+  dim3 grid((n + WG - 1) / WG, S);
+  CorrArgs ca{};
+  CorrScalars cs{};
+  const bool on = corr && corr->enabled;
+  if (on) {
+    ca.grid = reinterpret_cast<const float4*>(corr->grid);
+    ca.n = corr->n;
+    ca.nmax = (float)(corr->n - 1);
+    ca.gmin = corr->grid_min;
+    ca.inv_dx = corr->inv_dx;
+    ca.L_ratio = corr->L_ratio;
+    cs.enabled = 1;
+    cs.common = corr->Fz_inv * corr->eta_ratio * corr->L_ratio;
+    cs.self_coeff = corr->self_coeff;
+    DEM_PROF_LAUNCH("k_conv_tile_corr", st,
+                    hipLaunchKernelGGL(k_conv_tile<true>, grid, dim3(WG), 0, st,
+                                       (const Rec*)l.rec, n, jchunk, choose_tile(),
+                                       (const DevScales*)l.sc, ca, l.part));
+  } else {
+    DEM_PROF_LAUNCH("k_conv_tile", st,
+                    hipLaunchKernelGGL(k_conv_tile<false>, grid, dim3(WG), 0, st,
+                                       (const Rec*)l.rec, n, jchunk, choose_tile(),
+                                       (const DevScales*)l.sc, ca, l.part));
+  }
+  DEM_CHECK_LAUNCH("k_conv_tile");
+  hipLaunchKernelGGL(k_conv_reduce, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)l.part,
+                     S, n, (const DevScales*)l.sc, ph->eta_const, cs, F, V);
+  DEM_CHECK_LAUNCH("k_conv_reduce");
+  return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dem_direct_workspace_bytes(int n) { return carve(nullptr, n < 0 ? 0 : n).bytes; }
+
+int dem_direct_set_tuning(int split, int tile) {
+  int prev = (g_split << 16) | (g_tile & 0xffff);
+  g_split = split > 0 ? split : 0;
+  g_tile = tile > 0 ? tile : 0;
+  return prev;
+}
+
+int dem_direct_forces(const double* pos, const double* radii, int n, const dem_phys* ph, double* F,
+                      void* ws, size_t ws_bytes, dem_stream_t stream) {
+  int rc = check_common(pos, radii, n, ph);
+  if (rc) return rc;
+  if (n > 0 && !F) return dem::set_error(DEM_ERR_INVALID, "null forces");
+  if (n == 0) return DEM_OK;
+  Layout l = carve(ws, n);
+  if (!ws || ws_bytes < l.bytes)
+    return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  hipStream_t st = dem::as_stream(stream);
+  if ((rc = prepare(l, pos, radii, nullptr, n, ph, st))) return rc;
+  return run_forces(l, radii, n, ph, F, st);
+}
+
+int dem_direct_convection(const double* pos, const double* radii, const double* F, int n,
+                          const dem_phys* ph, const dem_correction* corr, double* V, void* ws,
+                          size_t ws_bytes, dem_stream_t stream) {
+  int rc = check_common(pos, radii, n, ph);
+  if (rc) return rc;
+  if ((rc = check_corr(corr))) return rc;
+  if (n > 0 && (!F || !V)) return dem::set_error(DEM_ERR_INVALID, "null forces/velocities");
+  if (n == 0) return DEM_OK;
+  Layout l = carve(ws, n);
+  if (!ws || ws_bytes < l.bytes)
+    return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  hipStream_t st = dem::as_stream(stream);
+  if ((rc = prepare(l, pos, radii, F, n, ph, st))) return rc;
+  return run_conv(l, n, ph, corr, F, V, st);
+}
+
+int dem_direct_forces_and_convection(const double* pos, const double* radii, int n,
+                                     const dem_phys* ph, const dem_correction* corr, double* F,
+                                     double* V, void* ws, size_t ws_bytes, dem_stream_t stream) {
+  int rc = check_common(pos, radii, n, ph);
+  if (rc) return rc;
+  if ((rc = check_corr(corr))) return rc;
+  if (n > 0 && (!F || !V)) return dem::set_error(DEM_ERR_INVALID, "null forces/velocities");
+  if (n == 0) return DEM_OK;
+  Layout l = carve(ws, n);
+  if (!ws || ws_bytes < l.bytes)
+    return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  hipStream_t st = dem::as_stream(stream);
+  if ((rc = prepare(l, pos, radii, nullptr, n, ph, st))) return rc;
+  if ((rc = run_forces(l, radii, n, ph, F, st))) return rc;
+  return run_conv(l, n, ph, corr, F, V, st);
+}
+
+int dem_direct_forces_and_convection_f64(const double* pos, const double* radii, int n,
+                                         const dem_phys* ph, const dem_correction* corr,
+                                         double* F, double* V, dem_stream_t stream) {
+  int rc = check_common(pos, radii, n, ph);
+  if (rc) return rc;
+  if (n > 0 && (!F || !V)) return dem::set_error(DEM_ERR_INVALID, "null forces/velocities");
+  Corr64Args c{};
+  if (corr && corr->enabled) {
+    if (!corr->grid64 || corr->n < 2)
+      return dem::set_error(DEM_ERR_INVALID, "FP64 oracle needs the float64 correction grid");
+    c.grid = corr->grid64;
+    c.n = corr->n;
+    c.enabled = 1;
+    c.gmin = corr->grid_min;
+    c.inv_dx = corr->inv_dx;
+    c.Fz_inv = corr->Fz_inv;
+    c.L_ratio = corr->L_ratio;
+    c.eta_ratio = corr->eta_ratio;
+    c.self_coeff = corr->self_coeff;
+  }
+  if (n == 0) return DEM_OK;
+  hipStream_t st = dem::as_stream(stream);
+  hipLaunchKernelGGL(k_f64_forces, dim3((n + 255) / 256), dim3(256), 0, st, pos, radii, n,
+                     ph->m_const, ph->L, ph->periodic, F);
+  DEM_CHECK_LAUNCH("k_f64_forces");
+  hipLaunchKernelGGL(k_f64_conv, dim3((n + 255) / 256), dim3(256), 0, st, pos, (const double*)F, n,
+                     ph->eta_const, ph->L, ph->periodic, c, V);
+  DEM_CHECK_LAUNCH("k_f64_conv");
+  return DEM_OK;
+}
+
+}  // extern "C"

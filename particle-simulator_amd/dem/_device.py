@@ -1,0 +1,75 @@
+"""Device plumbing: PyTorch ROCm tensors as the buffers handed to libdemhip.
+
+PyTorch only provides memory and the stream here; every computation on the
+hot path is a libdemhip kernel.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+_checked = False
+
+
+def torch():
+    import torch as _t
+
+    return _t
+
+
+def device():
+    """The current ROCm device; raises if none is visible (no CPU fallback)."""
+    global _checked
+    t = torch()
+    if not _checked:
+        _native.load()
+        if not t.cuda.is_available():
+            raise RuntimeError(
+                "dem: no ROCm GPU is visible; the DEM operators run only on the HIP path"
+            )
+        _checked = True
+    return t.device("cuda", t.cuda.current_device())
+
+
+def stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def to_device(a, dtype=np.float64):
+    """numpy (or tensor) -> contiguous device tensor of `dtype`."""
+    t = torch()
+    if isinstance(a, t.Tensor):
+        tt = {np.float64: t.float64, np.float32: t.float32, np.int32: t.int32}[dtype]
+        return a.to(device=device(), dtype=tt).contiguous()
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return t.from_numpy(arr).to(device())
+
+
+def empty(shape, dtype="float64"):
+    t = torch()
+    return t.empty(shape, dtype=getattr(t, dtype), device=device())
+
+
+class Workspace:
+    """A grow-only device scratch buffer (uint8)."""
+
+    def __init__(self):
+        self._buf = None
+
+    def get(self, nbytes: int):
+        t = torch()
+        nbytes = max(int(nbytes), 256)
+        if self._buf is None or self._buf.numel() < nbytes:
+            self._buf = t.empty(nbytes, dtype=t.uint8, device=device())
+        return self._buf
+
+    def release(self):
+        self._buf = None

@@ -1,0 +1,116 @@
+"""Seeded synthetic droplet systems for the benchmark configurations.
+
+Every system: radii r = clip(5 um * exp(0.3 z), 2 um, 20 um) with
+z ~ N(0, 1) (log-normal, median 5 um, sigma 0.3); a periodic cube of edge
+L = (sum 4/3 pi r^3 / phi)^(1/3) for volume fraction phi; positions uniform in
+[0, L)^3 with overlapping droplets (minimum-image distance < r_i + r_j)
+re-drawn until none overlap.  Physical constants: eps_oil = 2.85,
+eta_oil = 0.05 Pa s, eta_water = 0.001 Pa s, E = 3e5 V/m along z unless an
+ensemble draws its own.  Everything is drawn from numpy.random.default_rng.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+EPS0 = 8.85418781762039e-12
+
+
+@dataclass
+class System:
+    positions: np.ndarray  # (N,3) float64, metres
+    radii: np.ndarray      # (N,)  float64, metres
+    L: float
+    E: float = 3e5
+    eps_oil: float = 2.85
+    eta_oil: float = 0.05
+    eta_water: float = 0.001
+    phi: float = 0.02
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.radii.shape[0])
+
+    @property
+    def m_const(self) -> float:
+        return 12 * np.pi * EPS0 * self.eps_oil * self.E**2
+
+    @property
+    def eta_const(self) -> float:
+        return 1 / (8 * np.pi * self.eta_oil)
+
+    @property
+    def stokes_factor(self) -> float:
+        eo, ew = self.eta_oil, self.eta_water
+        return 2 * np.pi * eo * (2 * eo + 3 * ew) / (eo + ew)
+
+
+def lognormal_radii(rng: np.random.Generator, n: int) -> np.ndarray:
+    z = rng.standard_normal(n)
+    return np.clip(5e-6 * np.exp(0.3 * z), 2e-6, 20e-6)
+
+
+def box_edge(radii: np.ndarray, phi: float) -> float:
+    return float((np.sum(4.0 / 3.0 * np.pi * radii**3) / phi) ** (1.0 / 3.0))
+
+
+def place_without_overlap(rng: np.random.Generator, radii: np.ndarray, L: float,
+                          max_rounds: int = 10000) -> np.ndarray:
+    """Uniform positions in [0,L)^3, re-drawing droplets that overlap under the
+    minimum-image convention."""
+    from scipy.spatial import cKDTree
+
+    n = radii.shape[0]
+    pos = rng.uniform(0.0, L, size=(n, 3))
+    reach = 2.0 * float(np.max(radii))
+    for _ in range(max_rounds):
+        tree = cKDTree(pos, boxsize=L)
+        pairs = tree.query_pairs(reach, output_type="ndarray")
+        if len(pairs) == 0:
+            return pos
+        d = pos[pairs[:, 1]] - pos[pairs[:, 0]]
+        d -= L * np.round(d / L)
+        dist = np.sqrt(np.sum(d * d, axis=1))
+        bad = pairs[dist < radii[pairs[:, 0]] + radii[pairs[:, 1]]]
+        if len(bad) == 0:
+            return pos
+        redo = np.unique(bad[:, 1])
+        pos[redo] = rng.uniform(0.0, L, size=(len(redo), 3))
+        pos %= L
+    raise ValueError("could not place droplets without overlap")
+
+
+def make_system(n: int, seed: int = 0, phi: float = 0.02, E: float = 3e5) -> System:
+    rng = np.random.default_rng(seed)
+    radii = lognormal_radii(rng, n)
+    L = box_edge(radii, phi)
+    pos = place_without_overlap(rng, radii, L)
+    return System(positions=pos, radii=radii, L=L, E=E, phi=phi, seed=seed)
+
+
+def config(index: int) -> System:
+    """The single-system configurations: 0 (N=2000), 1 (N=65536), 2 (N=2^20), 4 (N=4096)."""
+    sizes = {0: 2000, 1: 65536, 2: 1 << 20, 4: 4096}
+    if index not in sizes:
+        raise ValueError(f"no single-system configuration {index}")
+    s = make_system(sizes[index], seed=0, phi=0.02)
+    s.meta["config"] = index
+    return s
+
+
+def ensemble_specs(count: int = 256, base_seed: int = 0):
+    """(N_e, E_e, phi_e) of the batched-ensemble configuration."""
+    rng = np.random.default_rng(base_seed)
+    ns = rng.integers(500, 20001, size=count)
+    Es = rng.uniform(1e5, 5e5, size=count)
+    phis = rng.uniform(0.01, 0.05, size=count)
+    return [(int(ns[e]), float(Es[e]), float(phis[e])) for e in range(count)]
+
+
+def ensembles(count: int = 256, base_seed: int = 0) -> list[System]:
+    return [make_system(n, seed=e, phi=phi, E=E)
+            for e, (n, E, phi) in enumerate(ensemble_specs(count, base_seed))]

@@ -1,0 +1,49 @@
+"""Writes the committed golden fixtures from the FP64 oracle.
+
+direct_n2000.npz — the N=2000, seed-0, phi=0.02 periodic system (the
+    CPU-runnable configuration): positions, radii, constants, FP64 oracle
+    forces and Oseen velocities.
+hash_n2000.npz   — the same positions with radii inflated 2.2x (so that
+    contacts exist) and the oracle's unique sorted contact pairs for a table
+    of next_pow2(2*2000) buckets, plus the brute-force contact set.
+Run from the repository root: ``python tests/golden/make_golden.py``.
+"""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path[:0] = [str(ROOT), str(ROOT / "particle-simulator_amd")]
+
+import oracle  # noqa: E402
+from dem.workloads import make_system  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def main() -> None:
+    s = make_system(2000, seed=0, phi=0.02)
+    m, eta = oracle.m_const(s.eps_oil, s.E), oracle.eta_const(s.eta_oil)
+    F = oracle.direct_forces(s.positions, s.radii, m, s.L, True)
+    V = oracle.direct_convection(s.positions, F, eta, s.L, True)
+    np.savez_compressed(OUT / "direct_n2000.npz", positions=s.positions, radii=s.radii, L=s.L,
+                        E=s.E, eps_oil=s.eps_oil, eta_oil=s.eta_oil, eta_water=s.eta_water,
+                        forces=F, velocities=V)
+    r = s.radii * 2.2
+    table = 1
+    while table < 2 * s.n:
+        table <<= 1
+    _, pairs = oracle.hash_pairs(s.positions, r, s.L, True, table)
+    brute = oracle.brute_pairs(s.positions, r, s.L, True)
+    np.savez_compressed(OUT / "hash_n2000.npz", positions=s.positions, radii=r, L=s.L,
+                        table_size=table, pairs=pairs, brute=brute)
+    print(f"direct: |F|rms={np.sqrt((F**2).sum(1).mean()):.3e}  hash: {len(pairs)} pairs "
+          f"({len(brute)} by brute force)")
+
+
+if __name__ == "__main__":
+    main()

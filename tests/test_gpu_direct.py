@@ -1,0 +1,173 @@
+"""GPU parity of the direct force + Oseen kernels (libdemhip
+``dem_direct_*``) against the FP64 oracle and the committed golden fixture.
+
+Tolerance (the build's accuracy contract for FP32 compute): per-particle
+relative error e_i = |x_i - ref_i| / max(|ref_i|, 1e-3 * rms|ref|) must stay
+below 1e-5 for forces and velocities, against FP64 on the same float64
+positions.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from dem.force_calculator import DirectDropletForceCalculator
+from dem.workloads import make_system
+
+from ._errors import summary
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+FLOOR = 1e-3
+GOLDEN = Path(__file__).parent / "golden" / "direct_n2000.npz"
+
+
+def _calc(s, boundary="periodic", n=None, corr=0):
+    return DirectDropletForceCalculator(num_particles=n or s.n, eps_oil=s.eps_oil,
+                                        eta_oil=s.eta_oil, eta_water=s.eta_water, E=s.E, L=s.L,
+                                        boundary_mode=boundary, correction_grid_resolution=corr)
+
+
+# --- the reference's own tests, on the GPU operator (tests/force_calculator/...:22-95)
+
+
+def test_single_particle_zero_force(gpu):
+    fc = DirectDropletForceCalculator(num_particles=4, L=1.0, boundary_mode="open")
+    forces = fc.calculate(np.array([[0.5, 0.5, 0.5]]), np.array([1e-5]))
+    assert forces.shape == (1, 3)
+    np.testing.assert_allclose(forces[0], np.zeros(3))
+
+
+def test_pair_newton_third_law(gpu):
+    fc = DirectDropletForceCalculator(num_particles=4, L=1.0, boundary_mode="open")
+    pos = np.array([[0.40, 0.50, 0.50], [0.60, 0.50, 0.50]])
+    forces = fc.calculate(pos, np.array([2e-5, 2e-5]))
+    np.testing.assert_allclose(forces[0], -forces[1], rtol=1e-10)
+    # and the closed form F_x = -m r^6 / d^4 (repulsion across the field)
+    np.testing.assert_allclose(forces[0, 0], -fc.m_const * (2e-5) ** 6 / 0.2**4, rtol=1e-5)
+
+
+def test_returns_correct_shape(gpu):
+    fc = DirectDropletForceCalculator(num_particles=8, L=1.0, boundary_mode="open")
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(0.1, 0.9, size=(5, 3))
+    r = rng.uniform(1e-6, 5e-6, size=5)
+    assert fc.calculate(pos, r).shape == (5, 3)
+    F, V = fc.calculate_forces_and_convection(pos, r)
+    assert F.shape == (5, 3) and V.shape == (5, 3)
+
+
+def test_periodic_changes_force_vs_open(gpu):
+    pos = np.array([[0.05, 0.50, 0.50], [0.95, 0.50, 0.50]])
+    r = np.array([1e-5, 1e-5])
+    f_open = DirectDropletForceCalculator(num_particles=4, L=1.0, boundary_mode="open").calculate(pos, r)
+    f_per = DirectDropletForceCalculator(num_particles=4, L=1.0, boundary_mode="periodic").calculate(pos, r)
+    assert f_open[0, 0] * f_per[0, 0] < 0
+
+
+# --- parity against the FP64 oracle
+
+
+@pytest.mark.parametrize("boundary", ["periodic", "open"])
+def test_forces_and_convection_match_oracle_n2000(gpu, boundary):
+    s = make_system(2000, seed=0, phi=0.02)
+    per = boundary == "periodic"
+    fc = _calc(s, boundary)
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    Fo = oracle.direct_forces(s.positions, s.radii, fc.m_const, s.L, per)
+    Vo = oracle.direct_convection(s.positions, Fo, fc.eta_const, s.L, per)
+    ef, ev = summary(F, Fo, FLOOR), summary(V, Vo, FLOOR)
+    print(f"\n[{boundary}] force err {ef}\n[{boundary}] conv  err {ev}")
+    assert ef["max"] < TOL and ev["max"] < TOL
+
+
+def test_convection_from_caller_forces(gpu):
+    s = make_system(1500, seed=3, phi=0.03)
+    fc = _calc(s)
+    rng = np.random.default_rng(1)
+    F = rng.normal(size=(s.n, 3)) * 1e-12
+    V = fc.calculate_convection(s.positions, s.radii, F)
+    Vo = oracle.direct_convection(s.positions, F, fc.eta_const, s.L, True)
+    assert summary(V, Vo, FLOOR)["max"] < TOL
+
+
+def test_golden_fixture(gpu):
+    g = np.load(GOLDEN)
+    fc = DirectDropletForceCalculator(num_particles=len(g["radii"]), eps_oil=float(g["eps_oil"]),
+                                      eta_oil=float(g["eta_oil"]), E=float(g["E"]),
+                                      L=float(g["L"]), boundary_mode="periodic")
+    F, V = fc.calculate_forces_and_convection(g["positions"], g["radii"])
+    assert summary(F, g["forces"], FLOOR)["max"] < TOL
+    assert summary(V, g["velocities"], FLOOR)["max"] < TOL
+
+
+@pytest.fixture(scope="module")
+def correction():
+    from dem.periodic_correction import COMSOLLatticeCorrection
+
+    return COMSOLLatticeCorrection.load_default()
+
+
+def test_comsol_correction_matches_oracle(gpu, correction):
+    s = make_system(1200, seed=5, phi=0.02)
+    fc = _calc(s, corr=correction.grid_resolution)
+    fc.load_periodic_correction(correction, L_sim=s.L)
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    gd = correction.get_grid_data()
+    c = oracle.Correction(gd["grid_u"], gd["grid_v"], gd["grid_w"], gd["grid_min"], gd["grid_dx"],
+                          gd["Fz_comsol"], gd["L_comsol"] / s.L, gd["eta_comsol"] / s.eta_oil,
+                          fc._corr.self_coeff)
+    Fo = oracle.direct_forces(s.positions, s.radii, fc.m_const, s.L, True)
+    Vo = oracle.direct_convection(s.positions, Fo, fc.eta_const, s.L, True, c)
+    Vn = oracle.direct_convection(s.positions, Fo, fc.eta_const, s.L, True)
+    e = summary(V, Vo, FLOOR)
+    print(f"\n[comsol] conv err {e}; correction/total = "
+          f"{np.linalg.norm(Vo - Vn) / np.linalg.norm(Vo):.3e}")
+    assert e["max"] < TOL
+    # the FP64 GPU oracle kernel agrees with the CPU oracle
+    import torch
+
+    from dem import _device
+
+    F64, V64 = fc.forces_and_convection_f64_device(_device.to_device(s.positions),
+                                                   _device.to_device(s.radii))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(F64.cpu().numpy(), Fo, rtol=1e-9, atol=1e-12 * np.abs(Fo).max())
+    np.testing.assert_allclose(V64.cpu().numpy(), Vo, rtol=1e-9, atol=1e-12 * np.abs(Vo).max())
+
+
+def test_fp64_oracle_kernel_matches_cpu_oracle(gpu):
+    s = make_system(1000, seed=2, phi=0.02)
+    fc = _calc(s)
+    from dem import _device
+
+    F64, V64 = fc.forces_and_convection_f64_device(_device.to_device(s.positions),
+                                                   _device.to_device(s.radii))
+    Fo = oracle.direct_forces(s.positions, s.radii, fc.m_const, s.L, True)
+    Vo = oracle.direct_convection(s.positions, Fo, fc.eta_const, s.L, True)
+    np.testing.assert_allclose(F64.cpu().numpy(), Fo, rtol=1e-10, atol=1e-13 * np.abs(Fo).max())
+    np.testing.assert_allclose(V64.cpu().numpy(), Vo, rtol=1e-10, atol=1e-13 * np.abs(Vo).max())
+
+
+def test_n65536_against_fp64_kernel(gpu):
+    """Full config-1 size: FP32 path vs the FP64 oracle kernel on the GPU."""
+    s = make_system(65536, seed=0, phi=0.02)
+    fc = _calc(s)
+    from dem import _device
+
+    pd, rd = _device.to_device(s.positions), _device.to_device(s.radii)
+    F, V = fc.forces_and_convection_device(pd, rd)
+    F64, V64 = fc.forces_and_convection_f64_device(pd, rd)
+    ef = summary(F.cpu().numpy(), F64.cpu().numpy(), FLOOR)
+    ev = summary(V.cpu().numpy(), V64.cpu().numpy(), FLOOR)
+    print(f"\n[N=65536] force err {ef}\n[N=65536] conv  err {ev}")
+    assert ef["max"] < TOL and ev["max"] < TOL
+    # spot-check the FP64 kernel against the CPU oracle on a slice of targets
+    Fo = oracle.direct_forces(s.positions, s.radii, fc.m_const, s.L, True, targets=(0, 256))
+    np.testing.assert_allclose(F64.cpu().numpy()[:256], Fo[:256], rtol=1e-9,
+                               atol=1e-12 * np.abs(Fo[:256]).max())

@@ -1,0 +1,60 @@
+"""CPU checks of the C ABI: libdemhip.so loads and exports every function
+declared in include/dem_hip.h; host-only entry points behave (no kernels are
+launched here)."""
+
+from __future__ import annotations
+
+import ctypes
+
+import pytest
+
+from dem import _native
+
+
+def test_library_loads_and_exports_header():
+    lib = _native.load()
+    names = _native.header_functions()
+    assert "dem_direct_forces_and_convection" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+    # every exported entry point also has a ctypes signature
+    assert set(names) <= set(_native.SIGNATURES)
+
+
+def test_version_and_error_string():
+    lib = _native.load()
+    assert lib.dem_version() >= 1
+    assert isinstance(_native.last_error(), str)
+
+
+def test_workspace_sizes_are_host_only_and_monotone():
+    lib = _native.load()
+    a = lib.dem_direct_workspace_bytes(1000)
+    b = lib.dem_direct_workspace_bytes(100000)
+    assert 0 < a < b and a % 256 == 0
+    assert lib.dem_hash_workspace_bytes(1000, 2048) < lib.dem_hash_workspace_bytes(1000, 1 << 20)
+
+
+def test_invalid_arguments_are_rejected_before_any_launch():
+    lib = _native.load()
+    ph = _native.DemPhys(1.0, 1.0, -1.0, 1, 0)  # periodic with L < 0
+    rc = lib.dem_direct_forces(ctypes.c_void_p(8), ctypes.c_void_p(8), 10, ctypes.byref(ph),
+                               ctypes.c_void_p(8), ctypes.c_void_p(8), 1 << 30, None)
+    assert rc == _native.DEM_ERR_INVALID
+    assert "L > 0" in _native.last_error()
+    rc = lib.dem_hash_pairs(None, None, 4, ctypes.c_double(1.0), 1, 3, None, 0,
+                            ctypes.c_void_p(8), None, 0, None)
+    assert rc == _native.DEM_ERR_INVALID  # table size not a power of two
+    ph = _native.DemPhys(1.0, 1.0, 1.0, 1, 0)
+    rc = lib.dem_direct_forces(ctypes.c_void_p(8), ctypes.c_void_p(8), 10, ctypes.byref(ph),
+                               ctypes.c_void_p(8), ctypes.c_void_p(8), 16, None)
+    assert rc == _native.DEM_ERR_WORKSPACE
+    with pytest.raises(_native.NativeError):
+        _native.call("dem_euler_step", None, None, None, None, 5, 1.0, 1.0, 1.0, 0, None)
+
+
+def test_zero_particles_is_a_no_op():
+    lib = _native.load()
+    ph = _native.DemPhys(1.0, 1.0, 1.0, 1, 0)
+    assert lib.dem_direct_forces(None, None, 0, ctypes.byref(ph), None, None, 0, None) == 0
+    assert lib.dem_euler_step(None, None, None, None, 0, 1.0, 1.0, 1.0, 1, None) == 0
