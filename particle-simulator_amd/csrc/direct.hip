@@ -83,7 +83,11 @@ inline int choose_split(int n) {
   return s;
 }
 
-inline int choose_tile() { return g_tile > 0 ? g_tile : 128; }
+// FP32 partial-sum length before the float64 hand-off.  The Oseen sum is far
+// worse conditioned than the force sum (1/r decay: sum|t|/|v| ~ 1e3-1e5 at
+// N=65536), so it hands off four times more often.
+inline int choose_tile_force() { return g_tile > 0 ? g_tile : 32; }
+inline int choose_tile_conv() { return g_tile > 0 ? g_tile : 16; }
 
 constexpr int kBBoxBlocks = 256;
 
@@ -593,7 +597,7 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
   dim3 grid((n + WG - 1) / WG, S);
   DEM_PROF_LAUNCH("k_force_tile", st,
                   hipLaunchKernelGGL(k_force_tile, grid, dim3(WG), 0, st, (const Rec*)l.rec, n,
-                                     jchunk, choose_tile(), l.part));
+                                     jchunk, choose_tile_force(), l.part));
   DEM_CHECK_LAUNCH("k_force_tile");
   hipLaunchKernelGGL(k_force_reduce, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)l.part,
                      S, n, radii, (const DevScales*)l.sc, ph->m_const, F, l.rec);
@@ -621,12 +625,12 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
     cs.self_coeff = corr->self_coeff;
     DEM_PROF_LAUNCH("k_conv_tile_corr", st,
                     hipLaunchKernelGGL(k_conv_tile<true>, grid, dim3(WG), 0, st,
-                                       (const Rec*)l.rec, n, jchunk, choose_tile(),
+                                       (const Rec*)l.rec, n, jchunk, choose_tile_conv(),
                                        (const DevScales*)l.sc, ca, l.part));
   } else {
     DEM_PROF_LAUNCH("k_conv_tile", st,
                     hipLaunchKernelGGL(k_conv_tile<false>, grid, dim3(WG), 0, st,
-                                       (const Rec*)l.rec, n, jchunk, choose_tile(),
+                                       (const Rec*)l.rec, n, jchunk, choose_tile_conv(),
                                        (const DevScales*)l.sc, ca, l.part));
   }
   DEM_CHECK_LAUNCH("k_conv_tile");

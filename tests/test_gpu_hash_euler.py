@@ -129,14 +129,14 @@ def test_euler_step_bit_exact(gpu, periodic):
     sf = oracle.stokes_factor(0.05, 0.001)
     dt = 0.013
     ref = oracle.euler_step(pos, F, conv, r, sf, dt, L, periodic)
-    pd = _device.to_device(pos)
-    _native.call("dem_euler_step", _device.ptr(pd), _device.ptr(_device.to_device(F)),
-                 _device.ptr(_device.to_device(conv)), _device.ptr(_device.to_device(r)), n,
-                 sf, dt, L, int(periodic), _device.stream())
+    # keep every device tensor referenced until the kernel has run
+    pd, Fd, cd, rd = (_device.to_device(a) for a in (pos, F, conv, r))
+    _native.call("dem_euler_step", _device.ptr(pd), _device.ptr(Fd), _device.ptr(cd),
+                 _device.ptr(rd), n, sf, dt, L, int(periodic), _device.stream())
     assert np.array_equal(pd.cpu().numpy(), ref)
     v = rng.normal(size=(n, 3)) * 1e-4
     ref2 = oracle.euler_step_total(pos, v, dt, L, periodic)
-    pd = _device.to_device(pos)
-    _native.call("dem_euler_step_total", _device.ptr(pd), _device.ptr(_device.to_device(v)), n,
-                 dt, L, int(periodic), _device.stream())
+    pd, vd = _device.to_device(pos), _device.to_device(v)
+    _native.call("dem_euler_step_total", _device.ptr(pd), _device.ptr(vd), n, dt, L,
+                 int(periodic), _device.stream())
     assert np.array_equal(pd.cpu().numpy(), ref2)
