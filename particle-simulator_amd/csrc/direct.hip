@@ -42,14 +42,28 @@ constexpr double kSigma = 0x1p-22;   // work unit / fixed unit
 constexpr float kSigma2f = 0x1p-44f; // kSigma^2
 constexpr double kSigma3 = 0x1p-66;
 
-// Source / target record: 32 bytes, read whole by the pair kernels.
+// Phase-1 source / target record (32 bytes, read whole by the force kernel):
+// 32-bit fixed-point position (rounded), radius and radius^3 in work units.
 struct alignas(16) Rec {
-  uint32_t x, y, z;  // fixed-point position
-  float r;           // radius in work units
-  float r3;          // radius^3 in work units
-  float fx, fy, fz;  // force / Fs (phase-2 sources)
+  uint32_t x, y, z;
+  float r;
+  float r3;
+  uint32_t pad[3];
 };
 static_assert(sizeof(Rec) == 32, "record layout");
+
+// Phase-2 record (48 bytes): 64-bit fixed-point position split into high and
+// low words, and the source force in units of Fs.  The Oseen sum decays as
+// 1/r, so a pair whose separation lies within one 32-bit quantum of L/2 would
+// change the velocity visibly if it took the other periodic image; 64-bit
+// positions make the image choice agree with the float64 rule |d| > L/2.
+struct alignas(16) RecC {
+  uint32_t xh, yh, zh;
+  float fx, fy, fz;
+  uint32_t xl, yl, zl;
+  uint32_t pad[3];
+};
+static_assert(sizeof(RecC) == 48, "record layout");
 
 // Per-call scales, computed on the device so open-boundary boxes need no host sync.
 struct DevScales {
@@ -95,6 +109,7 @@ struct Layout {
   DevScales* sc;
   double* bbox;  // [kBBoxBlocks][6]
   Rec* rec;
+  RecC* recc;
   double* part;  // [split][6][n]
   size_t bytes;
 };
@@ -105,6 +120,7 @@ Layout carve(void* ws, int n) {
   l.sc = c.take<DevScales>(1);
   l.bbox = c.take<double>(kBBoxBlocks * 6);
   l.rec = c.take<Rec>((size_t)(n > 0 ? n : 1));
+  l.recc = c.take<RecC>((size_t)(n > 0 ? n : 1));
   l.part = c.take<double>((size_t)split_max(n) * 6 * (size_t)(n > 0 ? n : 1));
   l.bytes = dem::align_up(c.used, 256);
   return l;
@@ -180,36 +196,50 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
                                               const double* __restrict__ radii,
                                               const double* __restrict__ forces, int n,
                                               const DevScales* __restrict__ scp,
-                                              Rec* __restrict__ rec) {
+                                              Rec* __restrict__ rec, RecC* __restrict__ recc) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const DevScales sc = *scp;
-  uint32_t q[3];
+  unsigned long long q[3];
   for (int a = 0; a < 3; ++a) {
     double x = pos[3 * (size_t)i + a];
+    double v;
     if (sc.periodic) {
       double u = x * sc.inv_U * 0x1p-32;  // x / L
       u -= floor(u);
-      q[a] = (uint32_t)(unsigned long long)rint(u * kTwo32);
+      v = rint(u * 0x1p64);
+      if (v >= 0x1p64) v = 0.0;  // u rounded up to 1: wraps to the origin
     } else {
-      q[a] = (uint32_t)(long long)rint((x - sc.origin[a]) * sc.inv_U);
+      v = rint((x - sc.origin[a]) * sc.inv_U * kTwo32);
     }
+    q[a] = (unsigned long long)v;
   }
   double rl = radii[i] / sc.lambda;
   Rec o;
-  o.x = q[0];
-  o.y = q[1];
-  o.z = q[2];
+  // phase-1 position: the 64-bit value rounded to its high word
+  o.x = (uint32_t)((q[0] + 0x80000000ull) >> 32);
+  o.y = (uint32_t)((q[1] + 0x80000000ull) >> 32);
+  o.z = (uint32_t)((q[2] + 0x80000000ull) >> 32);
   o.r = (float)rl;
   o.r3 = (float)(rl * rl * rl);
-  if (forces) {
-    o.fx = (float)(forces[3 * (size_t)i + 0] / sc.Fs);
-    o.fy = (float)(forces[3 * (size_t)i + 1] / sc.Fs);
-    o.fz = (float)(forces[3 * (size_t)i + 2] / sc.Fs);
-  } else {
-    o.fx = o.fy = o.fz = 0.f;
-  }
+  o.pad[0] = o.pad[1] = o.pad[2] = 0u;
   rec[i] = o;
+  RecC c;
+  c.xh = (uint32_t)(q[0] >> 32);
+  c.yh = (uint32_t)(q[1] >> 32);
+  c.zh = (uint32_t)(q[2] >> 32);
+  c.xl = (uint32_t)q[0];
+  c.yl = (uint32_t)q[1];
+  c.zl = (uint32_t)q[2];
+  if (forces) {
+    c.fx = (float)(forces[3 * (size_t)i + 0] / sc.Fs);
+    c.fy = (float)(forces[3 * (size_t)i + 1] / sc.Fs);
+    c.fz = (float)(forces[3 * (size_t)i + 2] / sc.Fs);
+  } else {
+    c.fx = c.fy = c.fz = 0.f;
+  }
+  c.pad[0] = c.pad[1] = c.pad[2] = 0u;
+  recc[i] = c;
 }
 
 // ---------------------------------------------------------------- phase 1
@@ -263,7 +293,7 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
                                                       const double* __restrict__ radii,
                                                       const DevScales* __restrict__ scp,
                                                       double m_const, double* __restrict__ F,
-                                                      Rec* __restrict__ rec) {
+                                                      RecC* __restrict__ recc) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double sx = 0.0, sy = 0.0, sz = 0.0;
@@ -279,9 +309,9 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
   F[3 * (size_t)i + 0] = Fx;
   F[3 * (size_t)i + 1] = Fy;
   F[3 * (size_t)i + 2] = Fz;
-  rec[i].fx = (float)(Fx / sc.Fs);
-  rec[i].fy = (float)(Fy / sc.Fs);
-  rec[i].fz = (float)(Fz / sc.Fs);
+  recc[i].fx = (float)(Fx / sc.Fs);
+  recc[i].fy = (float)(Fy / sc.Fs);
+  recc[i].fz = (float)(Fz / sc.Fs);
 }
 
 // ---------------------------------------------------------------- phase 2
@@ -301,7 +331,7 @@ __device__ __forceinline__ void grid_coord(float g, float nmax, int n, int& i, f
 }
 
 template <bool CORR>
-__global__ __launch_bounds__(WG) void k_conv_tile(const Rec* __restrict__ rec, int n, int jchunk,
+__global__ __launch_bounds__(WG) void k_conv_tile(const RecC* __restrict__ rec, int n, int jchunk,
                                                   int tile, const DevScales* __restrict__ scp,
                                                   CorrArgs ca, double* __restrict__ part) {
   const int i = blockIdx.x * WG + threadIdx.x;
@@ -309,7 +339,9 @@ __global__ __launch_bounds__(WG) void k_conv_tile(const Rec* __restrict__ rec, i
   const int s = blockIdx.y;
   const int j0 = s * jchunk;
   const int j1 = min(n, j0 + jchunk);
-  const uint32_t xi = rec[ii].x, yi = rec[ii].y, zi = rec[ii].z;
+  const unsigned long long xi = ((unsigned long long)rec[ii].xh << 32) | rec[ii].xl;
+  const unsigned long long yi = ((unsigned long long)rec[ii].yh << 32) | rec[ii].yl;
+  const unsigned long long zi = ((unsigned long long)rec[ii].zh << 32) | rec[ii].zl;
   // Grid coordinate of x_i - x_j = -U D  is  a * D + b.
   float ga = 0.f, gb = 0.f;
   if (CORR) {
@@ -325,10 +357,11 @@ __global__ __launch_bounds__(WG) void k_conv_tile(const Rec* __restrict__ rec, i
     float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll 4
     for (int j = jt; j < je; ++j) {
-      const Rec sj = rec[j];
-      const float dx = (float)(int32_t)(sj.x - xi);
-      const float dy = (float)(int32_t)(sj.y - yi);
-      const float dz = (float)(int32_t)(sj.z - zi);
+      const RecC sj = rec[j];
+      // 64-bit minimum-image difference; its high word (floor) in fixed units
+      const float dx = (float)(int32_t)(((((unsigned long long)sj.xh << 32) | sj.xl) - xi) >> 32);
+      const float dy = (float)(int32_t)(((((unsigned long long)sj.yh << 32) | sj.yl) - yi) >> 32);
+      const float dz = (float)(int32_t)(((((unsigned long long)sj.zh << 32) | sj.zl) - zi) >> 32);
       const float r2 = dx * dx + dy * dy + dz * dz;  // fixed units^2
       const bool other = r2 > 0.f;                     // drops j == i
       const float ir = other ? __builtin_amdgcn_rsqf(r2) : 0.f;
@@ -585,7 +618,7 @@ int prepare(const Layout& l, const double* pos, const double* radii, const doubl
                      ph->m_const, l.sc);
   DEM_CHECK_LAUNCH("k_setup");
   hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, st, pos, radii, forces, n, l.sc,
-                     l.rec);
+                     l.rec, l.recc);
   DEM_CHECK_LAUNCH("k_pack");
   return DEM_OK;
 }
@@ -600,7 +633,7 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
                                      jchunk, choose_tile_force(), l.part));
   DEM_CHECK_LAUNCH("k_force_tile");
   hipLaunchKernelGGL(k_force_reduce, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)l.part,
-                     S, n, radii, (const DevScales*)l.sc, ph->m_const, F, l.rec);
+                     S, n, radii, (const DevScales*)l.sc, ph->m_const, F, l.recc);
   DEM_CHECK_LAUNCH("k_force_reduce");
   return DEM_OK;
 }
@@ -625,12 +658,12 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
     cs.self_coeff = corr->self_coeff;
     DEM_PROF_LAUNCH("k_conv_tile_corr", st,
                     hipLaunchKernelGGL(k_conv_tile<true>, grid, dim3(WG), 0, st,
-                                       (const Rec*)l.rec, n, jchunk, choose_tile_conv(),
+                                       (const RecC*)l.recc, n, jchunk, choose_tile_conv(),
                                        (const DevScales*)l.sc, ca, l.part));
   } else {
     DEM_PROF_LAUNCH("k_conv_tile", st,
                     hipLaunchKernelGGL(k_conv_tile<false>, grid, dim3(WG), 0, st,
-                                       (const Rec*)l.rec, n, jchunk, choose_tile_conv(),
+                                       (const RecC*)l.recc, n, jchunk, choose_tile_conv(),
                                        (const DevScales*)l.sc, ca, l.part));
   }
   DEM_CHECK_LAUNCH("k_conv_tile");
