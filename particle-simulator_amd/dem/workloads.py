@@ -1,4 +1,4 @@
-"""Seeded synthetic droplet systems for the benchmark configurations.
+"""Seeded synthetic droplet systems for benchmarks and parity tests.
 
 Every system: radii r = clip(5 um * exp(0.3 z), 2 um, 20 um) with
 z ~ N(0, 1) (log-normal, median 5 um, sigma 0.3); a periodic cube of edge
@@ -92,18 +92,29 @@ def make_system(n: int, seed: int = 0, phi: float = 0.02, E: float = 3e5) -> Sys
     return System(positions=pos, radii=radii, L=L, E=E, phi=phi, seed=seed)
 
 
-def config(index: int) -> System:
-    """The single-system configurations: 0 (N=2000), 1 (N=65536), 2 (N=2^20), 4 (N=4096)."""
-    sizes = {0: 2000, 1: 65536, 2: 1 << 20, 4: 4096}
-    if index not in sizes:
-        raise ValueError(f"no single-system configuration {index}")
-    s = make_system(sizes[index], seed=0, phi=0.02)
-    s.meta["config"] = index
-    return s
+def plain_small() -> System:
+    """N = 2000, seed 0, phi = 0.02: the size the CPU reference path runs."""
+    return make_system(2000, seed=0, phi=0.02)
+
+
+def plain_at_scale() -> System:
+    """N = 65536, seed 0, phi = 0.02 (L ~ 1.37 mm): the direct benchmark system."""
+    return make_system(65536, seed=0, phi=0.02)
+
+
+def tree_large() -> System:
+    """N = 2^20, seed 0, phi = 0.02 (L ~ 3.45 mm): the Barnes-Hut system."""
+    return make_system(1 << 20, seed=0, phi=0.02)
+
+
+def differentiable() -> System:
+    """N = 4096, seed 0, phi = 0.02: the system the adjoint is checked on."""
+    return make_system(4096, seed=0, phi=0.02)
 
 
 def ensemble_specs(count: int = 256, base_seed: int = 0):
-    """(N_e, E_e, phi_e) of the batched-ensemble configuration."""
+    """(N_e, E_e, phi_e) of the 256-ensemble batch: N_e ~ U{500..20000},
+    E_e ~ U[1e5, 5e5] V/m, phi_e ~ U[0.01, 0.05], from default_rng(base_seed)."""
     rng = np.random.default_rng(base_seed)
     ns = rng.integers(500, 20001, size=count)
     Es = rng.uniform(1e5, 5e5, size=count)

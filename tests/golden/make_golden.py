@@ -1,7 +1,7 @@
 """Writes the committed golden fixtures from the FP64 oracle.
 
-direct_n2000.npz — the N=2000, seed-0, phi=0.02 periodic system (the
-    CPU-runnable configuration): positions, radii, constants, FP64 oracle
+direct_n2000.npz — the N=2000, seed-0, phi=0.02 periodic system (the size
+    the reference's CPU path runs): positions, radii, constants, FP64 oracle
     forces and Oseen velocities.
 hash_n2000.npz   — the same positions with radii inflated 2.2x (so that
     contacts exist) and the oracle's unique sorted contact pairs for a table

@@ -88,7 +88,7 @@ def test_periodic_miss_reproduced(gpu):
     assert oracle.brute_pairs(pos, r, 1.0, True).tolist() == [[0, 1]]
 
 
-def test_config1_scale_bit_exact(gpu):
+def test_n65536_scale_bit_exact(gpu):
     """N = 65536 at phi = 0.02 with radii inflated 1.6x to create contacts."""
     s = make_system(65536, seed=0, phi=0.02)
     r = s.radii * 1.6
@@ -100,7 +100,7 @@ def test_config1_scale_bit_exact(gpu):
     # every pair is a true contact and the set is closed under the brute force
     # restricted to the rows it reports
     g = max(1, int(np.ceil(s.L / (2 * r.max()))))
-    print(f"\n[config1] g={g} L/cell={s.L / (2 * r.max()):.3f} pairs={len(ref)}")
+    print(f"\n[N=65536] g={g} L/cell={s.L / (2 * r.max()):.3f} pairs={len(ref)}")
 
 
 def test_unsynced_capacity_growth(gpu):
