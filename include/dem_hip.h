@@ -158,6 +158,73 @@ int dem_euler_step_total(double* pos, const double* vel, int n, double dt,
                          double L, int periodic, dem_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Flat Barnes-Hut octree                                                     */
+/* ------------------------------------------------------------------------ */
+
+#define DEM_TREE_MAX_LEVELS 64
+
+/* Host-side description of a built tree.  The caller sets n_capacity and
+ * max_nodes (reference FlatOctree.__init__: max_nodes =
+ * max(4*max(N,64)//max(mpl,1), 1024), flat_tree.py:63-67) before the first
+ * build; dem_octree_build fills the rest.  level_offsets[k] is the index in
+ * the parents array where parent level k starts (the reference's
+ * _parent_level_offsets). */
+typedef struct dem_tree_info {
+  int32_t n_capacity;
+  int32_t max_nodes;
+  int32_t n;
+  int32_t node_count;
+  int32_t parent_count;
+  int32_t leaf_slot_count;
+  int32_t num_levels;
+  int32_t max_depth;
+  int32_t level_offsets[DEM_TREE_MAX_LEVELS];
+  int32_t periodic;
+  int32_t mpl;
+  int32_t forces_valid; /* particle_forces holds the last dem_octree_forces output */
+  int32_t reserved;
+  double L;
+} dem_tree_info;
+
+size_t dem_octree_workspace_bytes(int n_capacity, int max_nodes);
+
+/* Builds the tree of the root cube [0, L]^3 (FlatOctree.build,
+ * flat_tree.py:180-246): same node numbering (breadth-first, children in
+ * octant order, split when count > mpl, capacity clamp), leaf slots, next
+ * pointers and R^3 aggregates.  m_const only sets the internal force unit.
+ * Synchronises `stream` once per tree level. */
+int dem_octree_build(const double* pos, const double* radii, int n, double L, int periodic,
+                     int mpl, double m_const, dem_tree_info* info, void* ws, size_t ws_bytes,
+                     dem_stream_t stream);
+
+/* Dipole forces by the stackless walk with opening test s^2 < theta_sq * D^2
+ * (_compute_forces_kernel,
+ * flat_tree.py:783-906); also kept as the tree's particle_forces. */
+int dem_octree_forces(dem_tree_info* info, double theta_sq, double m_const, double* forces,
+                      void* ws, size_t ws_bytes, dem_stream_t stream);
+
+/* Oseen velocities (monopole + dipole correction, optional COMSOL term and
+ * self term; _compute_stokeslet_kernel, flat_tree.py:1000-1312).  forces ==
+ * NULL uses particle_forces. */
+int dem_octree_stokeslet(const dem_tree_info* info, double theta_sq, const double* forces,
+                         double eta_const, const dem_correction* corr, double* vel, void* ws,
+                         size_t ws_bytes, dem_stream_t stream);
+
+/* Migration + Oseen velocity in one walk, without the COMSOL self term
+ * (_compute_total_velocity_kernel, flat_tree.py:1326-1612). */
+int dem_octree_total_velocity(const dem_tree_info* info, double theta_sq, const double* forces,
+                              double eta_const, double stokes_factor,
+                              const dem_correction* corr, double* vel, void* ws,
+                              size_t ws_bytes, dem_stream_t stream);
+
+/* Byte offset and size inside the octree workspace of a named node / particle
+ * array ("count", "first_child", "next", "leaf_start", "min", "max",
+ * "R3_sum", "R3_c", "force_sum", "fmom", "max_radius", "parents",
+ * "parent_first_child", "leaf_indices", "perm", "particle_forces"). */
+int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset,
+                     size_t* bytes);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing (measurement support; no counterpart in the reference)       */
 /* ------------------------------------------------------------------------ */
 

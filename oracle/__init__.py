@@ -94,6 +94,11 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_brute_pairs.restype = i
         _lib.oracle_num_threads.restype = i
         _lib.oracle_set_threads.argtypes = [i]
+        _lib.oracle_tree_build.argtypes = [p]
+        _lib.oracle_tree_build.restype = i
+        _lib.oracle_tree_propagate_forces.argtypes = [p, p]
+        _lib.oracle_tree_forces.argtypes = [p, d, d, p, i, i]
+        _lib.oracle_tree_stokes.argtypes = [p, i, d, p, d, d, p, p, i, i]
     return _lib
 
 
@@ -247,6 +252,90 @@ def euler_step_total(positions, vel, dt, L, periodic):
     return x
 
 
+# ---------------------------------------------------------------- octree
+
+
+class _Tree(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("max_nodes", ctypes.c_int),
+        ("pos", ctypes.c_void_p), ("r", ctypes.c_void_p),
+        ("L", ctypes.c_double), ("periodic", ctypes.c_int), ("mpl", ctypes.c_int),
+        ("bmin", ctypes.c_void_p), ("bmax", ctypes.c_void_p), ("r3sum", ctypes.c_void_p),
+        ("r3c", ctypes.c_void_p), ("fsum", ctypes.c_void_p), ("fmom", ctypes.c_void_p),
+        ("maxr", ctypes.c_void_p), ("first_child", ctypes.c_void_p), ("next", ctypes.c_void_p),
+        ("leaf_start", ctypes.c_void_p), ("count", ctypes.c_void_p),
+        ("parents", ctypes.c_void_p), ("pfc", ctypes.c_void_p),
+        ("leaf_indices", ctypes.c_void_p),
+        ("level_offsets", ctypes.c_int * 64), ("num_levels", ctypes.c_int),
+        ("node_count", ctypes.c_int), ("parent_count", ctypes.c_int),
+        ("leaf_slot_count", ctypes.c_int),
+    ]
+
+
+def max_nodes_for(num_particles: int, mpl: int) -> int:
+    """FlatOctree capacity rule (flat_tree.py:63-67)."""
+    n = max(num_particles, 64)
+    return max(4 * n // max(mpl, 1), 1024)
+
+
+class OracleTree:
+    """The reference's level-by-level octree, restated serially in C
+    (oracle/dem_octree_oracle.c), with its force / Stokeslet / total-velocity walks."""
+
+    def __init__(self, pos, radii, L, periodic=False, mpl=1, max_nodes=None):
+        self.pos = _c(pos).reshape(-1, 3)
+        self.r = _c(radii)
+        n = self.pos.shape[0]
+        M = int(max_nodes if max_nodes is not None else max_nodes_for(n, mpl))
+        self.arr = {
+            "bmin": np.zeros(3 * M), "bmax": np.zeros(3 * M), "r3sum": np.zeros(M),
+            "r3c": np.zeros(3 * M), "fsum": np.zeros(3 * M), "fmom": np.zeros(9 * M),
+            "maxr": np.zeros(M),
+            "first_child": np.full(M, -1, np.int32), "next": np.full(M, -1, np.int32),
+            "leaf_start": np.full(M, -1, np.int32), "count": np.zeros(M, np.int32),
+            "parents": np.zeros(M, np.int32), "pfc": np.zeros(M, np.int32),
+            "leaf_indices": np.zeros(max(n, 1), np.int32),
+        }
+        a = self.arr
+        self.st = _Tree(n, M, _ptr(self.pos), _ptr(self.r), float(L), int(bool(periodic)),
+                        int(mpl), *(_ptr(a[k]) for k in (
+                            "bmin", "bmax", "r3sum", "r3c", "fsum", "fmom", "maxr",
+                            "first_child", "next", "leaf_start", "count", "parents", "pfc",
+                            "leaf_indices")))
+        lib().oracle_tree_build(ctypes.byref(self.st))
+        self.node_count = self.st.node_count
+        self.parent_count = self.st.parent_count
+        self.leaf_slot_count = self.st.leaf_slot_count
+        self.level_offsets = [self.st.level_offsets[k] for k in range(self.st.num_levels)]
+
+    @property
+    def n(self) -> int:
+        return self.pos.shape[0]
+
+    def forces(self, theta, m_const, targets=None) -> np.ndarray:
+        i0, i1 = targets if targets is not None else (0, self.n)
+        F = np.zeros((self.n, 3))
+        lib().oracle_tree_forces(ctypes.byref(self.st), float(theta), float(m_const), _ptr(F),
+                                 i0, i1)
+        return F
+
+    def _stokes(self, mode, theta, F, eta, stokes, corr, targets):
+        F = _c(F).reshape(-1, 3)
+        lib().oracle_tree_propagate_forces(ctypes.byref(self.st), _ptr(F))
+        i0, i1 = targets if targets is not None else (0, self.n)
+        V = np.zeros((self.n, 3))
+        cp = ctypes.byref(corr.st) if corr is not None else None
+        lib().oracle_tree_stokes(ctypes.byref(self.st), mode, float(theta), _ptr(F), float(eta),
+                                 float(stokes), cp, _ptr(V), i0, i1)
+        return V
+
+    def stokeslet(self, theta, F, eta, corr=None, targets=None) -> np.ndarray:
+        return self._stokes(0, theta, F, eta, 0.0, corr, targets)
+
+    def total_velocity(self, theta, F, eta, stokes, corr=None, targets=None) -> np.ndarray:
+        return self._stokes(1, theta, F, eta, stokes, corr, targets)
+
+
 def stokes_factor(eta_oil, eta_water) -> float:
     """euler_droplet_solver.py:37-41."""
     return 2 * np.pi * eta_oil * (2 * eta_oil + 3 * eta_water) / (eta_oil + eta_water)
@@ -268,7 +357,7 @@ def eta_const(eta_oil) -> float:
 __all__ = [
     "build", "lib", "num_threads", "set_threads", "direct_forces", "direct_convection",
     "Correction", "hash_pairs", "brute_pairs", "merge", "euler_step", "euler_step_total",
-    "stokes_factor", "m_const", "eta_const", "EPS0",
+    "stokes_factor", "m_const", "eta_const", "EPS0", "OracleTree", "max_nodes_for",
 ]
 
 if os.environ.get("DEM_ORACLE_EAGER_BUILD"):

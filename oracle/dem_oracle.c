@@ -356,3 +356,11 @@ int oracle_brute_pairs(int n, const double* pos, const double* r, double L, int 
       }
   return m;
 }
+
+/* Exported for the octree oracle: one pair's correction and the self term. */
+void oracle_corr_pair_term(const oracle_corr* c, double x, double y, double z, double fx,
+                           double fy, double fz, double* vx, double* vy, double* vz) {
+  if (c && c->enabled) corr_pair(c, x, y, z, fx, fy, fz, vx, vy, vz);
+}
+
+double oracle_corr_self(const oracle_corr* c) { return (c && c->enabled) ? c->self_coeff : 0.0; }

@@ -20,10 +20,12 @@ struct Carver {
   char* base;
   size_t used = 0;
   explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  // With a null base the returned "pointers" are byte offsets (used to size
+  // workspaces and to locate named sub-buffers without a device).
   template <class T>
   T* take(size_t count) {
     used = align_up(used, 256);
-    T* p = reinterpret_cast<T*>(base ? base + used : nullptr);
+    T* p = reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(base) + used);
     used += count * sizeof(T);
     return p;
   }

@@ -1,0 +1,1241 @@
+// Flat Barnes-Hut octree for gfx950: build, aggregates and stackless walks.
+//
+// Reference: dem/octree/flat_tree.py (FlatOctree) and octree_node.py.
+//
+// Structure parity.  The reference builds level by level: every node with
+// count > mpl is split into 8 children (empty ones included), nodes of a level
+// are processed in index order, children get consecutive indices, and a split
+// is skipped once node_count + 8 would exceed max_nodes (flat_tree.py:308-354).
+// Node indices are therefore breadth-first with children in octant order,
+// which within a level is Morton order.  This build reproduces that exact
+// numbering:
+//   1. every particle computes its octant path (21 levels, 63-bit key) by
+//      replaying the reference's float64 bound recursion (cx = (min+max)*0.5,
+//      bit = x >= cx; :283-306, octree_node.py:83-95), so particles exactly on a
+//      split plane land where the reference puts them;
+//   2. a stable radix sort orders particles by key: every node owns a
+//      contiguous range of the sorted array;
+//   3. level by level, a scan over the level's split candidates gives each
+//      split node its rank, and the capacity clamp becomes "rank < budget";
+//      children ranges come from binary searches of the next octant digit.
+// Leaf slots (leaf_start / leaf_indices, :370-397), next pointers (two
+// top-down passes, :399-463) and the bottom-up R^3 / force-moment aggregates
+// (:469-777) follow the reference's definitions; leaf contents are listed in
+// sorted order (the reference's order inside a leaf is an atomic race).
+//
+// Walks (:783-1612).  One target per lane, targets in Morton order so the
+// lanes of a wave walk similar paths.  The opening test
+// `s*s < theta^2 * D^2 and D^2 > 0` and the skip tests are evaluated in
+// float64 exactly as the reference writes them, so the set of opened nodes
+// matches; the interaction arithmetic is FP32 (fixed-point positions as in
+// the direct kernel), accumulated in float64.
+#include <math.h>
+#include <string.h>
+
+#include "dem_common.h"
+#include "profile.h"
+#include "scan.h"
+#include "sort.h"
+
+namespace {
+
+constexpr int kKeyLevels = 21;  // 3 * 21 = 63 key bits
+constexpr int kMaxLevels = 40;  // reference MAX_LEVELS
+constexpr double kTwo32 = 4294967296.0;
+constexpr double kSigma = 0x1p-22;
+constexpr float kSigma2f = 0x1p-44f;
+constexpr double kSigma3 = 0x1p-66;
+constexpr double kR3Skip = 1e-30;   // flat_tree.py:29
+constexpr double kFsqSkip = 1e-60;  // flat_tree.py:32
+
+enum NodeFlags { kSkipR3 = 1, kSkipF = 2 };
+
+// Per-node walk record (64 B).
+struct alignas(16) NodeW {
+  double cx, cy, cz;  // R^3-weighted centre (geometric centre if R3_sum == 0)
+  double s2;          // s*s, s = largest bbox extent
+  int first_child, next, count, mstart;
+  float r3w;          // R3_sum in work units^3
+  int flags;
+  int pad0, pad1;
+};
+static_assert(sizeof(NodeW) == 64, "node record");
+
+// Per-node Stokeslet data: force sum / Fs and the centred dipole tensor
+// D_ab = P_ab - cm_a F_b divided by Fs*U (fixed units).
+struct alignas(16) NodeF {
+  float fx, fy, fz, pad;
+  float d[9];
+  float pad2[3];
+};
+static_assert(sizeof(NodeF) == 64, "node force record");
+
+struct RecF {
+  uint32_t x, y, z;
+  float r, r3;
+  uint32_t pad[3];
+};
+
+struct Scales {
+  double origin[3];
+  double inv_U, U, lambda, Fs;
+  int periodic, pad;
+};
+
+// Workspace layout.
+struct TreeLayout {
+  Scales* sc;
+  double* bbox;                  // [256][6]
+  // nodes (M = max_nodes)
+  double* bmin;                  // [3M]
+  double* bmax;                  // [3M]
+  double* r3sum;                 // [M]
+  double* r3c;                   // [3M]
+  double* fsum;                  // [3M]
+  double* fmom;                  // [9M] (xx,xy,xz,yx,yy,yz,zx,zy,zz)
+  double* maxr;                  // [M]
+  int* first_child;              // [M]
+  int* next;                     // [M]
+  int* leaf_start;               // [M]
+  int* count;                    // [M]
+  int* mstart;                   // [M]
+  int* parents;                  // [M]
+  int* pfc;                      // [M]
+  int* flag;                     // [M]
+  int* rank;                     // [M]
+  int* leafcnt;                  // [M]
+  NodeW* nw;                     // [M]
+  NodeF* nf;                     // [M]
+  int* counter;                  // [16]
+  // particles (N = capacity)
+  unsigned long long* keys;      // [N]
+  unsigned long long* keys_alt;  // [N]
+  int* perm;                     // [N] sorted -> original
+  int* perm_alt;                 // [N]
+  int* leaf_indices;             // [N]
+  double* spos;                  // [3N] sorted float64 positions
+  double* srad;                  // [N]
+  RecF* srec;                    // [N]
+  float* sF;                     // [3N] sorted source forces / Fs
+  double* pforces;               // [3N] forces of the last force walk (original order)
+  int* scan_ws;
+  int* sort_ws;
+  size_t bytes;
+};
+
+TreeLayout carve(void* ws, int ncap, int M) {
+  dem::Carver c(ws);
+  TreeLayout l;
+  const size_t N = (size_t)(ncap > 0 ? ncap : 1);
+  const size_t m = (size_t)(M > 0 ? M : 1);
+  l.sc = c.take<Scales>(1);
+  l.bbox = c.take<double>(256 * 6);
+  l.bmin = c.take<double>(3 * m);
+  l.bmax = c.take<double>(3 * m);
+  l.r3sum = c.take<double>(m);
+  l.r3c = c.take<double>(3 * m);
+  l.fsum = c.take<double>(3 * m);
+  l.fmom = c.take<double>(9 * m);
+  l.maxr = c.take<double>(m);
+  l.first_child = c.take<int>(m);
+  l.next = c.take<int>(m);
+  l.leaf_start = c.take<int>(m);
+  l.count = c.take<int>(m);
+  l.mstart = c.take<int>(m);
+  l.parents = c.take<int>(m);
+  l.pfc = c.take<int>(m);
+  l.flag = c.take<int>(m);
+  l.rank = c.take<int>(m);
+  l.leafcnt = c.take<int>(m);
+  l.nw = c.take<NodeW>(m);
+  l.nf = c.take<NodeF>(m);
+  l.counter = c.take<int>(16);
+  l.keys = c.take<unsigned long long>(N);
+  l.keys_alt = c.take<unsigned long long>(N);
+  l.perm = c.take<int>(N);
+  l.perm_alt = c.take<int>(N);
+  l.leaf_indices = c.take<int>(N);
+  l.spos = c.take<double>(3 * N);
+  l.srad = c.take<double>(N);
+  l.srec = c.take<RecF>(N);
+  l.sF = c.take<float>(3 * N);
+  l.pforces = c.take<double>(3 * N);
+  const size_t big = N > m ? N : m;
+  l.scan_ws = c.take<int>(dem::scan_workspace_ints((int)big));
+  l.sort_ws = c.take<int>(dem::radix_sort_workspace_ints((int)N));
+  l.bytes = dem::align_up(c.used, 256);
+  return l;
+}
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double mic64(double d, double L, int periodic) {
+  if (periodic && fabs(d) > L * 0.5) d -= copysign(L, d);
+  return d;
+}
+
+__global__ void k_fill_i(int* p, int n, int v) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_d(double* p, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- build
+
+// Octant path of every particle: replays the reference's bound recursion
+// from the root [0, L]^3 (flat_tree.py:250-259, :338-349).
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ pos, int n, double L,
+                                              unsigned long long* __restrict__ keys,
+                                              int* __restrict__ perm) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
+  double mnx = 0.0, mny = 0.0, mnz = 0.0, mxx = L, mxy = L, mxz = L;
+  unsigned long long key = 0ull;
+  for (int l = 0; l < kKeyLevels; ++l) {
+    const double cx = (mnx + mxx) * 0.5, cy = (mny + mxy) * 0.5, cz = (mnz + mxz) * 0.5;
+    const int bx = x >= cx, by = y >= cy, bz = z >= cz;
+    key = (key << 3) | (unsigned long long)(bx | (by << 1) | (bz << 2));
+    if (bx) mnx = cx; else mxx = cx;
+    if (by) mny = cy; else mxy = cy;
+    if (bz) mnz = cz; else mxz = cz;
+  }
+  keys[i] = key;
+  perm[i] = i;
+}
+
+__device__ __forceinline__ int digit_at(unsigned long long key, int level) {
+  // octant digit that selects the child at `level` (1..21)
+  return (int)((key >> (3 * (kKeyLevels - level))) & 7ull);
+}
+
+__global__ void k_root(double L, int n, double* bmin, double* bmax, int* first_child, int* next,
+                       int* leaf_start, int* count, int* mstart) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  bmin[0] = bmin[1] = bmin[2] = 0.0;
+  bmax[0] = bmax[1] = bmax[2] = L;
+  first_child[0] = -1;
+  next[0] = -1;
+  leaf_start[0] = -1;
+  count[0] = n;
+  mstart[0] = 0;
+}
+
+// Split candidates of one level: leaves with count > mpl (depth-limited by the key).
+__global__ void k_level_flags(int ls, int le, int mpl, int level, const int* __restrict__ count,
+                              int* __restrict__ flag) {
+  int k = blockIdx.x * 256 + threadIdx.x;
+  int node = ls + k;
+  if (node >= le) return;
+  flag[k] = (level < kKeyLevels && count[node] > mpl) ? 1 : 0;
+}
+
+// Split the first `budget` candidates (in node order) of the level.
+__global__ void k_level_split(int ls, int le, int level, int budget, int node_base, int pc_base,
+                              const int* __restrict__ flag, const int* __restrict__ rank,
+                              const unsigned long long* __restrict__ keys, double* bmin,
+                              double* bmax, int* first_child, int* next, int* leaf_start,
+                              int* count, int* mstart, int* parents, int* pfc) {
+#pragma clang fp contract(off)
+  int k = blockIdx.x * 256 + threadIdx.x;
+  int node = ls + k;
+  if (node >= le || !flag[k] || rank[k] >= budget) return;
+  const int q = rank[k];
+  const int base = node_base + 8 * q;
+  first_child[node] = base;
+  parents[pc_base + q] = node;
+  pfc[pc_base + q] = base;
+  const double mnx = bmin[3 * node], mny = bmin[3 * node + 1], mnz = bmin[3 * node + 2];
+  const double mxx = bmax[3 * node], mxy = bmax[3 * node + 1], mxz = bmax[3 * node + 2];
+  const double cx = (mnx + mxx) * 0.5, cy = (mny + mxy) * 0.5, cz = (mnz + mxz) * 0.5;
+  const int s0 = mstart[node], s1 = s0 + count[node];
+  int lo = s0;
+  for (int oct = 0; oct < 8; ++oct) {
+    // first position in [lo, s1) whose digit exceeds `oct`
+    int a = lo, b = s1;
+    while (a < b) {
+      int mid = (a + b) >> 1;
+      if (digit_at(keys[mid], level + 1) <= oct) a = mid + 1; else b = mid;
+    }
+    const int child = base + oct;
+    bmin[3 * child] = (oct & 1) ? cx : mnx;
+    bmin[3 * child + 1] = (oct & 2) ? cy : mny;
+    bmin[3 * child + 2] = (oct & 4) ? cz : mnz;
+    bmax[3 * child] = (oct & 1) ? mxx : cx;
+    bmax[3 * child + 1] = (oct & 2) ? mxy : cy;
+    bmax[3 * child + 2] = (oct & 4) ? mxz : cz;
+    first_child[child] = -1;
+    next[child] = -1;
+    leaf_start[child] = -1;
+    count[child] = a - lo;
+    mstart[child] = lo;
+    lo = a;
+  }
+}
+
+__global__ void k_leaf_counts(int nn, const int* __restrict__ first_child,
+                              const int* __restrict__ count, int* __restrict__ leafcnt) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  leafcnt[i] = (first_child[i] < 0 && count[i] > 0) ? count[i] : 0;
+}
+
+// leaf_start from the exclusive scan of leaf counts (flat_tree.py:370-385),
+// leaf_indices filled in sorted order (:387-397).
+__global__ void k_leaf_fill(int nn, const int* __restrict__ first_child,
+                            const int* __restrict__ count, const int* __restrict__ mstart,
+                            const int* __restrict__ off, const int* __restrict__ perm,
+                            int* __restrict__ leaf_start, int* __restrict__ leaf_indices) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  if (first_child[i] < 0 && count[i] > 0) {
+    const int ls = off[i];
+    leaf_start[i] = ls;
+    const int m0 = mstart[i];
+    for (int k = 0; k < count[i]; ++k) leaf_indices[ls + k] = perm[m0 + k];
+  } else {
+    leaf_start[i] = -1;
+  }
+}
+
+// Sibling chain of the children of one parent level (pass 1, :399-415).
+__global__ void k_next_pass1(int p0, int p1, const int* __restrict__ parents,
+                             const int* __restrict__ pfc, int* next) {
+  int p = p0 + blockIdx.x * 256 + threadIdx.x;
+  if (p >= p1) return;
+  const int base = pfc[p];
+  const int pn = next[parents[p]];
+  for (int oct = 0; oct < 8; ++oct) next[base + oct] = (oct == 7) ? pn : base + oct + 1;
+}
+
+// Unlink empty children (pass 2, :417-446).
+__global__ void k_next_pass2(int p0, int p1, const int* __restrict__ parents,
+                             const int* __restrict__ pfc, const int* __restrict__ count, int* next,
+                             int* first_child) {
+  int p = p0 + blockIdx.x * 256 + threadIdx.x;
+  if (p >= p1) return;
+  const int parent = parents[p];
+  const int base = pfc[p];
+  if (count[parent] == 0) {
+    first_child[parent] = next[parent];
+    return;
+  }
+  int first = -1, last = -1;
+  for (int j = 0; j < 8; ++j) {
+    const int c = base + j;
+    if (count[c] > 0) {
+      if (first < 0) {
+        first = c;
+      } else {
+        next[last] = c;
+      }
+      last = c;
+    }
+  }
+  if (last >= 0) {
+    next[last] = next[parent];
+    first_child[parent] = first;
+  }
+}
+
+// Leaf R^3 aggregates, zeroing every node first (:469-511).
+__global__ void k_leaf_r3(int nn, const int* __restrict__ first_child,
+                          const int* __restrict__ count, const int* __restrict__ mstart,
+                          const double* __restrict__ spos, const double* __restrict__ srad,
+                          double* r3sum, double* r3c, double* maxr) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  double s = 0.0, wx = 0.0, wy = 0.0, wz = 0.0, mr = 0.0;
+  if (first_child[i] < 0 && count[i] > 0) {
+    const int m0 = mstart[i];
+    for (int k = 0; k < count[i]; ++k) {
+      const int p = m0 + k;
+      const double r = srad[p];
+      const double r3 = r * r * r;
+      s += r3;
+      wx += spos[3 * (size_t)p] * r3;
+      wy += spos[3 * (size_t)p + 1] * r3;
+      wz += spos[3 * (size_t)p + 2] * r3;
+      if (r > mr) mr = r;
+    }
+  }
+  r3sum[i] = s;
+  r3c[3 * i] = wx;
+  r3c[3 * i + 1] = wy;
+  r3c[3 * i + 2] = wz;
+  maxr[i] = mr;
+}
+
+// One parent level of the bottom-up R^3 sweep (:513-542).
+__global__ void k_sweep_r3(int p0, int p1, const int* __restrict__ parents,
+                           const int* __restrict__ pfc, const int* __restrict__ count,
+                           double* r3sum, double* r3c, double* maxr) {
+  int p = p0 + blockIdx.x * 256 + threadIdx.x;
+  if (p >= p1) return;
+  const int parent = parents[p], base = pfc[p];
+  double s = 0.0, wx = 0.0, wy = 0.0, wz = 0.0, mr = 0.0;
+  for (int j = 0; j < 8; ++j) {
+    const int c = base + j;
+    if (count[c] > 0) {
+      s += r3sum[c];
+      wx += r3c[3 * c];
+      wy += r3c[3 * c + 1];
+      wz += r3c[3 * c + 2];
+      if (maxr[c] > mr) mr = maxr[c];
+    }
+  }
+  r3sum[parent] = s;
+  r3c[3 * parent] = wx;
+  r3c[3 * parent + 1] = wy;
+  r3c[3 * parent + 2] = wz;
+  maxr[parent] = mr;
+}
+
+// Leaf force sums and raw moments P_ab = sum x_a F_b (:556-626, :696-766).
+__global__ void k_leaf_forces(int nn, const int* __restrict__ first_child,
+                              const int* __restrict__ count, const int* __restrict__ mstart,
+                              const int* __restrict__ perm, const double* __restrict__ spos,
+                              const double* __restrict__ F, double* fsum, double* fmom) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  double f[3] = {0.0, 0.0, 0.0}, P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (first_child[i] < 0 && count[i] > 0) {
+    const int m0 = mstart[i];
+    for (int k = 0; k < count[i]; ++k) {
+      const int p = m0 + k;
+      const int o = perm[p];
+      const double Fx = F[3 * (size_t)o], Fy = F[3 * (size_t)o + 1], Fz = F[3 * (size_t)o + 2];
+      const double rx = spos[3 * (size_t)p], ry = spos[3 * (size_t)p + 1], rz = spos[3 * (size_t)p + 2];
+      f[0] += Fx;
+      f[1] += Fy;
+      f[2] += Fz;
+      P[0] += rx * Fx; P[1] += rx * Fy; P[2] += rx * Fz;
+      P[3] += ry * Fx; P[4] += ry * Fy; P[5] += ry * Fz;
+      P[6] += rz * Fx; P[7] += rz * Fy; P[8] += rz * Fz;
+    }
+  }
+  for (int a = 0; a < 3; ++a) fsum[3 * (size_t)i + a] = f[a];
+  for (int a = 0; a < 9; ++a) fmom[9 * (size_t)i + a] = P[a];
+}
+
+__global__ void k_sweep_forces(int p0, int p1, const int* __restrict__ parents,
+                               const int* __restrict__ pfc, const int* __restrict__ count,
+                               double* fsum, double* fmom) {
+  int p = p0 + blockIdx.x * 256 + threadIdx.x;
+  if (p >= p1) return;
+  const int parent = parents[p], base = pfc[p];
+  double f[3] = {0.0, 0.0, 0.0}, P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 8; ++j) {
+    const int c = base + j;
+    if (count[c] > 0) {
+      for (int a = 0; a < 3; ++a) f[a] += fsum[3 * (size_t)c + a];
+      for (int a = 0; a < 9; ++a) P[a] += fmom[9 * (size_t)c + a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) fsum[3 * (size_t)parent + a] = f[a];
+  for (int a = 0; a < 9; ++a) fmom[9 * (size_t)parent + a] = P[a];
+}
+
+// ---------------------------------------------------------------- scales / packing
+
+__global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ pos, int n,
+                                              double* __restrict__ part) {
+  __shared__ double lo_s[3][256], hi_s[3][256];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    for (int a = 0; a < 3; ++a) {
+      const double v = pos[3 * (size_t)i + a];
+      lo[a] = fmin(lo[a], v);
+      hi[a] = fmax(hi[a], v);
+    }
+  for (int a = 0; a < 3; ++a) {
+    lo_s[a][threadIdx.x] = lo[a];
+    hi_s[a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        lo_s[a][threadIdx.x] = fmin(lo_s[a][threadIdx.x], lo_s[a][threadIdx.x + w]);
+        hi_s[a][threadIdx.x] = fmax(hi_s[a][threadIdx.x], hi_s[a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; ++a) {
+      part[6 * blockIdx.x + a] = lo_s[a][0];
+      part[6 * blockIdx.x + 3 + a] = hi_s[a][0];
+    }
+}
+
+// Scales of the fixed-point / work units (same scheme as the direct kernel).
+__global__ void k_scales(const double* __restrict__ bbox, int nblk, double L, int periodic,
+                         double m_const, Scales* __restrict__ sc) {
+  if (threadIdx.x != 0) return;
+  Scales s;
+  s.periodic = periodic;
+  s.pad = 0;
+  if (periodic) {
+    s.origin[0] = s.origin[1] = s.origin[2] = 0.0;
+    s.U = L / kTwo32;
+    s.inv_U = kTwo32 / L;
+  } else {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nblk; ++b)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fmin(lo[a], bbox[6 * b + a]);
+        hi[a] = fmax(hi[a], bbox[6 * b + 3 + a]);
+      }
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) ext = fmax(ext, hi[a] - lo[a]);
+    if (!(ext > 0.0) || !isfinite(ext)) ext = 1.0;
+    for (int a = 0; a < 3; ++a) s.origin[a] = isfinite(lo[a]) ? lo[a] : 0.0;
+    s.U = ext / (2147483648.0 - 4096.0);
+    s.inv_U = 1.0 / s.U;
+  }
+  s.lambda = s.U / kSigma;
+  const double fs = m_const * s.lambda * s.lambda;
+  s.Fs = (fs > 0.0 && isfinite(fs)) ? fs : 1.0;
+  *sc = s;
+}
+
+// Sorted particle data: float64 positions / radii and fixed-point records.
+__global__ __launch_bounds__(256) void k_sorted_particles(
+    const double* __restrict__ pos, const double* __restrict__ radii, int n,
+    const int* __restrict__ perm, const Scales* __restrict__ scp, double* __restrict__ spos,
+    double* __restrict__ srad, RecF* __restrict__ srec) {
+  int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const Scales sc = *scp;
+  const int o = perm[p];
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const double x = pos[3 * (size_t)o + a];
+    spos[3 * (size_t)p + a] = x;
+    if (sc.periodic) {
+      double u = x * sc.inv_U * 0x1p-32;
+      u -= floor(u);
+      q[a] = (uint32_t)(unsigned long long)rint(u * kTwo32);
+    } else {
+      q[a] = (uint32_t)(long long)rint((x - sc.origin[a]) * sc.inv_U);
+    }
+  }
+  const double r = radii[o];
+  srad[p] = r;
+  const double rl = r / sc.lambda;
+  RecF rr;
+  rr.x = q[0];
+  rr.y = q[1];
+  rr.z = q[2];
+  rr.r = (float)rl;
+  rr.r3 = (float)(rl * rl * rl);
+  rr.pad[0] = rr.pad[1] = rr.pad[2] = 0u;
+  srec[p] = rr;
+}
+
+// Walk record of every node (centre as get_center, octree_node.py:97-114).
+__global__ void k_node_walk(int nn, const double* __restrict__ bmin, const double* __restrict__ bmax,
+                            const double* __restrict__ r3sum, const double* __restrict__ r3c,
+                            const int* __restrict__ first_child, const int* __restrict__ next,
+                            const int* __restrict__ count, const int* __restrict__ mstart,
+                            const Scales* __restrict__ scp, NodeW* __restrict__ nw) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  NodeW w;
+  const double s3 = r3sum[i];
+  if (s3 > 0.0) {
+    const double inv = 1.0 / s3;
+    w.cx = r3c[3 * i] * inv;
+    w.cy = r3c[3 * i + 1] * inv;
+    w.cz = r3c[3 * i + 2] * inv;
+  } else {
+    w.cx = (bmin[3 * i] + bmax[3 * i]) * 0.5;
+    w.cy = (bmin[3 * i + 1] + bmax[3 * i + 1]) * 0.5;
+    w.cz = (bmin[3 * i + 2] + bmax[3 * i + 2]) * 0.5;
+  }
+  const double ex = bmax[3 * i] - bmin[3 * i], ey = bmax[3 * i + 1] - bmin[3 * i + 1],
+               ez = bmax[3 * i + 2] - bmin[3 * i + 2];
+  const double s = fmax(fmax(ex, ey), ez);
+  w.s2 = s * s;
+  w.first_child = first_child[i];
+  w.next = next[i];
+  w.count = count[i];
+  w.mstart = mstart[i];
+  const double lam = scp->lambda;
+  w.r3w = (float)(s3 / (lam * lam * lam));
+  w.flags = (count[i] == 0 || s3 < kR3Skip) ? kSkipR3 : 0;
+  w.pad0 = w.pad1 = 0;
+  nw[i] = w;
+}
+
+// Stokeslet node data + the skip flag on the force sum (:1021-1029, :1215-1240).
+__global__ void k_node_force(int nn, const double* __restrict__ fsum, const double* __restrict__ fmom,
+                             const NodeW* __restrict__ nwr, const Scales* __restrict__ scp,
+                             NodeF* __restrict__ nf, int* __restrict__ flags_out) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nn) return;
+  const Scales sc = *scp;
+  const double Fx = fsum[3 * (size_t)i], Fy = fsum[3 * (size_t)i + 1], Fz = fsum[3 * (size_t)i + 2];
+  const double fs2 = Fx * Fx + Fy * Fy + Fz * Fz;
+  const NodeW w = nwr[i];
+  NodeF o;
+  o.fx = (float)(Fx / sc.Fs);
+  o.fy = (float)(Fy / sc.Fs);
+  o.fz = (float)(Fz / sc.Fs);
+  o.pad = 0.f;
+  const double cm[3] = {w.cx, w.cy, w.cz};
+  const double F[3] = {Fx, Fy, Fz};
+  const double k = 1.0 / (sc.Fs * sc.U);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) o.d[3 * a + b] = (float)((fmom[9 * (size_t)i + 3 * a + b] - cm[a] * F[b]) * k);
+  o.pad2[0] = o.pad2[1] = o.pad2[2] = 0.f;
+  nf[i] = o;
+  int fl = w.flags & ~kSkipF;
+  if (w.count == 0 || fs2 < kFsqSkip) fl |= kSkipF;
+  flags_out[i] = fl;
+}
+
+__global__ void k_apply_flags(int nn, const int* __restrict__ fl, NodeW* nw) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nn) nw[i].flags = fl[i];
+}
+
+// Sorted source forces / Fs.
+__global__ void k_sorted_forces(int n, const int* __restrict__ perm, const double* __restrict__ F,
+                                const Scales* __restrict__ scp, float* __restrict__ sF) {
+  int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const double fs = scp->Fs;
+  const int o = perm[p];
+  for (int a = 0; a < 3; ++a) sF[3 * (size_t)p + a] = (float)(F[3 * (size_t)o + a] / fs);
+}
+
+// ---------------------------------------------------------------- walks
+
+struct CorrArgs {
+  const float4* grid;
+  int n, enabled;
+  float nmax;
+  float ga, gb;           // grid coordinate of x_i - x_j = -U D  is  ga * D + gb
+};
+
+__device__ __forceinline__ void grid_coord(float g, float nmax, int n, int& i, float& t) {
+  g = fminf(fmaxf(g, 0.f), nmax);
+  i = min((int)floorf(g), n - 2);
+  t = g - (float)i;
+}
+
+// Three permuted trilinear lookups applied to force fh (reference :413-457).
+__device__ __forceinline__ void corr_lookup(const CorrArgs& ca, float dx, float dy, float dz,
+                                            float fx, float fy, float fz, float& ux, float& uy,
+                                            float& uz) {
+  int gx, gy, gz;
+  float tx, ty, tz;
+  grid_coord(ca.ga * dx + ca.gb, ca.nmax, ca.n, gx, tx);
+  grid_coord(ca.ga * dy + ca.gb, ca.nmax, ca.n, gy, ty);
+  grid_coord(ca.ga * dz + ca.gb, ca.nmax, ca.n, gz, tz);
+  float w[2][2][2];
+  const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
+  w[0][0][0] = ox * oy * oz; w[0][0][1] = ox * oy * tz;
+  w[0][1][0] = ox * ty * oz; w[0][1][1] = ox * ty * tz;
+  w[1][0][0] = tx * oy * oz; w[1][0][1] = tx * oy * tz;
+  w[1][1][0] = tx * ty * oz; w[1][1][1] = tx * ty * tz;
+  const int nn = ca.n;
+  float gzx = 0.f, gzy = 0.f, gzz = 0.f, gxx = 0.f, gxy = 0.f, gxz = 0.f, gyx = 0.f, gyy = 0.f,
+        gyz = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float4 g1 = ca.grid[((gx + a) * nn + (gy + b)) * nn + (gz + c)];
+        const float w1 = w[a][b][c];
+        gzx += w1 * g1.x; gzy += w1 * g1.y; gzz += w1 * g1.z;
+        const float4 g2 = ca.grid[((gz + a) * nn + (gy + b)) * nn + (gx + c)];
+        const float w2 = w[c][b][a];
+        gxx += w2 * g2.z; gxy += w2 * g2.y; gxz += w2 * g2.x;
+        const float4 g3 = ca.grid[((gx + a) * nn + (gz + b)) * nn + (gy + c)];
+        const float w3 = w[a][c][b];
+        gyx += w3 * g3.x; gyy += w3 * g3.z; gyz += w3 * g3.y;
+      }
+  ux += gxx * fx + gyx * fy + gzx * fz;
+  uy += gxy * fx + gyy * fy + gzy * fz;
+  uz += gxz * fx + gyz * fy + gzz * fz;
+}
+
+struct WalkArgs {
+  const NodeW* nw;
+  const NodeF* nf;
+  const RecF* srec;
+  const double* spos;
+  const double* srad;
+  const float* sF;
+  const int* perm;
+  int n;
+  int guard;  // iteration bound of a walk (2 * node_count + 8)
+  double L;
+  int periodic;
+  double theta_sq;
+};
+
+// Force walk (:783-906).  Writes forces (original order) and the stored copy.
+__global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* __restrict__ scp,
+                                                     double m_const, double* __restrict__ F,
+                                                     double* __restrict__ pforces) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= wa.n) return;
+  const Scales sc = *scp;
+  const RecF me = wa.srec[t];
+  const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
+               pz = wa.spos[3 * (size_t)t + 2];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int node = 0;
+  int steps = wa.guard;  // a preorder walk visits each node at most once
+  while (node >= 0 && --steps > 0) {
+    const NodeW nd = wa.nw[node];
+    if (nd.flags & kSkipR3) {
+      node = nd.next;
+      continue;
+    }
+    if (nd.first_child < 0) {
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      const int e = nd.mstart + nd.count;
+      for (int j = nd.mstart; j < e; ++j) {
+        if (j == t) continue;
+        const RecF sj = wa.srec[j];
+        const float dx = (float)(int32_t)(sj.x - me.x);
+        const float dy = (float)(int32_t)(sj.y - me.y);
+        const float dz = (float)(int32_t)(sj.z - me.z);
+        const float dz2 = dz * dz;
+        const float dxy2 = dx * dx + dy * dy;
+        const float r2 = (dxy2 + dz2) * kSigma2f;
+        const float rs = me.r + sj.r;
+        if (r2 >= rs * rs) {  // tree leaves use >= (:835)
+          const float ir = __builtin_amdgcn_rsqf(r2);
+          const float ir2 = ir * ir;
+          const float s7 = sj.r3 * ((ir2 * ir2) * (ir2 * ir));
+          const float txy = s7 * (4.f * dz2 - dxy2);
+          fx += txy * dx;
+          fy += txy * dy;
+          fz += (s7 * (2.f * dz2 - 3.f * dxy2)) * dz;
+        }
+      }
+      ax += (double)fx;
+      ay += (double)fy;
+      az += (double)fz;
+      node = nd.next;
+      continue;
+    }
+    double dx = mic64(nd.cx - px, wa.L, wa.periodic);
+    double dy = mic64(nd.cy - py, wa.L, wa.periodic);
+    double dz = mic64(nd.cz - pz, wa.L, wa.periodic);
+    double d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
+      const float fdx = (float)(dx * sc.inv_U), fdy = (float)(dy * sc.inv_U),
+                  fdz = (float)(dz * sc.inv_U);
+      const float dz2 = fdz * fdz;
+      const float dxy2 = fdx * fdx + fdy * fdy;
+      const float r2 = (dxy2 + dz2) * kSigma2f;
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float ir2 = ir * ir;
+      const float s7 = nd.r3w * ((ir2 * ir2) * (ir2 * ir));
+      const float txy = s7 * (4.f * dz2 - dxy2);
+      ax += (double)(txy * fdx);
+      ay += (double)(txy * fdy);
+      az += (double)((s7 * (2.f * dz2 - 3.f * dxy2)) * fdz);
+      node = nd.next;
+    } else {
+      node = nd.first_child;
+    }
+  }
+  if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
+  const double rl = wa.srad[t] / sc.lambda;
+  const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
+  const int o = wa.perm[t];
+  const double Fx = c * ax, Fy = c * ay, Fz = c * az;
+  F[3 * (size_t)o] = Fx;
+  F[3 * (size_t)o + 1] = Fy;
+  F[3 * (size_t)o + 2] = Fz;
+  pforces[3 * (size_t)o] = Fx;
+  pforces[3 * (size_t)o + 1] = Fy;
+  pforces[3 * (size_t)o + 2] = Fz;
+}
+
+// Stokeslet / total-velocity walk (:1000-1280, :1326-1588).
+// MODE 0: Stokeslet (+ COMSOL self term); MODE 1: total velocity (migration
+// F_i / (stokes r_i) added, no self term — as the reference).
+template <int MODE, bool CORR>
+__global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* __restrict__ scp,
+                                                     CorrArgs ca, double eta_const, double common,
+                                                     double self_coeff, double stokes_factor,
+                                                     const double* __restrict__ Fself,
+                                                     double* __restrict__ V) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= wa.n) return;
+  const Scales sc = *scp;
+  const RecF me = wa.srec[t];
+  const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
+               pz = wa.spos[3 * (size_t)t + 2];
+  double ax = 0.0, ay = 0.0, az = 0.0;  // Oseen sum, fixed units
+  double cx = 0.0, cy = 0.0, cz = 0.0;  // COMSOL sum
+  int node = 0;
+  int steps = wa.guard;  // a preorder walk visits each node at most once
+  while (node >= 0 && --steps > 0) {
+    const NodeW nd = wa.nw[node];
+    if (nd.flags & kSkipF) {
+      node = nd.next;
+      continue;
+    }
+    if (nd.first_child < 0) {
+      float vx = 0.f, vy = 0.f, vz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+      const int e = nd.mstart + nd.count;
+      for (int j = nd.mstart; j < e; ++j) {
+        if (j == t) continue;
+        const RecF sj = wa.srec[j];
+        // D = x_j - x_i (fixed units); the Oseen tensor is even in D
+        const float dx = (float)(int32_t)(sj.x - me.x);
+        const float dy = (float)(int32_t)(sj.y - me.y);
+        const float dz = (float)(int32_t)(sj.z - me.z);
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float rs = me.r + sj.r;
+        if (r2 * kSigma2f >= rs * rs) {  // contact mask >= (:1055)
+          const float Fx = wa.sF[3 * (size_t)j], Fy = wa.sF[3 * (size_t)j + 1],
+                      Fz = wa.sF[3 * (size_t)j + 2];
+          const float ir = __builtin_amdgcn_rsqf(r2);
+          const float tt = (dx * Fx + dy * Fy + dz * Fz) * (ir * ir);
+          vx += ir * (Fx + tt * dx);
+          vy += ir * (Fy + tt * dy);
+          vz += ir * (Fz + tt * dz);
+          if (CORR) corr_lookup(ca, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+        }
+      }
+      ax += (double)vx;
+      ay += (double)vy;
+      az += (double)vz;
+      if (CORR) {
+        cx += (double)ux;
+        cy += (double)uy;
+        cz += (double)uz;
+      }
+      node = nd.next;
+      continue;
+    }
+    // x_i - centre (:1139-1150)
+    double dx = mic64(px - nd.cx, wa.L, wa.periodic);
+    double dy = mic64(py - nd.cy, wa.L, wa.periodic);
+    double dz = mic64(pz - nd.cz, wa.L, wa.periodic);
+    double d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    d2 = d2 + dz * dz;
+    if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
+      const NodeF nf = wa.nf[node];
+      // R = centre - x_i in fixed units (sign irrelevant for the monopole)
+      const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
+                  Rz = (float)(-dz * sc.inv_U);
+      const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float ir2 = ir * ir;
+      const float tt = (Rx * nf.fx + Ry * nf.fy + Rz * nf.fz) * ir2;
+      float vx = ir * (nf.fx + tt * Rx);
+      float vy = ir * (nf.fy + tt * Ry);
+      float vz = ir * (nf.fz + tt * Rz);
+      // dipole correction with r = x_i - centre = -R (:1242-1265), written with
+      // the unit vector n = r / |r| to stay inside the FP32 range
+      const float nx = -Rx * ir, ny = -Ry * ir, nz = -Rz * ir;
+      const float* D = nf.d;
+      const float ux = D[0] * nx + D[3] * ny + D[6] * nz;
+      const float uy = D[1] * nx + D[4] * ny + D[7] * nz;
+      const float uz = D[2] * nx + D[5] * ny + D[8] * nz;
+      const float wx = D[0] * nx + D[1] * ny + D[2] * nz;
+      const float wy = D[3] * nx + D[4] * ny + D[5] * nz;
+      const float wz = D[6] * nx + D[7] * ny + D[8] * nz;
+      const float str = D[0] + D[4] + D[8];
+      const float q = nx * wx + ny * wy + nz * wz;
+      vx += ir2 * ((ux - wx - str * nx) + 3.f * nx * q);
+      vy += ir2 * ((uy - wy - str * ny) + 3.f * ny * q);
+      vz += ir2 * ((uz - wz - str * nz) + 3.f * nz * q);
+      ax += (double)vx;
+      ay += (double)vy;
+      az += (double)vz;
+      if (CORR) {
+        float ux2 = 0.f, uy2 = 0.f, uz2 = 0.f;
+        corr_lookup(ca, Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, ux2, uy2, uz2);
+        cx += (double)ux2;
+        cy += (double)uy2;
+        cz += (double)uz2;
+      }
+      node = nd.next;
+    } else {
+      node = nd.first_child;
+    }
+  }
+  if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
+  const int o = wa.perm[t];
+  const double km = eta_const * sc.Fs / sc.U;
+  const double kc = sc.Fs * common;
+  double v[3] = {km * ax, km * ay, km * az};
+  if (CORR) {
+    v[0] += kc * cx;
+    v[1] += kc * cy;
+    v[2] += kc * cz;
+  }
+  if (MODE == 0 && CORR) {
+    for (int a = 0; a < 3; ++a) v[a] += self_coeff * Fself[3 * (size_t)o + a];
+  }
+  if (MODE == 1) {
+    const double inv = 1.0 / (stokes_factor * wa.srad[t]);
+    for (int a = 0; a < 3; ++a) v[a] = Fself[3 * (size_t)o + a] * inv + v[a];
+  }
+  for (int a = 0; a < 3; ++a) V[3 * (size_t)o + a] = v[a];
+}
+
+// ---------------------------------------------------------------- host helpers
+
+inline int nb256(long long n) { return (int)((n + 255) / 256); }
+
+int check_info(const dem_tree_info* info, const void* ws, size_t ws_bytes, TreeLayout& l) {
+  if (!info) return dem::set_error(DEM_ERR_INVALID, "tree info is NULL");
+  l = carve(const_cast<void*>(ws), info->n_capacity, info->max_nodes);
+  if (!ws || ws_bytes < l.bytes)
+    return dem::set_error(DEM_ERR_WORKSPACE, "octree workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  if (info->node_count < 1) return dem::set_error(DEM_ERR_INVALID, "octree has not been built");
+  return DEM_OK;
+}
+
+int propagate_forces(const dem_tree_info* info, const TreeLayout& l, const double* F,
+                     hipStream_t st) {
+  const int nn = info->node_count;
+  hipLaunchKernelGGL(k_leaf_forces, dim3(nb256(nn)), dim3(256), 0, st, nn,
+                     (const int*)l.first_child, (const int*)l.count, (const int*)l.mstart,
+                     (const int*)l.perm, (const double*)l.spos, F, l.fsum, l.fmom);
+  DEM_CHECK_LAUNCH("k_leaf_forces");
+  for (int lev = info->num_levels - 1; lev >= 0; --lev) {
+    const int p0 = info->level_offsets[lev];
+    const int p1 = lev + 1 < info->num_levels ? info->level_offsets[lev + 1] : info->parent_count;
+    if (p1 > p0) {
+      hipLaunchKernelGGL(k_sweep_forces, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
+                         (const int*)l.parents, (const int*)l.pfc, (const int*)l.count, l.fsum,
+                         l.fmom);
+      DEM_CHECK_LAUNCH("k_sweep_forces");
+    }
+  }
+  hipLaunchKernelGGL(k_node_force, dim3(nb256(nn)), dim3(256), 0, st, nn, (const double*)l.fsum,
+                     (const double*)l.fmom, (const NodeW*)l.nw, (const Scales*)l.sc, l.nf, l.flag);
+  DEM_CHECK_LAUNCH("k_node_force");
+  hipLaunchKernelGGL(k_apply_flags, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.flag,
+                     l.nw);
+  DEM_CHECK_LAUNCH("k_apply_flags");
+  hipLaunchKernelGGL(k_sorted_forces, dim3(nb256(info->n)), dim3(256), 0, st, info->n,
+                     (const int*)l.perm, F, (const Scales*)l.sc, l.sF);
+  DEM_CHECK_LAUNCH("k_sorted_forces");
+  return DEM_OK;
+}
+
+WalkArgs walk_args(const dem_tree_info* info, const TreeLayout& l, double theta_sq) {
+  WalkArgs w;
+  w.nw = l.nw;
+  w.nf = l.nf;
+  w.srec = l.srec;
+  w.spos = l.spos;
+  w.srad = l.srad;
+  w.sF = l.sF;
+  w.perm = l.perm;
+  w.n = info->n;
+  w.guard = 2 * info->node_count + 8;
+  w.L = info->L;
+  w.periodic = info->periodic;
+  w.theta_sq = theta_sq;
+  return w;
+}
+
+int corr_args(const dem_correction* corr, const dem_tree_info* info, CorrArgs& ca,
+              double& common, double& self_coeff) {
+  ca = CorrArgs{};
+  common = 0.0;
+  self_coeff = 0.0;
+  if (!corr || !corr->enabled) return DEM_OK;
+  if (!corr->grid || corr->n < 2)
+    return dem::set_error(DEM_ERR_INVALID, "correction enabled without a float grid");
+  ca.grid = reinterpret_cast<const float4*>(corr->grid);
+  ca.n = corr->n;
+  ca.enabled = 1;
+  ca.nmax = (float)(corr->n - 1);
+  // fixed unit U is known on the host for periodic boxes; open boxes read it on the device
+  ca.ga = 0.f;
+  ca.gb = (float)(-corr->grid_min * corr->inv_dx);
+  common = corr->Fz_inv * corr->eta_ratio * corr->L_ratio;
+  self_coeff = corr->self_coeff;
+  (void)info;
+  return DEM_OK;
+}
+
+__global__ void k_corr_scale(const Scales* __restrict__ scp, double L_ratio, double inv_dx,
+                             float* out) {
+  if (threadIdx.x == 0) out[0] = (float)(-scp->U * L_ratio * inv_dx);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dem_octree_workspace_bytes(int n_capacity, int max_nodes) {
+  return carve(nullptr, n_capacity < 0 ? 0 : n_capacity, max_nodes < 1 ? 1 : max_nodes).bytes;
+}
+
+int dem_octree_build(const double* pos, const double* radii, int n, double L, int periodic,
+                     int mpl, double m_const, dem_tree_info* info, void* ws, size_t ws_bytes,
+                     dem_stream_t stream) {
+  if (!info) return dem::set_error(DEM_ERR_INVALID, "tree info is NULL");
+  if (n < 0) return dem::set_error(DEM_ERR_INVALID, "n must be >= 0");
+  if (n > info->n_capacity)
+    return dem::set_error(DEM_ERR_CAPACITY,
+                          "Number of particles (%d) exceeds allocated capacity (%d)", n,
+                          info->n_capacity);
+  if (info->max_nodes < 1) return dem::set_error(DEM_ERR_INVALID, "max_nodes must be >= 1");
+  if (n > 0 && (!pos || !radii)) return dem::set_error(DEM_ERR_INVALID, "null positions/radii");
+  if (!(L > 0.0)) return dem::set_error(DEM_ERR_INVALID, "L must be > 0 (got %g)", L);
+  TreeLayout l = carve(ws, info->n_capacity, info->max_nodes);
+  if (!ws || ws_bytes < l.bytes)
+    return dem::set_error(DEM_ERR_WORKSPACE, "octree workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  hipStream_t st = dem::as_stream(stream);
+  const int M = info->max_nodes;
+  info->n = n;
+  info->L = L;
+  info->periodic = periodic ? 1 : 0;
+  info->mpl = mpl;
+  info->node_count = 1;
+  info->parent_count = 0;
+  info->num_levels = 0;
+  info->leaf_slot_count = 0;
+  info->max_depth = 0;
+
+  // scales + keys + sort
+  int nblk = n > 0 ? nb256(n) : 1;
+  if (nblk > 256) nblk = 256;
+  if (!periodic && n > 0) {
+    hipLaunchKernelGGL(k_bbox, dim3(nblk), dim3(256), 0, st, pos, n, l.bbox);
+    DEM_CHECK_LAUNCH("k_bbox");
+  }
+  hipLaunchKernelGGL(k_scales, dim3(1), dim3(64), 0, st, (const double*)l.bbox, nblk, L,
+                     periodic ? 1 : 0, m_const, l.sc);
+  DEM_CHECK_LAUNCH("k_scales");
+  if (n > 0) {
+    hipLaunchKernelGGL(k_keys, dim3(nb256(n)), dim3(256), 0, st, pos, n, L, l.keys, l.perm);
+    DEM_CHECK_LAUNCH("k_keys");
+    int rc = dem::radix_sort_u64(l.keys, l.perm, l.keys_alt, l.perm_alt, n, 3 * kKeyLevels,
+                                 l.sort_ws, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sorted_particles, dim3(nb256(n)), dim3(256), 0, st, pos, radii, n,
+                       (const int*)l.perm, (const Scales*)l.sc, l.spos, l.srad, l.srec);
+    DEM_CHECK_LAUNCH("k_sorted_particles");
+  }
+  hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, L, n, l.bmin, l.bmax, l.first_child,
+                     l.next, l.leaf_start, l.count, l.mstart);
+  DEM_CHECK_LAUNCH("k_root");
+
+  // level-by-level split (flat_tree.py:209-236)
+  int ls = 0, le = 1;
+  for (int level = 0; level < kMaxLevels; ++level) {
+    const int cnt = le - ls;
+    if (cnt <= 0) break;
+    hipLaunchKernelGGL(k_level_flags, dim3(nb256(cnt)), dim3(256), 0, st, ls, le, mpl, level,
+                       (const int*)l.count, l.flag);
+    DEM_CHECK_LAUNCH("k_level_flags");
+    int rc = dem::exclusive_scan(l.flag, l.rank, cnt, l.scan_ws, l.counter, st);
+    if (rc) return rc;
+    int cand = 0;
+    DEM_CHECK_HIP(hipMemcpyAsync(&cand, l.counter, sizeof(int), hipMemcpyDeviceToHost, st));
+    DEM_CHECK_HIP(hipStreamSynchronize(st));
+    const int room = (M - info->node_count) / 8;
+    const int k = cand < room ? cand : (room > 0 ? room : 0);
+    if (k <= 0) break;  // no split at this level: the reference's overflow flag stays 0
+    hipLaunchKernelGGL(k_level_split, dim3(nb256(cnt)), dim3(256), 0, st, ls, le, level, k,
+                       info->node_count, info->parent_count, (const int*)l.flag,
+                       (const int*)l.rank, (const unsigned long long*)l.keys, l.bmin, l.bmax,
+                       l.first_child, l.next, l.leaf_start, l.count, l.mstart, l.parents, l.pfc);
+    DEM_CHECK_LAUNCH("k_level_split");
+    if (info->num_levels >= DEM_TREE_MAX_LEVELS)
+      return dem::set_error(DEM_ERR_CAPACITY, "too many tree levels");
+    info->level_offsets[info->num_levels++] = info->parent_count;
+    info->parent_count += k;
+    ls = le;
+    le = info->node_count + 8 * k;
+    info->node_count = le;
+    info->max_depth = level + 1;
+  }
+  const int nn = info->node_count;
+
+  // leaf slots (flat_tree.py:370-397)
+  hipLaunchKernelGGL(k_leaf_counts, dim3(nb256(nn)), dim3(256), 0, st, nn,
+                     (const int*)l.first_child, (const int*)l.count, l.leafcnt);
+  DEM_CHECK_LAUNCH("k_leaf_counts");
+  int rc = dem::exclusive_scan(l.leafcnt, l.rank, nn, l.scan_ws, l.counter + 1, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_leaf_fill, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.first_child,
+                     (const int*)l.count, (const int*)l.mstart, (const int*)l.rank,
+                     (const int*)l.perm, l.leaf_start, l.leaf_indices);
+  DEM_CHECK_LAUNCH("k_leaf_fill");
+
+  // next pointers, two top-down passes (flat_tree.py:448-463)
+  for (int pass = 0; pass < 2; ++pass)
+    for (int lev = 0; lev < info->num_levels; ++lev) {
+      const int p0 = info->level_offsets[lev];
+      const int p1 = lev + 1 < info->num_levels ? info->level_offsets[lev + 1] : info->parent_count;
+      if (p1 <= p0) continue;
+      if (pass == 0)
+        hipLaunchKernelGGL(k_next_pass1, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
+                           (const int*)l.parents, (const int*)l.pfc, l.next);
+      else
+        hipLaunchKernelGGL(k_next_pass2, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
+                           (const int*)l.parents, (const int*)l.pfc, (const int*)l.count, l.next,
+                           l.first_child);
+      DEM_CHECK_LAUNCH("k_next_pass");
+    }
+
+  // R^3 aggregates (flat_tree.py:544-554)
+  hipLaunchKernelGGL(k_leaf_r3, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.first_child,
+                     (const int*)l.count, (const int*)l.mstart, (const double*)l.spos,
+                     (const double*)l.srad, l.r3sum, l.r3c, l.maxr);
+  DEM_CHECK_LAUNCH("k_leaf_r3");
+  for (int lev = info->num_levels - 1; lev >= 0; --lev) {
+    const int p0 = info->level_offsets[lev];
+    const int p1 = lev + 1 < info->num_levels ? info->level_offsets[lev + 1] : info->parent_count;
+    if (p1 > p0) {
+      hipLaunchKernelGGL(k_sweep_r3, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
+                         (const int*)l.parents, (const int*)l.pfc, (const int*)l.count, l.r3sum,
+                         l.r3c, l.maxr);
+      DEM_CHECK_LAUNCH("k_sweep_r3");
+    }
+  }
+  hipLaunchKernelGGL(k_node_walk, dim3(nb256(nn)), dim3(256), 0, st, nn, (const double*)l.bmin,
+                     (const double*)l.bmax, (const double*)l.r3sum, (const double*)l.r3c,
+                     (const int*)l.first_child, (const int*)l.next, (const int*)l.count,
+                     (const int*)l.mstart, (const Scales*)l.sc, l.nw);
+  DEM_CHECK_LAUNCH("k_node_walk");
+  int lsc = 0;
+  DEM_CHECK_HIP(hipMemcpyAsync(&lsc, l.counter + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  DEM_CHECK_HIP(hipStreamSynchronize(st));
+  info->leaf_slot_count = lsc;
+  info->forces_valid = 0;
+  return DEM_OK;
+}
+
+int dem_octree_forces(dem_tree_info* info, double theta_sq, double m_const, double* forces,
+                      void* ws, size_t ws_bytes, dem_stream_t stream) {
+  TreeLayout l;
+  int rc = check_info(info, ws, ws_bytes, l);
+  if (rc) return rc;
+  if (info->n == 0) return DEM_OK;
+  if (!forces) return dem::set_error(DEM_ERR_INVALID, "null forces");
+  hipStream_t st = dem::as_stream(stream);
+  WalkArgs wa = walk_args(info, l, theta_sq);
+  DEM_PROF_LAUNCH("k_walk_forces", st,
+                  hipLaunchKernelGGL(k_walk_forces, dim3(nb256(info->n)), dim3(256), 0, st, wa,
+                                     (const Scales*)l.sc, m_const, forces, l.pforces));
+  DEM_CHECK_LAUNCH("k_walk_forces");
+  info->forces_valid = 1;
+  return DEM_OK;
+}
+
+static int stokes_common(int mode, const dem_tree_info* info, double theta_sq, const double* forces,
+                         double eta_const, double stokes_factor, const dem_correction* corr,
+                         double* vel, void* ws, size_t ws_bytes, dem_stream_t stream) {
+  TreeLayout l;
+  int rc = check_info(info, ws, ws_bytes, l);
+  if (rc) return rc;
+  if (info->n == 0) return DEM_OK;
+  if (!vel) return dem::set_error(DEM_ERR_INVALID, "null velocities");
+  hipStream_t st = dem::as_stream(stream);
+  const double* F = forces ? forces : (const double*)l.pforces;
+  if (!forces && !info->forces_valid)
+    return dem::set_error(DEM_ERR_INVALID, "no forces: pass them or run dem_octree_forces first");
+  if ((rc = propagate_forces(info, l, F, st))) return rc;
+  CorrArgs ca;
+  double common, self_coeff;
+  if ((rc = corr_args(corr, info, ca, common, self_coeff))) return rc;
+  if (ca.enabled) {
+    // ga = -U * L_ratio * inv_dx needs U, which lives on the device for open boxes
+    float* tmp = reinterpret_cast<float*>(l.counter + 8);
+    hipLaunchKernelGGL(k_corr_scale, dim3(1), dim3(64), 0, st, (const Scales*)l.sc, corr->L_ratio,
+                       corr->inv_dx, tmp);
+    DEM_CHECK_LAUNCH("k_corr_scale");
+    float ga = 0.f;
+    DEM_CHECK_HIP(hipMemcpyAsync(&ga, tmp, sizeof(float), hipMemcpyDeviceToHost, st));
+    DEM_CHECK_HIP(hipStreamSynchronize(st));
+    ca.ga = ga;
+  }
+  WalkArgs wa = walk_args(info, l, theta_sq);
+  const dim3 g(nb256(info->n)), b(256);
+  const char* name = mode == 0 ? "k_walk_stokeslet" : "k_walk_total";
+  int slot = dem::prof_begin(name, st);
+  if (mode == 0 && ca.enabled)
+    hipLaunchKernelGGL((k_walk_stokes<0, true>), g, b, 0, st, wa, (const Scales*)l.sc, ca,
+                       eta_const, common, self_coeff, stokes_factor, F, vel);
+  else if (mode == 0)
+    hipLaunchKernelGGL((k_walk_stokes<0, false>), g, b, 0, st, wa, (const Scales*)l.sc, ca,
+                       eta_const, common, self_coeff, stokes_factor, F, vel);
+  else if (ca.enabled)
+    hipLaunchKernelGGL((k_walk_stokes<1, true>), g, b, 0, st, wa, (const Scales*)l.sc, ca,
+                       eta_const, common, self_coeff, stokes_factor, F, vel);
+  else
+    hipLaunchKernelGGL((k_walk_stokes<1, false>), g, b, 0, st, wa, (const Scales*)l.sc, ca,
+                       eta_const, common, self_coeff, stokes_factor, F, vel);
+  dem::prof_end(slot, st);
+  DEM_CHECK_LAUNCH(name);
+  return DEM_OK;
+}
+
+int dem_octree_stokeslet(const dem_tree_info* info, double theta_sq, const double* forces,
+                         double eta_const, const dem_correction* corr, double* vel, void* ws,
+                         size_t ws_bytes, dem_stream_t stream) {
+  return stokes_common(0, info, theta_sq, forces, eta_const, 0.0, corr, vel, ws, ws_bytes, stream);
+}
+
+int dem_octree_total_velocity(const dem_tree_info* info, double theta_sq, const double* forces,
+                              double eta_const, double stokes_factor, const dem_correction* corr,
+                              double* vel, void* ws, size_t ws_bytes, dem_stream_t stream) {
+  return stokes_common(1, info, theta_sq, forces, eta_const, stokes_factor, corr, vel, ws, ws_bytes,
+                       stream);
+}
+
+int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset, size_t* bytes) {
+  if (!info || !name || !offset || !bytes)
+    return dem::set_error(DEM_ERR_INVALID, "dem_octree_field: null argument");
+  TreeLayout l = carve(nullptr, info->n_capacity, info->max_nodes);
+  const size_t M = (size_t)info->max_nodes, N = (size_t)(info->n_capacity > 0 ? info->n_capacity : 1);
+  struct F {
+    const char* n;
+    const void* p;
+    size_t b;
+  } table[] = {
+      {"min", l.bmin, 3 * M * 8},          {"max", l.bmax, 3 * M * 8},
+      {"R3_sum", l.r3sum, M * 8},          {"R3_c", l.r3c, 3 * M * 8},
+      {"force_sum", l.fsum, 3 * M * 8},    {"fmom", l.fmom, 9 * M * 8},
+      {"max_radius", l.maxr, M * 8},       {"first_child", l.first_child, M * 4},
+      {"next", l.next, M * 4},             {"leaf_start", l.leaf_start, M * 4},
+      {"count", l.count, M * 4},           {"parents", l.parents, M * 4},
+      {"parent_first_child", l.pfc, M * 4}, {"leaf_indices", l.leaf_indices, N * 4},
+      {"perm", l.perm, N * 4},             {"particle_forces", l.pforces, 3 * N * 8},
+  };
+  for (const F& f : table)
+    if (strcmp(f.n, name) == 0) {
+      *offset = (size_t)reinterpret_cast<uintptr_t>(f.p);
+      *bytes = f.b;
+      return DEM_OK;
+    }
+  return dem::set_error(DEM_ERR_INVALID, "unknown octree field '%s'", name);
+}
+
+}  // extern "C"

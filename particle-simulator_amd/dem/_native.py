@@ -57,6 +57,28 @@ class DemCorrection(ctypes.Structure):
     ]
 
 
+DEM_TREE_MAX_LEVELS = 64
+
+
+class DemTreeInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_capacity", ctypes.c_int32),
+        ("max_nodes", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("node_count", ctypes.c_int32),
+        ("parent_count", ctypes.c_int32),
+        ("leaf_slot_count", ctypes.c_int32),
+        ("num_levels", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("level_offsets", ctypes.c_int32 * DEM_TREE_MAX_LEVELS),
+        ("periodic", ctypes.c_int32),
+        ("mpl", ctypes.c_int32),
+        ("forces_valid", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("L", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -77,6 +99,12 @@ SIGNATURES: dict[str, tuple] = {
     "dem_hash_pairs": (_I, [_P, _P, _I, _D, _I, _I, _P, _I, _P, _P, _S, _P]),
     "dem_euler_step": (_I, [_P, _P, _P, _P, _I, _D, _D, _D, _I, _P]),
     "dem_euler_step_total": (_I, [_P, _P, _I, _D, _D, _I, _P]),
+    "dem_octree_workspace_bytes": (_S, [_I, _I]),
+    "dem_octree_build": (_I, [_P, _P, _I, _D, _I, _I, _D, _P, _P, _S, _P]),
+    "dem_octree_forces": (_I, [_P, _D, _D, _P, _P, _S, _P]),
+    "dem_octree_stokeslet": (_I, [_P, _D, _P, _D, _P, _P, _P, _S, _P]),
+    "dem_octree_total_velocity": (_I, [_P, _D, _P, _D, _D, _P, _P, _P, _S, _P]),
+    "dem_octree_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "dem_profile_start": (_I, []),
     "dem_profile_stop": (_I, []),
     "dem_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

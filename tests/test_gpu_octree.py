@@ -1,0 +1,261 @@
+"""GPU octree (libdemhip ``dem_octree_*``) against the oracle's restatement of
+the reference's level-by-level build and walks.
+
+Structure: the node arrays (count, first_child, next, leaf_start, bounds,
+parents) and level offsets must be identical to the oracle's; leaf contents
+equal as sets; R^3 aggregates equal to float64 rounding.  Walks: the opening
+decisions are float64 and identical, so GPU (FP32 interaction arithmetic)
+and oracle (FP64) agree to FP32 accuracy: per-particle error (floor 1e-3 rms)
+max <= 1e-4 and field error <= 1e-5.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from dem.force_calculator import DirectDropletForceCalculator
+from dem.octree import FlatOctree, TreeDropletForceCalculator, compute_tree_stats
+from dem.octree.tree_stats import TreeStatistics, print_tree_stats
+from dem.workloads import make_system
+
+from ._errors import summary
+
+pytestmark = pytest.mark.gpu
+
+M = oracle.m_const(2.85, 3e5)
+ETA = oracle.eta_const(0.065)
+
+
+def _rel(ref, x, eps=1e-10):
+    d = np.linalg.norm(ref - x, axis=1)
+    r = np.linalg.norm(ref, axis=1)
+    return d / (r + np.mean(r) * eps)
+
+
+# --- the reference's own tests (tests/octree/test_flat_tree.py, test_force_tree.py)
+
+
+def test_root_contains_all_particles(gpu):
+    rng = np.random.default_rng(42)
+    n = 50
+    pos = rng.random((n, 3))
+    tree = FlatOctree(theta=0.5, mpl=1, num_particles=n)
+    tree.build(pos, rng.random(n) * 0.005 + 0.005, L=1.0, periodic=False)
+    assert tree.nodes.count.to_numpy()[0] == n
+    assert tree.num_particles[None] == n
+
+
+def test_node_count_growth_with_smaller_mpl(gpu):
+    rng = np.random.default_rng(0)
+    pos = rng.random((80, 3))
+    counts = []
+    for mpl in (8, 1):
+        tree = FlatOctree(theta=0.5, mpl=mpl, num_particles=80)
+        tree.build(pos, np.full(80, 0.005), L=1.0, periodic=False)
+        counts.append(int(tree.node_count[None]))
+    assert counts[1] > counts[0]
+
+
+def test_periodic_flag_set(gpu):
+    rng = np.random.default_rng(1)
+    pos = rng.random((10, 3))
+    tree = FlatOctree(theta=0.5, mpl=1, num_particles=10)
+    tree.build(pos, np.full(10, 0.01), L=1.0, periodic=False)
+    assert tree.periodic[None] == 0
+    tree.build(pos, np.full(10, 0.01), L=1.0, periodic=True)
+    assert tree.periodic[None] == 1
+
+
+def test_build_beyond_capacity_raises(gpu):
+    tree = FlatOctree(num_particles=64)
+    with pytest.raises(ValueError):
+        tree.build(np.random.default_rng(0).random((65, 3)), np.full(65, 0.01), L=1.0)
+
+
+def _case(seed=42):
+    rng = np.random.default_rng(seed)
+    n, L = 30, 0.01
+    return rng.random((n, 3)) * L, rng.random(n) * 30e-6 + 20e-6, L
+
+
+def test_forces_match_direct_within_tolerance(gpu):
+    pos, r, L = _case()
+    f_d = DirectDropletForceCalculator(num_particles=30, L=L, boundary_mode="open").calculate(pos, r)
+    f_t = TreeDropletForceCalculator(num_particles=30, theta=0.3, mpl=1, L=L,
+                                     periodic=False).calculate(pos, r)
+    assert np.mean(_rel(f_d, f_t)) < 0.05
+
+
+def test_convection_match_direct_within_tolerance(gpu):
+    pos, r, L = _case()
+    direct = DirectDropletForceCalculator(num_particles=30, L=L, boundary_mode="open")
+    v_d = direct.calculate_convection(pos, r, direct.calculate(pos, r))
+    tree = TreeDropletForceCalculator(num_particles=30, theta=0.3, mpl=1, L=L, periodic=False)
+    v_t = tree.calculate_convection(pos, r, tree.calculate(pos, r))
+    assert np.mean(_rel(v_d, v_t)) < 0.1
+
+
+def test_smaller_theta_decreases_error(gpu):
+    pos, r, L = _case()
+    f_d = DirectDropletForceCalculator(num_particles=30, L=L, boundary_mode="open").calculate(pos, r)
+    prev = np.inf
+    for theta in (0.8, 0.5, 0.3):
+        f_t = TreeDropletForceCalculator(num_particles=30, theta=theta, mpl=1, L=L,
+                                         periodic=False).calculate(pos, r)
+        err = float(np.mean(_rel(f_d, f_t)))
+        assert err <= prev + 1e-4
+        prev = err
+
+
+# --- tree statistics (tests/octree/test_tree_stats.py)
+
+
+def test_tree_stats_invariants(gpu, capsys):
+    rng = np.random.default_rng(0)
+    tree = FlatOctree(theta=0.5, mpl=4, num_particles=30)
+    tree.build(rng.random((30, 3)), np.full(30, 0.01), L=1.0, periodic=False)
+    st = compute_tree_stats(tree, build_time_ms=12.5, compute_time_ms=4.5)
+    assert isinstance(st, TreeStatistics)
+    assert st.total_particles == 30
+    assert st.internal_nodes + st.total_leaves == st.total_nodes
+    assert st.non_empty_leaves + st.empty_leaves == st.total_leaves
+    assert st.build_time_ms == 12.5 and st.compute_time_ms == 4.5
+    assert st.tree_depth >= 1
+    assert sum(c for _, c in st.nodes_per_level) == st.total_nodes
+    print_tree_stats(st)
+    out = capsys.readouterr().out
+    assert "Octree Statistics" in out and "Tree depth" in out
+
+
+# --- structural parity with the oracle
+
+
+def _structure(tree):
+    nc = int(tree.node_count[None])
+    nd = tree.nodes
+    return {
+        "count": nd.count.to_numpy()[:nc],
+        "first_child": nd.first_child.to_numpy()[:nc],
+        "next": nd.next.to_numpy()[:nc],
+        "leaf_start": nd.leaf_start.to_numpy()[:nc],
+        "bmin": np.stack([nd.min_x.to_numpy(), nd.min_y.to_numpy(), nd.min_z.to_numpy()], 1)[:nc],
+        "bmax": np.stack([nd.max_x.to_numpy(), nd.max_y.to_numpy(), nd.max_z.to_numpy()], 1)[:nc],
+        "r3sum": nd.R3_sum.to_numpy()[:nc],
+        "maxr": nd.max_radius.to_numpy()[:nc],
+        "parents": tree.parents.to_numpy()[: int(tree.parent_count[None])],
+        "pfc": tree.parent_first_child.to_numpy()[: int(tree.parent_count[None])],
+    }
+
+
+@pytest.mark.parametrize("mpl,periodic,max_nodes,seed", [
+    (1, False, None, 0), (4, True, None, 1), (16, True, None, 2), (1, True, 1500, 3),
+    (8, False, 300, 4)])
+def test_structure_identical_to_oracle(gpu, mpl, periodic, max_nodes, seed):
+    rng = np.random.default_rng(seed)
+    n, L = 3000, 1e-3
+    pos = rng.random((n, 3)) * L
+    if not periodic:
+        pos[:20] = rng.uniform(-0.2 * L, 1.2 * L, size=(20, 3))  # open box: strays outside
+    r = rng.uniform(2e-6, 8e-6, size=n)
+    tree = FlatOctree(theta=0.5, mpl=mpl, num_particles=n)
+    if max_nodes is not None:
+        tree.max_nodes = max_nodes
+        tree._info.max_nodes = max_nodes
+    tree.build(pos, r, L, periodic)
+    ref = oracle.OracleTree(pos, r, L, periodic, mpl=mpl, max_nodes=tree.max_nodes)
+    g = _structure(tree)
+    nc = ref.node_count
+    assert int(tree.node_count[None]) == nc and int(tree.parent_count[None]) == ref.parent_count
+    assert tree._parent_level_offsets == ref.level_offsets
+    assert int(tree.leaf_slot_count[None]) == ref.leaf_slot_count
+    for k in ("count", "first_child", "next", "leaf_start"):
+        assert np.array_equal(g[k], ref.arr[k][:nc]), k
+    assert np.array_equal(g["parents"], ref.arr["parents"][: ref.parent_count])
+    assert np.array_equal(g["pfc"], ref.arr["pfc"][: ref.parent_count])
+    assert np.array_equal(g["bmin"], ref.arr["bmin"][: 3 * nc].reshape(-1, 3))
+    assert np.array_equal(g["bmax"], ref.arr["bmax"][: 3 * nc].reshape(-1, 3))
+    np.testing.assert_allclose(g["r3sum"], ref.arr["r3sum"][:nc], rtol=1e-12, atol=0)
+    assert np.array_equal(g["maxr"], ref.arr["maxr"][:nc])
+    li = tree.leaf_indices.to_numpy()
+    for i in np.nonzero((g["first_child"] < 0) & (g["count"] > 0))[0]:
+        s, c = g["leaf_start"][i], g["count"][i]
+        assert set(li[s: s + c]) == set(ref.arr["leaf_indices"][s: s + c])
+
+
+# --- walk parity with the oracle
+
+
+def _tree_pair(n, mpl, periodic, seed, theta):
+    s = make_system(n, seed=seed, phi=0.02)
+    fc = TreeDropletForceCalculator(num_particles=n, theta=theta, mpl=mpl, eps_oil=s.eps_oil,
+                                    eta_oil=s.eta_oil, eta_water=s.eta_water, E=s.E, L=s.L,
+                                    periodic=periodic)
+    ref = oracle.OracleTree(s.positions, s.radii, s.L, periodic, mpl=mpl,
+                            max_nodes=fc.octree.max_nodes)
+    return s, fc, ref
+
+
+@pytest.mark.parametrize("mpl,periodic,theta", [(16, True, 0.5), (1, True, 0.35),
+                                                 (4, False, 0.5)])
+def test_walks_match_oracle(gpu, mpl, periodic, theta):
+    s, fc, ref = _tree_pair(4000, mpl, periodic, 7, theta)
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    Fo = ref.forces(theta, fc.m_const)
+    Vo = ref.stokeslet(theta, Fo, fc.eta_const)
+    ef, ev = summary(F, Fo), summary(V, Vo)
+    print(f"\n[tree mpl={mpl} per={periodic}] force {ef}\n  stokeslet {ev}")
+    assert ef["max"] <= 1e-4 and ef["norm"] <= 1e-5
+    assert ev["max"] <= 1e-3 and ev["norm"] <= 1e-5
+    F2, Vt = fc.calculate_forces_and_total_velocity(s.positions, s.radii, s.stokes_factor)
+    Vto = ref.total_velocity(theta, Fo, fc.eta_const, s.stokes_factor)
+    et = summary(Vt, Vto)
+    assert et["max"] <= 1e-3 and et["norm"] <= 1e-5
+
+
+def test_theta_zero_tree_equals_direct_on_gpu(gpu):
+    s = make_system(3000, seed=3, phi=0.02)
+    tree = TreeDropletForceCalculator(num_particles=s.n, theta=0.0, mpl=8, eps_oil=s.eps_oil,
+                                      eta_oil=s.eta_oil, E=s.E, L=s.L, periodic=True)
+    direct = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                          E=s.E, L=s.L, boundary_mode="periodic")
+    Ft, Vt = tree.calculate_forces_and_convection(s.positions, s.radii)
+    Fd, Vd = direct.calculate_forces_and_convection(s.positions, s.radii)
+    assert summary(Ft, Fd)["max"] <= 1e-5
+    assert summary(Vt, Vd)["norm"] <= 1e-5
+
+
+def test_comsol_correction_in_walks(gpu):
+    from dem.periodic_correction import COMSOLLatticeCorrection
+
+    corr = COMSOLLatticeCorrection.load_default()
+    s, _, ref = _tree_pair(2500, 8, True, 9, 0.5)
+    fc = TreeDropletForceCalculator(num_particles=s.n, theta=0.5, mpl=8, eps_oil=s.eps_oil,
+                                    eta_oil=s.eta_oil, E=s.E, L=s.L, periodic=True,
+                                    correction_grid_resolution=corr.grid_resolution)
+    fc.load_periodic_correction(corr, L_sim=s.L)
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    gd = corr.get_grid_data()
+    c = oracle.Correction(gd["grid_u"], gd["grid_v"], gd["grid_w"], gd["grid_min"], gd["grid_dx"],
+                          gd["Fz_comsol"], gd["L_comsol"] / s.L, gd["eta_comsol"] / s.eta_oil,
+                          fc.octree._corr.self_coeff)
+    Fo = ref.forces(0.5, fc.m_const)
+    Vo = ref.stokeslet(0.5, Fo, fc.eta_const, c)
+    e = summary(V, Vo)
+    print(f"\n[tree comsol] {e}")
+    assert e["max"] <= 1e-3 and e["norm"] <= 1e-5
+    _, Vt = fc.calculate_forces_and_total_velocity(s.positions, s.radii, s.stokes_factor)
+    Vto = ref.total_velocity(0.5, Fo, fc.eta_const, s.stokes_factor, c)  # no self term
+    assert summary(Vt, Vto)["norm"] <= 1e-5
+
+
+def test_capacity_regrowth_and_empty(gpu):
+    tc = TreeDropletForceCalculator(num_particles=100, theta=0.5, mpl=2, L=1e-3, periodic=True)
+    s = make_system(500, seed=1, phi=0.02)
+    tc.L = s.L
+    F, V = tc.calculate_forces_and_total_velocity(s.positions, s.radii, s.stokes_factor)
+    assert F.shape == (500, 3) and tc.num_particles == 500
+    tree = FlatOctree(num_particles=10)
+    tree.build(np.zeros((0, 3)), np.zeros(0), L=1.0)
+    assert tree.node_count[None] == 1 and tree.compute_forces(M).shape == (0, 3)

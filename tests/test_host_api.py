@@ -1,0 +1,150 @@
+"""CPU tests of the host-side drop-in API (no kernels): solver / solution /
+state semantics mirroring the reference's tests, and objects that must be
+constructible without a GPU."""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from dem.collision_detector.spatial_hash_collision_detector import _next_power_of_2
+from dem.octree import FlatOctree
+from dem.particle_state import DropletState
+from dem.solution import DropletSolution
+from dem.solver import EulerDropletSolver
+
+
+@dataclass
+class StubForceCalculator:
+    """Constant force (0,0,F), no convection (reference test_euler...py:22-37)."""
+
+    F: float = 1e-12
+    L: float = 1.0
+    eta_oil: float = 0.065
+    eta_water: float = 0.001
+    boundary_mode: str = "open"
+
+    def calculate_forces_and_convection(self, positions, radii):
+        n = len(radii)
+        forces = np.zeros((n, 3))
+        forces[:, 2] = self.F
+        return forces, np.zeros((n, 3))
+
+
+class StubPostProcessor:
+    def __init__(self):
+        self.stop_simulation = False
+
+    def update_solution(self, s):
+        self.solution = s
+
+    def update_live_plot(self):
+        pass
+
+    def live_plot(self):
+        pass
+
+    def finalize_live_plot(self):
+        pass
+
+
+class StubDetector:
+    """Reports one contact (0, 1) on the first call, none afterwards."""
+
+    def __init__(self):
+        self.calls = 0
+
+    def detect(self, positions, radii):
+        self.calls += 1
+        if self.calls == 1:
+            return True, np.array([[0, 1]], dtype=np.int32)
+        return False, np.empty((0, 2), dtype=np.int32)
+
+
+def _solver(F=1e-12, n=2, det=None):
+    state = DropletState(np.zeros((n, 3)), np.full(n, 50e-6), time=0.0)
+    sol = DropletSolution(initial_droplet_state=state, length=20)
+    return EulerDropletSolver(force_calculator=StubForceCalculator(F=F), solution=sol,
+                              post_processor=StubPostProcessor(), collision_detector=det)
+
+
+def test_drops_move_in_z():
+    s = _solver()
+    s.solve(dt=1e-3, total_time=1e-2)
+    p = s.solution.get_current_positions()
+    assert np.all(p[:, 2] > 0)
+    np.testing.assert_allclose(p[:, :2], 0.0)
+    # exact Euler update: 11 steps of F / (stokes * r) * dt
+    v = 1e-12 / (s.stokes_factor * 50e-6)
+    np.testing.assert_allclose(p[:, 2], 11 * v * 1e-3, rtol=1e-12)
+
+
+def test_solver_stops_when_flag_set():
+    s = _solver()
+    s.post_processor.stop_simulation = True
+    s.solve(dt=1e-3, total_time=1.0)
+    assert s.solution.current_step == 0
+
+
+def test_positions_wrap_in_periodic_mode():
+    s = _solver(F=1e-9)
+    s.force_calculator.boundary_mode = "periodic"
+    s.force_calculator.L = 0.001
+    s.solve(dt=1e-4, total_time=1e-3)
+    p = s.solution.get_current_positions()
+    assert np.all((p[:, 2] >= 0) & (p[:, 2] < 0.001))
+
+
+def test_collision_restarts_step_without_advancing_time():
+    s = _solver(n=3, det=StubDetector())
+    s.solution.initial_droplet_state.positions[2] = [1e-3, 0, 0]
+    s.solve(dt=1e-3, total_time=3e-3)
+    first = s.solution.get_first_solution()
+    assert first._next is s.solution
+    assert first.get_times()[-1][0] == 0.0  # merged at t = 0, before any step
+    assert s.solution.num_particles == 2
+    assert np.isclose(s.solution.radii[-1] ** 3, 2 * (50e-6) ** 3)
+
+
+def test_merge_and_chain_persistence(tmp_path: Path):
+    pos = np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.5, 0.5, 0.5]])
+    sol = DropletSolution(DropletState(pos, np.full(3, 0.05), time=0.0), length=5)
+    sol.collided_droplets = np.array([[0, 1]])
+    nxt = sol.generate_next_solution()
+    assert nxt.num_particles == 2 and np.isclose(nxt.radii[-1] ** 3, 2 * 0.05**3)
+    assert nxt._prev is sol and sol._next is nxt
+    path = tmp_path / "chain.npz"
+    nxt.save_step(0.5, np.ones((2, 3)))
+    sol.save_chain_to_file(str(path))
+    loaded = DropletSolution.load_chain_from_file(str(path))
+    np.testing.assert_allclose(loaded.radii, sol.radii)
+    np.testing.assert_allclose(loaded._next.trajectories, nxt.get_trajectories())
+
+
+def test_state_npz_and_xlsx_round_trip(tmp_path: Path):
+    rng = np.random.default_rng(0)
+    st = DropletState(rng.random((7, 3)), rng.random(7) * 1e-5, time=2.5)
+    st.save(str(tmp_path / "s.npz"))
+    back = DropletState(filename=str(tmp_path / "s.npz"))
+    np.testing.assert_array_equal(back.positions, st.positions)
+    assert back.time == 2.5
+    st.export_to_xlsx(str(tmp_path / "s.xlsx"))
+    x = DropletState(filename=str(tmp_path / "s.xlsx"))
+    np.testing.assert_array_equal(x.positions, st.positions)
+    np.testing.assert_array_equal(x.radii, st.radii)
+    assert x.time == 0
+
+
+def test_next_power_of_2():
+    assert [_next_power_of_2(k) for k in (1, 2, 3, 5, 16, 17)] == [1, 2, 4, 8, 16, 32]
+
+
+def test_flat_octree_params_without_gpu():
+    tree = FlatOctree(theta=0.5, mpl=1, num_particles=16)
+    tree.update_params(theta=0.3, mpl=4)
+    assert tree.theta == 0.3 and tree.mpl == 4
+    assert tree.theta_sq[None] == 0.3 * 0.3 and tree.mpl_field[None] == 4
+    assert tree.max_nodes == max(4 * 64 // 1, 1024)
+    assert tree.nodes.count.to_numpy().shape == (tree.max_nodes,)
