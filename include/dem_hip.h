@@ -114,6 +114,16 @@ int dem_direct_forces_and_convection_f64(const double* pos, const double* radii,
                                          double* forces, double* velocities,
                                          dem_stream_t stream);
 
+/* FP64 oracle restricted to the targets listed in targets[0..nt): forces of
+ * those targets (rows of F, nt x 3) over all n sources and, when V is not
+ * NULL, their Oseen velocities (nt x 3) from the given forces of all n
+ * sources.  Checker for sizes where the full FP64 sum is too slow. */
+int dem_direct_f64_targets(const double* pos, const double* radii, int n,
+                           const dem_phys* phys, const dem_correction* corr,
+                           const int32_t* targets, int nt,
+                           const double* sources_forces, double* F, double* V,
+                           dem_stream_t stream);
+
 /* Tuning knob of the direct kernels (j-split factor and FP32 tile length);
  * 0 keeps the built-in heuristic.  Returns the previous values packed as
  * (split << 16) | tile. */

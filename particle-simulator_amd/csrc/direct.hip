@@ -474,9 +474,12 @@ __device__ __forceinline__ double mic64(double d, double L, int periodic) {
 __global__ __launch_bounds__(256) void k_f64_forces(const double* __restrict__ pos,
                                                     const double* __restrict__ r, int n, double m,
                                                     double L, int periodic,
+                                                    const int* __restrict__ tgt, int nt,
                                                     double* __restrict__ F) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  // targets: tgt[k] (k < nt), or k itself when tgt is NULL; output row k
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nt) return;
+  const int i = tgt ? tgt[k] : k;
   const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
   const double ri = r[i];
   double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -497,9 +500,9 @@ __global__ __launch_bounds__(256) void k_f64_forces(const double* __restrict__ p
       fz += M * (2.0 * dz2 - 3.0 * dx2 - 3.0 * dy2) * dz;
     }
   }
-  F[3 * (size_t)i] = fx;
-  F[3 * (size_t)i + 1] = fy;
-  F[3 * (size_t)i + 2] = fz;
+  F[3 * (size_t)k] = fx;
+  F[3 * (size_t)k + 1] = fy;
+  F[3 * (size_t)k + 2] = fz;
 }
 
 struct Corr64Args {
@@ -527,9 +530,11 @@ __device__ double interp64(const double* g, int comp, int n, int ix, int iy, int
 __global__ __launch_bounds__(256) void k_f64_conv(const double* __restrict__ pos,
                                                   const double* __restrict__ F, int n, double eta,
                                                   double L, int periodic, Corr64Args c,
+                                                  const int* __restrict__ tgt, int nt,
                                                   double* __restrict__ V) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nt) return;
+  const int i = tgt ? tgt[k] : k;
   const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
   double vx = 0.0, vy = 0.0, vz = 0.0;
   for (int j = 0; j < n; ++j) {
@@ -579,9 +584,9 @@ __global__ __launch_bounds__(256) void k_f64_conv(const double* __restrict__ pos
     vy += c.self_coeff * F[3 * (size_t)i + 1];
     vz += c.self_coeff * F[3 * (size_t)i + 2];
   }
-  V[3 * (size_t)i] = vx;
-  V[3 * (size_t)i + 1] = vy;
-  V[3 * (size_t)i + 2] = vz;
+  V[3 * (size_t)k] = vx;
+  V[3 * (size_t)k + 1] = vy;
+  V[3 * (size_t)k + 2] = vz;
 }
 
 // ---------------------------------------------------------------- host drivers
@@ -756,11 +761,48 @@ int dem_direct_forces_and_convection_f64(const double* pos, const double* radii,
   if (n == 0) return DEM_OK;
   hipStream_t st = dem::as_stream(stream);
   hipLaunchKernelGGL(k_f64_forces, dim3((n + 255) / 256), dim3(256), 0, st, pos, radii, n,
-                     ph->m_const, ph->L, ph->periodic, F);
+                     ph->m_const, ph->L, ph->periodic, (const int*)nullptr, n, F);
   DEM_CHECK_LAUNCH("k_f64_forces");
   hipLaunchKernelGGL(k_f64_conv, dim3((n + 255) / 256), dim3(256), 0, st, pos, (const double*)F, n,
-                     ph->eta_const, ph->L, ph->periodic, c, V);
+                     ph->eta_const, ph->L, ph->periodic, c, (const int*)nullptr, n, V);
   DEM_CHECK_LAUNCH("k_f64_conv");
+  return DEM_OK;
+}
+
+int dem_direct_f64_targets(const double* pos, const double* radii, int n, const dem_phys* ph,
+                           const dem_correction* corr, const int32_t* targets, int nt,
+                           const double* sources_forces, double* F, double* V,
+                           dem_stream_t stream) {
+  int rc = check_common(pos, radii, n, ph);
+  if (rc) return rc;
+  if (nt < 0 || (nt > 0 && (!targets || !F)))
+    return dem::set_error(DEM_ERR_INVALID, "bad target list / output");
+  if (V && !sources_forces)
+    return dem::set_error(DEM_ERR_INVALID, "velocities need the forces of all sources");
+  Corr64Args c{};
+  if (corr && corr->enabled) {
+    if (!corr->grid64 || corr->n < 2)
+      return dem::set_error(DEM_ERR_INVALID, "FP64 oracle needs the float64 correction grid");
+    c.grid = corr->grid64;
+    c.n = corr->n;
+    c.enabled = 1;
+    c.gmin = corr->grid_min;
+    c.inv_dx = corr->inv_dx;
+    c.Fz_inv = corr->Fz_inv;
+    c.L_ratio = corr->L_ratio;
+    c.eta_ratio = corr->eta_ratio;
+    c.self_coeff = corr->self_coeff;
+  }
+  if (nt == 0 || n == 0) return DEM_OK;
+  hipStream_t st = dem::as_stream(stream);
+  hipLaunchKernelGGL(k_f64_forces, dim3((nt + 255) / 256), dim3(256), 0, st, pos, radii, n,
+                     ph->m_const, ph->L, ph->periodic, (const int*)targets, nt, F);
+  DEM_CHECK_LAUNCH("k_f64_forces");
+  if (V) {
+    hipLaunchKernelGGL(k_f64_conv, dim3((nt + 255) / 256), dim3(256), 0, st, pos, sources_forces,
+                       n, ph->eta_const, ph->L, ph->periodic, c, (const int*)targets, nt, V);
+    DEM_CHECK_LAUNCH("k_f64_conv");
+  }
   return DEM_OK;
 }
 

@@ -147,6 +147,19 @@ class DirectDropletForceCalculator(ForceCalculator):
                      _device.ptr(V), _device.stream())
         return F, V
 
+    def f64_targets_device(self, pos_d, radii_d, targets_d, sources_forces_d=None):
+        """FP64 checker on a target subset: forces of the targets and, given the
+        forces of all sources, their Oseen velocities (device tensors)."""
+        nt = int(targets_d.shape[0])
+        F = _device.empty((nt, 3))
+        V = _device.empty((nt, 3)) if sources_forces_d is not None else None
+        ph = self._phys()
+        _native.call("dem_direct_f64_targets", _device.ptr(pos_d), _device.ptr(radii_d),
+                     int(pos_d.shape[0]), _byref(ph), self._corr_ref(), _device.ptr(targets_d),
+                     nt, _device.ptr(sources_forces_d), _device.ptr(F), _device.ptr(V),
+                     _device.stream())
+        return F, V
+
     # ------------------------------------------------------------------ reference API
 
     def calculate(self, positions: np.ndarray, radii: np.ndarray) -> np.ndarray:

@@ -28,22 +28,17 @@ class UniformDropletGenerator(ParticleGenerator):
         self._rng = np.random.default_rng(seed)
 
     def _overlapping(self, pos, radii):
-        """Indices of droplets overlapping an earlier one (open box)."""
-        n = len(radii)
-        bad = np.zeros(n, dtype=bool)
-        cell = max(2 * float(np.max(radii)) + self.minimum_distance, 1e-30)
-        keys = np.floor((pos - self.coord_range[0]) / cell).astype(np.int64)
-        buckets: dict[tuple, list[int]] = {}
-        for i in range(n):
-            k = tuple(keys[i])
-            for dx in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dz in (-1, 0, 1):
-                        for j in buckets.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
-                            if np.linalg.norm(pos[i] - pos[j]) < radii[i] + radii[j] + self.minimum_distance:
-                                bad[i] = True
-            buckets.setdefault(k, []).append(i)
-        return np.nonzero(bad)[0]
+        """Indices of droplets closer than r_i + r_j + minimum_distance to an
+        earlier droplet (open box)."""
+        from scipy.spatial import cKDTree
+
+        reach = 2 * float(np.max(radii)) + self.minimum_distance
+        pairs = cKDTree(pos).query_pairs(reach, output_type="ndarray")
+        if len(pairs) == 0:
+            return np.empty(0, dtype=np.int64)
+        d = np.linalg.norm(pos[pairs[:, 0]] - pos[pairs[:, 1]], axis=1)
+        close = d < radii[pairs[:, 0]] + radii[pairs[:, 1]] + self.minimum_distance
+        return np.unique(pairs[close].max(axis=1))
 
     def generate(self):
         lo, hi = self.coord_range

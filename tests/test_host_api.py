@@ -148,3 +148,31 @@ def test_flat_octree_params_without_gpu():
     assert tree.theta_sq[None] == 0.3 * 0.3 and tree.mpl_field[None] == 4
     assert tree.max_nodes == max(4 * 64 // 1, 1024)
     assert tree.nodes.count.to_numpy().shape == (tree.max_nodes,)
+
+
+def test_taichi_stand_in_and_script_inputs(tmp_path):
+    import importlib
+    import sys
+
+    from dem import run_script
+
+    run_script.ensure_paths()
+    ti = importlib.import_module("taichi")
+    ti.init(arch=ti.cpu, cpu_max_num_threads=16, default_fp=ti.f64)
+    assert ti.sync() is None
+    # the box the scripts compute (main.py:35-40)
+    r = np.array([2.5e-6, 7.5e-6])
+    assert run_script.script_box(1000, 0.02) == np.cbrt(
+        (np.pi * 1000 * np.sum(r) * np.sum(np.square(r))) / (3 * 0.02))
+    sys.modules.pop("taichi", None)
+
+
+def test_generator_produces_non_overlapping_droplets():
+    from dem.particle_generator import UniformDropletGenerator
+
+    g = UniformDropletGenerator(coord_range=(0, 2e-4), radii_range=(2.5e-6, 7.5e-6),
+                                num_particles=300, minimum_distance=1e-6, seed=1)
+    st = g.generate()
+    d = np.linalg.norm(st.positions[:, None] - st.positions[None], axis=2)
+    np.fill_diagonal(d, np.inf)
+    assert np.all(d >= st.radii[:, None] + st.radii[None] + 1e-6 - 1e-18)
