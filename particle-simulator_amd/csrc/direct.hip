@@ -32,6 +32,7 @@
 #include <math.h>
 
 #include "dem_common.h"
+#include "pair_math.h"
 #include "profile.h"
 
 namespace {
@@ -39,7 +40,6 @@ namespace {
 constexpr int WG = 256;
 constexpr double kTwo32 = 4294967296.0;
 constexpr double kSigma = 0x1p-22;   // work unit / fixed unit
-constexpr float kSigma2f = 0x1p-44f; // kSigma^2
 constexpr double kSigma3 = 0x1p-66;
 
 // Phase-1 source / target record (32 bytes, read whole by the force kernel):
@@ -260,23 +260,9 @@ __global__ __launch_bounds__(WG) void k_force_tile(const Rec* __restrict__ rec, 
 #pragma unroll 4
     for (int j = jt; j < je; ++j) {
       const Rec sj = rec[j];  // wave-uniform: scalar loads
-      const float dx = (float)(int32_t)(sj.x - xi);
-      const float dy = (float)(int32_t)(sj.y - yi);
-      const float dz = (float)(int32_t)(sj.z - zi);
-      const float dz2 = dz * dz;
-      const float dxy2 = dx * dx + dy * dy;
-      const float r2 = (dxy2 + dz2) * kSigma2f;  // work units^2
-      const float ir = __builtin_amdgcn_rsqf(r2);
-      const float ir2 = ir * ir;
-      const float ir7 = (ir2 * ir2) * (ir2 * ir);
-      const float rs = ri + sj.r;
-      const float s7 = (r2 > rs * rs) ? sj.r3 * ir7 : 0.f;  // contact mask; also drops j == i
-      const float axy = 4.f * dz2 - dxy2;
-      const float azz = 2.f * dz2 - 3.f * dxy2;
-      const float txy = s7 * axy;
-      fx += txy * dx;
-      fy += txy * dy;
-      fz += (s7 * azz) * dz;
+      // contact mask |d| > r_i + r_j also drops j == i
+      dem::dipole_pair<false>(dem::diff32(sj.x, xi), dem::diff32(sj.y, yi),
+                              dem::diff32(sj.z, zi), ri, sj.r, sj.r3, fx, fy, fz);
     }
     ax += (double)fx;
     ay += (double)fy;
@@ -323,13 +309,6 @@ struct CorrArgs {
   double gmin, inv_dx, L_ratio;
 };
 
-// Clamped grid coordinate: index in [0, n-2] and fraction (reference :141-157).
-__device__ __forceinline__ void grid_coord(float g, float nmax, int n, int& i, float& t) {
-  g = fminf(fmaxf(g, 0.f), nmax);
-  i = min((int)floorf(g), n - 2);
-  t = g - (float)i;
-}
-
 template <bool CORR>
 __global__ __launch_bounds__(WG) void k_conv_tile(const RecC* __restrict__ rec, int n, int jchunk,
                                                   int tile, const DevScales* __restrict__ scp,
@@ -342,12 +321,13 @@ __global__ __launch_bounds__(WG) void k_conv_tile(const RecC* __restrict__ rec, 
   const unsigned long long xi = ((unsigned long long)rec[ii].xh << 32) | rec[ii].xl;
   const unsigned long long yi = ((unsigned long long)rec[ii].yh << 32) | rec[ii].yl;
   const unsigned long long zi = ((unsigned long long)rec[ii].zh << 32) | rec[ii].zl;
-  // Grid coordinate of x_i - x_j = -U D  is  a * D + b.
-  float ga = 0.f, gb = 0.f;
+  dem::CorrGrid cg{};
   if (CORR) {
-    const double U = scp->U;
-    ga = (float)(-U * ca.L_ratio * ca.inv_dx);
-    gb = (float)(-ca.gmin * ca.inv_dx);
+    cg.grid = ca.grid;
+    cg.n = ca.n;
+    cg.nmax = ca.nmax;
+    cg.ga = (float)(-scp->U * ca.L_ratio * ca.inv_dx);
+    cg.gb = (float)(-ca.gmin * ca.inv_dx);
   }
   double ax = 0.0, ay = 0.0, az = 0.0;
   double cx = 0.0, cy = 0.0, cz = 0.0;
@@ -358,61 +338,14 @@ __global__ __launch_bounds__(WG) void k_conv_tile(const RecC* __restrict__ rec, 
 #pragma unroll 4
     for (int j = jt; j < je; ++j) {
       const RecC sj = rec[j];
-      // 64-bit minimum-image difference; its high word (floor) in fixed units
-      const float dx = (float)(int32_t)(((((unsigned long long)sj.xh << 32) | sj.xl) - xi) >> 32);
-      const float dy = (float)(int32_t)(((((unsigned long long)sj.yh << 32) | sj.yl) - yi) >> 32);
-      const float dz = (float)(int32_t)(((((unsigned long long)sj.zh << 32) | sj.zl) - zi) >> 32);
-      const float r2 = dx * dx + dy * dy + dz * dz;  // fixed units^2
-      const bool other = r2 > 0.f;                     // drops j == i
-      const float ir = other ? __builtin_amdgcn_rsqf(r2) : 0.f;
-      const float dot = dx * sj.fx + dy * sj.fy + dz * sj.fz;
-      const float t = dot * (ir * ir);
-      vx += ir * (sj.fx + t * dx);
-      vy += ir * (sj.fy + t * dy);
-      vz += ir * (sj.fz + t * dz);
-      if (CORR) {
-        int gx, gy, gz;
-        float tx, ty, tz;
-        grid_coord(ga * dx + gb, ca.nmax, ca.n, gx, tx);
-        grid_coord(ga * dy + gb, ca.nmax, ca.n, gy, ty);
-        grid_coord(ga * dz + gb, ca.nmax, ca.n, gz, tz);
-        // Eight trilinear weights w[a][b][c] of fractions (tx, ty, tz).
-        float w[2][2][2];
-        const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
-        w[0][0][0] = ox * oy * oz; w[0][0][1] = ox * oy * tz;
-        w[0][1][0] = ox * ty * oz; w[0][1][1] = ox * ty * tz;
-        w[1][0][0] = tx * oy * oz; w[1][0][1] = tx * oy * tz;
-        w[1][1][0] = tx * ty * oz; w[1][1][1] = tx * ty * tz;
-        const int nn = ca.n;
-        // G_z: grid at (x, y, z) -> (u, v, w)
-        float gzx = 0.f, gzy = 0.f, gzz = 0.f;
-        // G_x: grid at (z, y, x) -> (w, v, u)
-        float gxx = 0.f, gxy = 0.f, gxz = 0.f;
-        // G_y: grid at (x, z, y) -> (u, w, v)
-        float gyx = 0.f, gyy = 0.f, gyz = 0.f;
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              const float4 g1 = ca.grid[((gx + a) * nn + (gy + b)) * nn + (gz + c)];
-              const float w1 = w[a][b][c];
-              gzx += w1 * g1.x; gzy += w1 * g1.y; gzz += w1 * g1.z;
-              // lookup at (gz+a, gy+b, gx+c) carries fractions (tz, ty, tx)
-              const float4 g2 = ca.grid[((gz + a) * nn + (gy + b)) * nn + (gx + c)];
-              const float w2 = w[c][b][a];
-              gxx += w2 * g2.z; gxy += w2 * g2.y; gxz += w2 * g2.x;
-              // lookup at (gx+a, gz+b, gy+c) carries fractions (tx, tz, ty)
-              const float4 g3 = ca.grid[((gx + a) * nn + (gz + b)) * nn + (gy + c)];
-              const float w3 = w[a][c][b];
-              gyx += w3 * g3.x; gyy += w3 * g3.z; gyz += w3 * g3.y;
-            }
-        const float fxm = other ? sj.fx : 0.f, fym = other ? sj.fy : 0.f, fzm = other ? sj.fz : 0.f;
-        ux += gxx * fxm + gyx * fym + gzx * fzm;
-        uy += gxy * fxm + gyy * fym + gzy * fzm;
-        uz += gxz * fxm + gyz * fym + gzz * fzm;
-      }
+      const float dx = dem::diff64(sj.xh, sj.xl, xi);
+      const float dy = dem::diff64(sj.yh, sj.yl, yi);
+      const float dz = dem::diff64(sj.zh, sj.zl, zi);
+      const bool other = dx * dx + dy * dy + dz * dz > 0.f;  // drops j == i
+      dem::oseen_pair(dx, dy, dz, sj.fx, sj.fy, sj.fz, other, vx, vy, vz);
+      if (CORR)
+        dem::corr_lookup(cg, dx, dy, dz, other ? sj.fx : 0.f, other ? sj.fy : 0.f,
+                         other ? sj.fz : 0.f, ux, uy, uz);
     }
     ax += (double)vx;
     ay += (double)vy;

@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include "dem_common.h"
+#include "pair_math.h"
 #include "profile.h"
 #include "scan.h"
 #include "sort.h"
@@ -43,7 +44,6 @@ constexpr int kKeyLevels = 21;  // 3 * 21 = 63 key bits
 constexpr int kMaxLevels = 40;  // reference MAX_LEVELS
 constexpr double kTwo32 = 4294967296.0;
 constexpr double kSigma = 0x1p-22;
-constexpr float kSigma2f = 0x1p-44f;
 constexpr double kSigma3 = 0x1p-66;
 constexpr double kR3Skip = 1e-30;   // flat_tree.py:29
 constexpr double kFsqSkip = 1e-60;  // flat_tree.py:32
@@ -620,56 +620,9 @@ __global__ void k_sorted_forces(int n, const int* __restrict__ perm, const doubl
 // ---------------------------------------------------------------- walks
 
 struct CorrArgs {
-  const float4* grid;
-  int n, enabled;
-  float nmax;
-  float ga, gb;           // grid coordinate of x_i - x_j = -U D  is  ga * D + gb
+  dem::CorrGrid g;
+  int enabled;
 };
-
-__device__ __forceinline__ void grid_coord(float g, float nmax, int n, int& i, float& t) {
-  g = fminf(fmaxf(g, 0.f), nmax);
-  i = min((int)floorf(g), n - 2);
-  t = g - (float)i;
-}
-
-// Three permuted trilinear lookups applied to force fh (reference :413-457).
-__device__ __forceinline__ void corr_lookup(const CorrArgs& ca, float dx, float dy, float dz,
-                                            float fx, float fy, float fz, float& ux, float& uy,
-                                            float& uz) {
-  int gx, gy, gz;
-  float tx, ty, tz;
-  grid_coord(ca.ga * dx + ca.gb, ca.nmax, ca.n, gx, tx);
-  grid_coord(ca.ga * dy + ca.gb, ca.nmax, ca.n, gy, ty);
-  grid_coord(ca.ga * dz + ca.gb, ca.nmax, ca.n, gz, tz);
-  float w[2][2][2];
-  const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
-  w[0][0][0] = ox * oy * oz; w[0][0][1] = ox * oy * tz;
-  w[0][1][0] = ox * ty * oz; w[0][1][1] = ox * ty * tz;
-  w[1][0][0] = tx * oy * oz; w[1][0][1] = tx * oy * tz;
-  w[1][1][0] = tx * ty * oz; w[1][1][1] = tx * ty * tz;
-  const int nn = ca.n;
-  float gzx = 0.f, gzy = 0.f, gzz = 0.f, gxx = 0.f, gxy = 0.f, gxz = 0.f, gyx = 0.f, gyy = 0.f,
-        gyz = 0.f;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const float4 g1 = ca.grid[((gx + a) * nn + (gy + b)) * nn + (gz + c)];
-        const float w1 = w[a][b][c];
-        gzx += w1 * g1.x; gzy += w1 * g1.y; gzz += w1 * g1.z;
-        const float4 g2 = ca.grid[((gz + a) * nn + (gy + b)) * nn + (gx + c)];
-        const float w2 = w[c][b][a];
-        gxx += w2 * g2.z; gxy += w2 * g2.y; gxz += w2 * g2.x;
-        const float4 g3 = ca.grid[((gx + a) * nn + (gz + b)) * nn + (gy + c)];
-        const float w3 = w[a][c][b];
-        gyx += w3 * g3.x; gyy += w3 * g3.z; gyz += w3 * g3.y;
-      }
-  ux += gxx * fx + gyx * fy + gzx * fz;
-  uy += gxy * fx + gyy * fy + gzy * fz;
-  uz += gxz * fx + gyz * fy + gzz * fz;
-}
 
 struct WalkArgs {
   const NodeW* nw;
@@ -712,22 +665,9 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
       for (int j = nd.mstart; j < e; ++j) {
         if (j == t) continue;
         const RecF sj = wa.srec[j];
-        const float dx = (float)(int32_t)(sj.x - me.x);
-        const float dy = (float)(int32_t)(sj.y - me.y);
-        const float dz = (float)(int32_t)(sj.z - me.z);
-        const float dz2 = dz * dz;
-        const float dxy2 = dx * dx + dy * dy;
-        const float r2 = (dxy2 + dz2) * kSigma2f;
-        const float rs = me.r + sj.r;
-        if (r2 >= rs * rs) {  // tree leaves use >= (:835)
-          const float ir = __builtin_amdgcn_rsqf(r2);
-          const float ir2 = ir * ir;
-          const float s7 = sj.r3 * ((ir2 * ir2) * (ir2 * ir));
-          const float txy = s7 * (4.f * dz2 - dxy2);
-          fx += txy * dx;
-          fy += txy * dy;
-          fz += (s7 * (2.f * dz2 - 3.f * dxy2)) * dz;
-        }
+        // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
+        dem::dipole_pair<true>(dem::diff32(sj.x, me.x), dem::diff32(sj.y, me.y),
+                               dem::diff32(sj.z, me.z), me.r, sj.r, sj.r3, fx, fy, fz);
       }
       ax += (double)fx;
       ay += (double)fy;
@@ -742,18 +682,12 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
     d2 = d2 + dy * dy;
     d2 = d2 + dz * dz;
     if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
-      const float fdx = (float)(dx * sc.inv_U), fdy = (float)(dy * sc.inv_U),
-                  fdz = (float)(dz * sc.inv_U);
-      const float dz2 = fdz * fdz;
-      const float dxy2 = fdx * fdx + fdy * fdy;
-      const float r2 = (dxy2 + dz2) * kSigma2f;
-      const float ir = __builtin_amdgcn_rsqf(r2);
-      const float ir2 = ir * ir;
-      const float s7 = nd.r3w * ((ir2 * ir2) * (ir2 * ir));
-      const float txy = s7 * (4.f * dz2 - dxy2);
-      ax += (double)(txy * fdx);
-      ay += (double)(txy * fdy);
-      az += (double)((s7 * (2.f * dz2 - 3.f * dxy2)) * fdz);
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      dem::dipole_monopole((float)(dx * sc.inv_U), (float)(dy * sc.inv_U),
+                           (float)(dz * sc.inv_U), nd.r3w, fx, fy, fz);
+      ax += (double)fx;
+      ay += (double)fy;
+      az += (double)fz;
       node = nd.next;
     } else {
       node = nd.first_child;
@@ -805,20 +739,15 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
         if (j == t) continue;
         const RecF sj = wa.srec[j];
         // D = x_j - x_i (fixed units); the Oseen tensor is even in D
-        const float dx = (float)(int32_t)(sj.x - me.x);
-        const float dy = (float)(int32_t)(sj.y - me.y);
-        const float dz = (float)(int32_t)(sj.z - me.z);
+        const float dx = dem::diff32(sj.x, me.x);
+        const float dy = dem::diff32(sj.y, me.y);
+        const float dz = dem::diff32(sj.z, me.z);
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float rs = me.r + sj.r;
-        if (r2 * kSigma2f >= rs * rs) {  // contact mask >= (:1055)
+        if (dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r)) {  // contact mask >= (:1055)
           const float Fx = wa.sF[3 * (size_t)j], Fy = wa.sF[3 * (size_t)j + 1],
                       Fz = wa.sF[3 * (size_t)j + 2];
-          const float ir = __builtin_amdgcn_rsqf(r2);
-          const float tt = (dx * Fx + dy * Fy + dz * Fz) * (ir * ir);
-          vx += ir * (Fx + tt * dx);
-          vy += ir * (Fy + tt * dy);
-          vz += ir * (Fz + tt * dz);
-          if (CORR) corr_lookup(ca, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+          dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, true, vx, vy, vz);
+          if (CORR) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
         }
       }
       ax += (double)vx;
@@ -844,13 +773,10 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       // R = centre - x_i in fixed units (sign irrelevant for the monopole)
       const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
                   Rz = (float)(-dz * sc.inv_U);
-      const float r2 = Rx * Rx + Ry * Ry + Rz * Rz;
-      const float ir = __builtin_amdgcn_rsqf(r2);
+      float vx = 0.f, vy = 0.f, vz = 0.f;
+      dem::oseen_pair(Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, true, vx, vy, vz);
+      const float ir = __builtin_amdgcn_rsqf(Rx * Rx + Ry * Ry + Rz * Rz);
       const float ir2 = ir * ir;
-      const float tt = (Rx * nf.fx + Ry * nf.fy + Rz * nf.fz) * ir2;
-      float vx = ir * (nf.fx + tt * Rx);
-      float vy = ir * (nf.fy + tt * Ry);
-      float vz = ir * (nf.fz + tt * Rz);
       // dipole correction with r = x_i - centre = -R (:1242-1265), written with
       // the unit vector n = r / |r| to stay inside the FP32 range
       const float nx = -Rx * ir, ny = -Ry * ir, nz = -Rz * ir;
@@ -871,7 +797,7 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       az += (double)vz;
       if (CORR) {
         float ux2 = 0.f, uy2 = 0.f, uz2 = 0.f;
-        corr_lookup(ca, Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, ux2, uy2, uz2);
+        dem::corr_lookup(ca.g, Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, ux2, uy2, uz2);
         cx += (double)ux2;
         cy += (double)uy2;
         cz += (double)uz2;
@@ -968,13 +894,12 @@ int corr_args(const dem_correction* corr, const dem_tree_info* info, CorrArgs& c
   if (!corr || !corr->enabled) return DEM_OK;
   if (!corr->grid || corr->n < 2)
     return dem::set_error(DEM_ERR_INVALID, "correction enabled without a float grid");
-  ca.grid = reinterpret_cast<const float4*>(corr->grid);
-  ca.n = corr->n;
+  ca.g.grid = reinterpret_cast<const float4*>(corr->grid);
+  ca.g.n = corr->n;
   ca.enabled = 1;
-  ca.nmax = (float)(corr->n - 1);
-  // fixed unit U is known on the host for periodic boxes; open boxes read it on the device
-  ca.ga = 0.f;
-  ca.gb = (float)(-corr->grid_min * corr->inv_dx);
+  ca.g.nmax = (float)(corr->n - 1);
+  ca.g.ga = 0.f;  // set from the device-side fixed unit below
+  ca.g.gb = (float)(-corr->grid_min * corr->inv_dx);
   common = corr->Fz_inv * corr->eta_ratio * corr->L_ratio;
   self_coeff = corr->self_coeff;
   (void)info;
@@ -1174,7 +1099,7 @@ static int stokes_common(int mode, const dem_tree_info* info, double theta_sq, c
     float ga = 0.f;
     DEM_CHECK_HIP(hipMemcpyAsync(&ga, tmp, sizeof(float), hipMemcpyDeviceToHost, st));
     DEM_CHECK_HIP(hipStreamSynchronize(st));
-    ca.ga = ga;
+    ca.g.ga = ga;
   }
   WalkArgs wa = walk_args(info, l, theta_sq);
   const dim3 g(nb256(info->n)), b(256);
