@@ -235,6 +235,81 @@ int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset
                      size_t* bytes);
 
 /* ------------------------------------------------------------------------ */
+/* Batched ensembles: many independent periodic systems stepped together     */
+/* ------------------------------------------------------------------------ */
+
+/* The reference runs independent systems one after another (main_multi.py:
+ * 43-71, a host loop over EulerDropletSolver.solve, euler_droplet_solver.py:
+ * 43-159, with SpatialHashCollisionDetector.detect and
+ * DirectDropletForceCalculator.calculate_forces_and_convection per step).
+ * The batch keeps that per-system semantics — contact check first; on contact
+ * the system does not advance and the caller merges it; otherwise force,
+ * Oseen velocity and the Euler step with periodic wrap — and runs one global
+ * step for every system in five launches (contact flags, force items, force
+ * reduction, Oseen items, Oseen reduction + Euler + repack) whose sizes are
+ * fixed at the capacities, so one captured HIP graph stays valid while merges
+ * shrink the systems.  Each system's sums use the same source split as the
+ * single-system kernels run with split dem_batch_split(n, jlen), so results are
+ * bit-identical to dem_direct_forces_and_convection + dem_euler_step. */
+typedef struct dem_batch_ensemble {
+  int32_t offset;   /* first droplet of the system in the concatenated arrays  */
+  int32_t n;        /* droplets now (<= capacity)                              */
+  int32_t capacity; /* droplets reserved; also fixes the hash table size
+                       next_pow2(2*capacity) (spatial_hash...py:42-43)         */
+  int32_t steps;    /* Euler steps to take; the system stops after that many   */
+  double L;         /* periodic box edge (m)                                   */
+  double dt;        /* time step (s)                                           */
+  double stokes_factor;
+  double m_const;
+  double eta_const;
+} dem_batch_ensemble;
+
+typedef struct dem_batch_tuning {
+  int32_t block;      /* targets per work item: 64, 128 or 256                  */
+  int32_t jlen;       /* sources per work item (source split); <= 0: no split   */
+  int32_t force_tile; /* FP32 sum length before the float64 hand-off; 0: 32     */
+  int32_t conv_tile;  /* same for the Oseen sum; 0: 16                          */
+} dem_batch_tuning;
+
+typedef struct dem_batch* dem_batch_t; /* opaque host handle */
+
+/* Source split of a system of n droplets for a given jlen (also the `split`
+ * to pass to dem_direct_set_tuning for bit-identical single-system sums). */
+int dem_batch_split(int n, int jlen);
+
+size_t dem_batch_workspace_bytes(const dem_batch_ensemble* ens, int n_ens,
+                                 const dem_batch_tuning* tuning);
+
+/* Creates the handle over caller arrays pos (sum capacity x 3), radii, F, V
+ * (float64, device) and the workspace; uploads the plan and quantises every
+ * system.  The handle owns host tables, a capture stream and the graph; the
+ * caller owns all device memory. */
+int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_tuning* tuning,
+                     double* pos, double* radii, double* forces, double* velocities,
+                     void* ws, size_t ws_bytes, dem_stream_t stream, dem_batch_t* out);
+
+/* System e now holds n droplets (the caller rewrote its slice of pos/radii,
+ * e.g. after a merge): rebuilds the work tables and re-quantises system e.
+ * Synchronises `stream`. */
+int dem_batch_update(dem_batch_t b, int e, int n, dem_stream_t stream);
+
+/* One global step as five launches on `stream`. */
+int dem_batch_step(dem_batch_t b, dem_stream_t stream);
+
+/* Captures one global step into a HIP graph (on the handle's own stream). */
+int dem_batch_capture(dem_batch_t b);
+
+/* `steps` global steps: graph launches when captured, else dem_batch_step. */
+int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream);
+
+/* Device address and size of a per-system array: "contact" (int32: the last
+ * step found a contact), "advanced" (int32: the last step advanced),
+ * "steps_done" (int32), "time" (float64, sum of dt over advanced steps). */
+int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes);
+
+int dem_batch_destroy(dem_batch_t b);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing (measurement support; no counterpart in the reference)       */
 /* ------------------------------------------------------------------------ */
 

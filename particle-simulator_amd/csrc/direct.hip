@@ -32,49 +32,16 @@
 #include <math.h>
 
 #include "dem_common.h"
-#include "pair_math.h"
+#include "direct_core.h"
 #include "profile.h"
 
 namespace {
 
+using dem::DevScales;
+using dem::Rec;
+using dem::RecC;
+
 constexpr int WG = 256;
-constexpr double kTwo32 = 4294967296.0;
-constexpr double kSigma = 0x1p-22;   // work unit / fixed unit
-constexpr double kSigma3 = 0x1p-66;
-
-// Phase-1 source / target record (32 bytes, read whole by the force kernel):
-// 32-bit fixed-point position (rounded), radius and radius^3 in work units.
-struct alignas(16) Rec {
-  uint32_t x, y, z;
-  float r;
-  float r3;
-  uint32_t pad[3];
-};
-static_assert(sizeof(Rec) == 32, "record layout");
-
-// Phase-2 record (48 bytes): 64-bit fixed-point position split into high and
-// low words, and the source force in units of Fs.  The Oseen sum decays as
-// 1/r, so a pair whose separation lies within one 32-bit quantum of L/2 would
-// change the velocity visibly if it took the other periodic image; 64-bit
-// positions make the image choice agree with the float64 rule |d| > L/2.
-struct alignas(16) RecC {
-  uint32_t xh, yh, zh;
-  float fx, fy, fz;
-  uint32_t xl, yl, zl;
-  uint32_t pad[3];
-};
-static_assert(sizeof(RecC) == 48, "record layout");
-
-// Per-call scales, computed on the device so open-boundary boxes need no host sync.
-struct DevScales {
-  double origin[3];  // subtracted before quantisation (open boxes)
-  double inv_U;      // fixed units per metre
-  double U;          // metres per fixed unit
-  double lambda;     // metres per work unit (= U * 2^22)
-  double Fs;         // newtons per force unit of Rec::f
-  int periodic;
-  int pad;
-};
 
 int g_split = 0;  // 0: heuristic
 int g_tile = 0;   // 0: heuristic
@@ -163,32 +130,19 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double* __restrict__
 __global__ void k_setup(const double* __restrict__ bbox, int nblk, double L, int periodic,
                         double m_const, DevScales* __restrict__ sc) {
   if (threadIdx.x != 0) return;
-  DevScales s;
-  s.periodic = periodic;
-  s.pad = 0;
   if (periodic) {
-    s.origin[0] = s.origin[1] = s.origin[2] = 0.0;
-    s.U = L / kTwo32;
-    s.inv_U = kTwo32 / L;
-  } else {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nblk; ++b)
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = fmin(lo[a], bbox[6 * b + a]);
-        hi[a] = fmax(hi[a], bbox[6 * b + 3 + a]);
-      }
-    double ext = 0.0;
-    for (int a = 0; a < 3; ++a) ext = fmax(ext, hi[a] - lo[a]);
-    if (!(ext > 0.0) || !isfinite(ext)) ext = 1.0;
-    for (int a = 0; a < 3; ++a) s.origin[a] = isfinite(lo[a]) ? lo[a] : 0.0;
-    // Map the box onto [0, 2^31 - 2^12]: differences never leave int32.
-    s.U = ext / (2147483648.0 - 4096.0);
-    s.inv_U = 1.0 / s.U;
+    *sc = dem::periodic_scales(L, m_const);
+    return;
   }
-  s.lambda = s.U / kSigma;
-  double fs = m_const * s.lambda * s.lambda;
-  s.Fs = (fs > 0.0 && isfinite(fs)) ? fs : 1.0;
-  *sc = s;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nblk; ++b)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fmin(lo[a], bbox[6 * b + a]);
+      hi[a] = fmax(hi[a], bbox[6 * b + 3 + a]);
+    }
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) ext = fmax(ext, hi[a] - lo[a]);
+  *sc = dem::open_scales(lo, ext, m_const);
 }
 
 // Quantise positions, scale radii, and (optionally) load caller forces as sources.
@@ -200,45 +154,10 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const DevScales sc = *scp;
-  unsigned long long q[3];
-  for (int a = 0; a < 3; ++a) {
-    double x = pos[3 * (size_t)i + a];
-    double v;
-    if (sc.periodic) {
-      double u = x * sc.inv_U * 0x1p-32;  // x / L
-      u -= floor(u);
-      v = rint(u * 0x1p64);
-      if (v >= 0x1p64) v = 0.0;  // u rounded up to 1: wraps to the origin
-    } else {
-      v = rint((x - sc.origin[a]) * sc.inv_U * kTwo32);
-    }
-    q[a] = (unsigned long long)v;
-  }
-  double rl = radii[i] / sc.lambda;
   Rec o;
-  // phase-1 position: the 64-bit value rounded to its high word
-  o.x = (uint32_t)((q[0] + 0x80000000ull) >> 32);
-  o.y = (uint32_t)((q[1] + 0x80000000ull) >> 32);
-  o.z = (uint32_t)((q[2] + 0x80000000ull) >> 32);
-  o.r = (float)rl;
-  o.r3 = (float)(rl * rl * rl);
-  o.pad[0] = o.pad[1] = o.pad[2] = 0u;
-  rec[i] = o;
   RecC c;
-  c.xh = (uint32_t)(q[0] >> 32);
-  c.yh = (uint32_t)(q[1] >> 32);
-  c.zh = (uint32_t)(q[2] >> 32);
-  c.xl = (uint32_t)q[0];
-  c.yl = (uint32_t)q[1];
-  c.zl = (uint32_t)q[2];
-  if (forces) {
-    c.fx = (float)(forces[3 * (size_t)i + 0] / sc.Fs);
-    c.fy = (float)(forces[3 * (size_t)i + 1] / sc.Fs);
-    c.fz = (float)(forces[3 * (size_t)i + 2] / sc.Fs);
-  } else {
-    c.fx = c.fy = c.fz = 0.f;
-  }
-  c.pad[0] = c.pad[1] = c.pad[2] = 0u;
+  dem::pack_one(pos + 3 * (size_t)i, radii[i], forces ? forces + 3 * (size_t)i : nullptr, sc, o, c);
+  rec[i] = o;
   recc[i] = c;
 }
 
@@ -251,23 +170,8 @@ __global__ __launch_bounds__(WG) void k_force_tile(const Rec* __restrict__ rec, 
   const int s = blockIdx.y;
   const int j0 = s * jchunk;
   const int j1 = min(n, j0 + jchunk);
-  const uint32_t xi = rec[ii].x, yi = rec[ii].y, zi = rec[ii].z;
-  const float ri = rec[ii].r;
   double ax = 0.0, ay = 0.0, az = 0.0;
-  for (int jt = j0; jt < j1; jt += tile) {
-    const int je = min(j1, jt + tile);
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll 4
-    for (int j = jt; j < je; ++j) {
-      const Rec sj = rec[j];  // wave-uniform: scalar loads
-      // contact mask |d| > r_i + r_j also drops j == i
-      dem::dipole_pair<false>(dem::diff32(sj.x, xi), dem::diff32(sj.y, yi),
-                              dem::diff32(sj.z, zi), ri, sj.r, sj.r3, fx, fy, fz);
-    }
-    ax += (double)fx;
-    ay += (double)fy;
-    az += (double)fz;
-  }
+  dem::force_partial(rec, rec[ii].x, rec[ii].y, rec[ii].z, rec[ii].r, j0, j1, tile, ax, ay, az);
   if (i < n) {
     part[(size_t)(6 * s + 0) * n + i] = ax;
     part[(size_t)(6 * s + 1) * n + i] = ay;
@@ -282,22 +186,17 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
                                                       RecC* __restrict__ recc) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int s = 0; s < S; ++s) {
-    sx += part[(size_t)(6 * s + 0) * n + i];
-    sy += part[(size_t)(6 * s + 1) * n + i];
-    sz += part[(size_t)(6 * s + 2) * n + i];
-  }
+  double sum[3] = {0.0, 0.0, 0.0};
+  for (int s = 0; s < S; ++s)
+    for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(6 * s + a) * n + i];
   const DevScales sc = *scp;
-  const double rl = radii[i] / sc.lambda;
-  const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
-  const double Fx = c * sx, Fy = c * sy, Fz = c * sz;
-  F[3 * (size_t)i + 0] = Fx;
-  F[3 * (size_t)i + 1] = Fy;
-  F[3 * (size_t)i + 2] = Fz;
-  recc[i].fx = (float)(Fx / sc.Fs);
-  recc[i].fy = (float)(Fy / sc.Fs);
-  recc[i].fz = (float)(Fz / sc.Fs);
+  double Fi[3];
+  float fr[3];
+  dem::force_finish(sum, radii[i], sc, m_const, Fi, fr);
+  for (int a = 0; a < 3; ++a) F[3 * (size_t)i + a] = Fi[a];
+  recc[i].fx = fr[0];
+  recc[i].fy = fr[1];
+  recc[i].fz = fr[2];
 }
 
 // ---------------------------------------------------------------- phase 2
@@ -329,54 +228,18 @@ __global__ __launch_bounds__(WG) void k_conv_tile(const RecC* __restrict__ rec, 
     cg.ga = (float)(-scp->U * ca.L_ratio * ca.inv_dx);
     cg.gb = (float)(-ca.gmin * ca.inv_dx);
   }
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  double cx = 0.0, cy = 0.0, cz = 0.0;
-  for (int jt = j0; jt < j1; jt += tile) {
-    const int je = min(j1, jt + tile);
-    float vx = 0.f, vy = 0.f, vz = 0.f;
-    float ux = 0.f, uy = 0.f, uz = 0.f;
-#pragma unroll 4
-    for (int j = jt; j < je; ++j) {
-      const RecC sj = rec[j];
-      const float dx = dem::diff64(sj.xh, sj.xl, xi);
-      const float dy = dem::diff64(sj.yh, sj.yl, yi);
-      const float dz = dem::diff64(sj.zh, sj.zl, zi);
-      const bool other = dx * dx + dy * dy + dz * dz > 0.f;  // drops j == i
-      dem::oseen_pair(dx, dy, dz, sj.fx, sj.fy, sj.fz, other, vx, vy, vz);
-      if (CORR)
-        dem::corr_lookup(cg, dx, dy, dz, other ? sj.fx : 0.f, other ? sj.fy : 0.f,
-                         other ? sj.fz : 0.f, ux, uy, uz);
-    }
-    ax += (double)vx;
-    ay += (double)vy;
-    az += (double)vz;
-    if (CORR) {
-      cx += (double)ux;
-      cy += (double)uy;
-      cz += (double)uz;
-    }
-  }
+  double m[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+  dem::conv_partial<CORR>(rec, xi, yi, zi, cg, j0, j1, tile, m, c);
   if (i < n) {
-    part[(size_t)(6 * s + 0) * n + i] = ax;
-    part[(size_t)(6 * s + 1) * n + i] = ay;
-    part[(size_t)(6 * s + 2) * n + i] = az;
-    if (CORR) {
-      part[(size_t)(6 * s + 3) * n + i] = cx;
-      part[(size_t)(6 * s + 4) * n + i] = cy;
-      part[(size_t)(6 * s + 5) * n + i] = cz;
-    }
+    for (int a = 0; a < 3; ++a) part[(size_t)(6 * s + a) * n + i] = m[a];
+    if (CORR)
+      for (int a = 0; a < 3; ++a) part[(size_t)(6 * s + 3 + a) * n + i] = c[a];
   }
 }
 
-struct CorrScalars {
-  int enabled;
-  double common;      // Fz_inv * eta_ratio * L_ratio
-  double self_coeff;
-};
-
 __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ part, int S, int n,
                                                      const DevScales* __restrict__ scp,
-                                                     double eta_const, CorrScalars cs,
+                                                     double eta_const, dem::CorrScalars cs,
                                                      const double* __restrict__ F,
                                                      double* __restrict__ V) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -388,13 +251,9 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
       if (cs.enabled) c[a] += part[(size_t)(6 * s + 3 + a) * n + i];
     }
   const DevScales sc = *scp;
-  const double km = eta_const * sc.Fs / sc.U;
-  const double kc = sc.Fs * cs.common;
-  for (int a = 0; a < 3; ++a) {
-    double v = km * m[a];
-    if (cs.enabled) v += kc * c[a] + cs.self_coeff * F[3 * (size_t)i + a];
-    V[3 * (size_t)i + a] = v;
-  }
+  double Vi[3];
+  dem::conv_finish(m, c, sc, eta_const, cs, F + 3 * (size_t)i, Vi);
+  for (int a = 0; a < 3; ++a) V[3 * (size_t)i + a] = Vi[a];
 }
 
 // ---------------------------------------------------------------- FP64 oracle kernels
@@ -582,7 +441,7 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
   const int jchunk = (n + S - 1) / S;
   dim3 grid((n + WG - 1) / WG, S);
   CorrArgs ca{};
-  CorrScalars cs{};
+  dem::CorrScalars cs{};
   const bool on = corr && corr->enabled;
   if (on) {
     ca.grid = reinterpret_cast<const float4*>(corr->grid);

@@ -10,46 +10,28 @@
 #include <math.h>
 
 #include "dem_common.h"
+#include "euler_core.h"
 
 namespace {
-
-__device__ __forceinline__ double py_fmod_pos(double x, double L) {
-#pragma clang fp contract(off)
-  double m = fmod(x, L);
-  if (m != 0.0) {
-    if ((L < 0.0) != (m < 0.0)) m += L;
-  } else {
-    m = copysign(0.0, L);
-  }
-  return m;
-}
 
 __global__ __launch_bounds__(256) void k_euler(double* __restrict__ pos,
                                                const double* __restrict__ F,
                                                const double* __restrict__ conv,
                                                const double* __restrict__ r, int n3, double sf,
                                                double dt, double L, int periodic) {
-#pragma clang fp contract(off)
   int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n3) return;
   const int i = k / 3;
-  const double denom = sf * r[i];
-  double v = F[k] / denom;
-  if (conv) v = v + conv[k];
-  double x = pos[k] + v * dt;
-  if (periodic) x = py_fmod_pos(x, L);
-  pos[k] = x;
+  pos[k] = dem::euler_coord(pos[k], F[k], conv ? conv[k] : 0.0, conv != nullptr, r[i], sf, dt, L,
+                            periodic);
 }
 
 __global__ __launch_bounds__(256) void k_euler_total(double* __restrict__ pos,
                                                      const double* __restrict__ vel, int n3,
                                                      double dt, double L, int periodic) {
-#pragma clang fp contract(off)
   int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n3) return;
-  double x = pos[k] + vel[k] * dt;
-  if (periodic) x = py_fmod_pos(x, L);
-  pos[k] = x;
+  pos[k] = dem::euler_total_coord(pos[k], vel[k], dt, L, periodic);
 }
 
 }  // namespace

@@ -79,6 +79,29 @@ class DemTreeInfo(ctypes.Structure):
     ]
 
 
+class DemBatchEnsemble(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("capacity", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("L", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("stokes_factor", ctypes.c_double),
+        ("m_const", ctypes.c_double),
+        ("eta_const", ctypes.c_double),
+    ]
+
+
+class DemBatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int32),
+        ("jlen", ctypes.c_int32),
+        ("force_tile", ctypes.c_int32),
+        ("conv_tile", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -106,6 +129,15 @@ SIGNATURES: dict[str, tuple] = {
     "dem_octree_stokeslet": (_I, [_P, _D, _P, _D, _P, _P, _P, _S, _P]),
     "dem_octree_total_velocity": (_I, [_P, _D, _P, _D, _D, _P, _P, _P, _S, _P]),
     "dem_octree_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "dem_batch_split": (_I, [_I, _I]),
+    "dem_batch_workspace_bytes": (_S, [_P, _I, _P]),
+    "dem_batch_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _S, _P, ctypes.POINTER(_P)]),
+    "dem_batch_update": (_I, [_P, _I, _I, _P]),
+    "dem_batch_step": (_I, [_P, _P]),
+    "dem_batch_capture": (_I, [_P]),
+    "dem_batch_run": (_I, [_P, _I, _P]),
+    "dem_batch_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_S)]),
+    "dem_batch_destroy": (_I, [_P]),
     "dem_profile_start": (_I, []),
     "dem_profile_stop": (_I, []),
     "dem_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),

@@ -26,6 +26,48 @@ from ..particle_state import DropletState
 from .solution_base import Solution
 
 
+def merge_collided(positions, radii, collided):
+    """Volume-conserving greedy merge (reference droplet_solution.py:208-271).
+
+    Returns (new_positions, new_radii, collided sorted by first index).  Pairs
+    are visited in ``np.argsort(collided[:, 0])`` order; a pair is skipped when
+    either droplet already merged; r = (r1^3 + r2^3) ** (1/3); the centre is
+    the r^3-weighted mean of the raw coordinates; survivors keep their order
+    (``np.delete``) and merged droplets are appended in processing order.
+    """
+    new_radii = np.copy(radii)
+    new_positions = np.copy(positions)
+    new_droplets_radii = []
+    new_droplets_positions = []
+    collided_indices = set()
+
+    sorted_indices = np.argsort(collided[:, 0])
+    collided = collided[sorted_indices]
+
+    for idx1, idx2 in collided:
+        if idx1 in collided_indices or idx2 in collided_indices:
+            continue
+        radius1, radius2 = new_radii[idx1], new_radii[idx2]
+        position1, position2 = new_positions[idx1], new_positions[idx2]
+        radius1_cube = radius1**3
+        radius2_cube = radius2**3
+        new_radius = (radius1_cube + radius2_cube) ** (1 / 3)
+        new_position = (position1 * radius1_cube + position2 * radius2_cube) / (
+            radius1_cube + radius2_cube
+        )
+        new_droplets_radii.append(new_radius)
+        new_droplets_positions.append(new_position)
+        collided_indices.add(idx1)
+        collided_indices.add(idx2)
+
+    indices_to_remove = sorted(collided_indices, reverse=True)
+    new_radii = np.delete(new_radii, indices_to_remove)
+    new_positions = np.delete(new_positions, indices_to_remove, axis=0)
+    new_radii = np.append(new_radii, new_droplets_radii)
+    new_positions = np.vstack([new_positions, new_droplets_positions])
+    return new_positions, new_radii, collided
+
+
 class DropletSolution(Solution):
     def __init__(
         self,
@@ -127,39 +169,8 @@ class DropletSolution(Solution):
 
     def generate_next_solution(self):
         """Merge the collided pairs and start the next trajectory segment."""
-        new_radii = np.copy(self.radii)
-        new_positions = np.copy(self.trajectories[self.current_step])
-        new_droplets_radii = []
-        new_droplets_positions = []
-        resulting_droplet = []
-        collided_indices = set()
-
-        sorted_indices = np.argsort(self.collided_droplets[:, 0])
-        self.collided_droplets = self.collided_droplets[sorted_indices]
-
-        for idx1, idx2 in self.collided_droplets:
-            if idx1 in collided_indices or idx2 in collided_indices:
-                continue
-            radius1, radius2 = new_radii[idx1], new_radii[idx2]
-            position1, position2 = new_positions[idx1], new_positions[idx2]
-            radius1_cube = radius1**3
-            radius2_cube = radius2**3
-            new_radius = (radius1_cube + radius2_cube) ** (1 / 3)
-            new_position = (position1 * radius1_cube + position2 * radius2_cube) / (
-                radius1_cube + radius2_cube
-            )
-            new_droplets_radii.append(new_radius)
-            new_droplets_positions.append(new_position)
-            collided_indices.add(idx1)
-            collided_indices.add(idx2)
-
-        indices_to_remove = sorted(collided_indices, reverse=True)
-        new_radii = np.delete(new_radii, indices_to_remove)
-        new_positions = np.delete(new_positions, indices_to_remove, axis=0)
-        new_radii = np.append(new_radii, new_droplets_radii)
-        new_positions = np.vstack([new_positions, new_droplets_positions])
-        # computed but not stored, as in the reference (:270-271)
-        resulting_droplet.extend(range(len(new_radii) - len(new_droplets_radii), len(new_radii)))
+        new_positions, new_radii, self.collided_droplets = merge_collided(
+            self.trajectories[self.current_step], self.radii, self.collided_droplets)
 
         new_initial_state = DropletState(new_positions, new_radii, self.times[self.current_step])
         next_solution = DropletSolution(

@@ -1,0 +1,139 @@
+"""Batched ensembles (``dem_batch_*``) on the GPU.
+
+* Bit-for-bit: N uneven periodic systems stepped together (contacts, merges,
+  systems finishing at different step counts) end in exactly the states the
+  single-system path reaches system after system with the same source split.
+* Against the FP64 oracle: the first batched step's forces, velocities and
+  displacements, under the direct kernels' accuracy contract.
+* Graph replay equals eager launches; finished systems stay put; the update
+  call rejects sizes beyond the capacity.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from dem import _native
+from dem.batch import BatchedEnsembles, SequentialEnsembles, Tuning, autotune
+from dem.force_calculator import DirectDropletForceCalculator
+from dem.workloads import make_system
+
+from ._errors import summary
+
+pytestmark = pytest.mark.gpu
+
+TOL_F = 1e-5
+TOL_V_NORM = 1e-5
+TOL_V_P999 = 1e-4
+TOL_V_MAX = 1e-3
+FLOOR = 1e-3
+
+# (n, phi, E): uneven sizes, dense enough that merges happen within a few steps
+SPECS = [(300, 0.06, 4e5), (777, 0.05, 3e5), (1500, 0.04, 5e5), (1024, 0.06, 2e5),
+         (513, 0.08, 4.5e5), (2100, 0.03, 3.5e5)]
+STEPS = [15, 10, 15, 12, 15, 8]
+
+
+def _systems(specs=SPECS):
+    return [make_system(n, seed=10 + e, phi=phi, E=E) for e, (n, phi, E) in enumerate(specs)]
+
+
+def _dts(systems, frac=0.5):
+    """dt moving the fastest droplet frac * r_min in the first step."""
+    out = []
+    for s in systems:
+        fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                          eta_water=s.eta_water, E=s.E, L=s.L)
+        F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+        vmax = np.max(np.linalg.norm(F / (s.stokes_factor * s.radii[:, None]) + V, axis=1))
+        out.append(frac * s.radii.min() / vmax)
+    return out
+
+
+@pytest.fixture(scope="module")
+def setup():
+    systems = _systems()
+    return systems, _dts(systems)
+
+
+@pytest.mark.parametrize("block,jlen", [(64, 256), (256, 512), (128, 100000)])
+def test_batched_equals_sequential_bit_for_bit(gpu, setup, block, jlen):
+    systems, dts = setup
+    b = BatchedEnsembles(systems, dts, STEPS, Tuning(block, jlen), capture=True)
+    g = b.run_until_done(max_global_steps=200)
+    s = SequentialEnsembles(systems, dts, STEPS, jlen=jlen)
+    s.run_until_done()
+    assert b.done()
+    assert np.array_equal(b.steps_done.cpu().numpy(), np.array(STEPS))
+    assert b.merges == s.merges and b.merges > 0, "the run should contain merges"
+    assert g >= max(STEPS)
+    for e in range(len(systems)):
+        assert np.array_equal(b.radii_of(e), s.radii_of(e)), f"system {e} radii"
+        assert np.array_equal(b.positions(e), s.positions(e)), f"system {e} positions"
+        n = len(b.radii_of(e))
+        np.testing.assert_array_equal(b.forces(e), s.F[e].cpu().numpy()[:n])
+    # volume is conserved through the merges
+    for e, sy in enumerate(systems):
+        np.testing.assert_allclose(np.sum(b.radii_of(e) ** 3), np.sum(sy.radii ** 3), rtol=1e-12)
+    b.close()
+
+
+def test_first_step_against_fp64_oracle(gpu, setup):
+    systems, dts = setup
+    sub = [systems[0], systems[2], systems[5]]
+    sdt = [dts[0], dts[2], dts[5]]
+    b = BatchedEnsembles(sub, sdt, [1, 1, 1], Tuning(256, 512), capture=False)
+    b.run(1, merge=False)
+    assert np.array_equal(b.advanced.cpu().numpy(), np.array([1, 1, 1]))
+    for e, sy in enumerate(sub):
+        F_ref = oracle.direct_forces(sy.positions, sy.radii, sy.m_const, sy.L, True)
+        V_ref = oracle.direct_convection(sy.positions, F_ref, sy.eta_const, sy.L, True)
+        ef = summary(b.forces(e), F_ref, FLOOR)
+        ev = summary(b.velocities(e), V_ref, FLOOR)
+        assert ef["max"] <= TOL_F, ef
+        assert ev["norm"] <= TOL_V_NORM and ev["p999"] <= TOL_V_P999 and ev["max"] <= TOL_V_MAX, ev
+        x_ref = oracle.euler_step(sy.positions, F_ref, V_ref, sy.radii, sy.stokes_factor, sdt[e],
+                                  sy.L, True)
+        d = b.positions(e) - sy.positions
+        d_ref = x_ref - sy.positions
+        d -= sy.L * np.round(d / sy.L)
+        d_ref -= sy.L * np.round(d_ref / sy.L)
+        ed = summary(d, d_ref, FLOOR)
+        assert ed["norm"] <= TOL_V_NORM and ed["max"] <= TOL_V_MAX, ed
+    b.close()
+
+
+def test_graph_replay_equals_eager_launches(gpu, setup):
+    systems, dts = setup
+    a = BatchedEnsembles(systems[:3], dts[:3], [6, 6, 6], Tuning(128, 1024), capture=True)
+    c = BatchedEnsembles(systems[:3], dts[:3], [6, 6, 6], Tuning(128, 1024), capture=False)
+    a.run_until_done(100)
+    c.run_until_done(100)
+    assert a.captured and not c.captured
+    for e in range(3):
+        assert np.array_equal(a.positions(e), c.positions(e))
+        assert np.array_equal(a.radii_of(e), c.radii_of(e))
+    a.close()
+    c.close()
+
+
+def test_finished_and_empty_step_budgets(gpu, setup):
+    systems, dts = setup
+    b = BatchedEnsembles(systems[:2], dts[:2], [0, 3], Tuning(64, 256))
+    p0 = b.positions(0).copy()
+    b.run(5)
+    assert np.array_equal(b.positions(0), p0)
+    assert list(b.steps_done.cpu().numpy()) == [0, 3]
+    np.testing.assert_allclose(b.time.cpu().numpy()[1], 3 * dts[1], rtol=1e-15)
+    with pytest.raises(_native.NativeError):
+        _native.call("dem_batch_update", b._h, 1, int(b.capacity[1]) + 1, None)
+    b.close()
+
+
+def test_autotune_returns_a_timed_candidate(gpu, setup):
+    systems, dts = setup
+    best, times = autotune(systems[:3], dts[:3], blocks=(64, 256), jlens=(512, 4096), reps=1)
+    assert (best.block, best.jlen) in times and len(times) == 4
+    assert all(t > 0 for t in times.values())

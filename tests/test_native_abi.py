@@ -58,3 +58,61 @@ def test_zero_particles_is_a_no_op():
     ph = _native.DemPhys(1.0, 1.0, 1.0, 1, 0)
     assert lib.dem_direct_forces(None, None, 0, ctypes.byref(ph), None, None, 0, None) == 0
     assert lib.dem_euler_step(None, None, None, None, 0, 1.0, 1.0, 1.0, 1, None) == 0
+
+
+def _split_max(n):
+    return max(1, min(64, (n + 63) // 64, (1 << 21) // max(n, 1)))
+
+
+def test_batch_split_rule_is_host_only():
+    lib = _native.load()
+    for n in (1, 63, 300, 2048, 20000, 65536):
+        for jlen in (0, 256, 1024, 4096):
+            want = 1 if jlen <= 0 else min(-(-n // jlen), _split_max(n))
+            assert lib.dem_batch_split(n, jlen) == want
+    # the split times n never grows when a system shrinks (capacity bounds)
+    prev = 0
+    for n in range(1, 5000, 37):
+        cur = lib.dem_batch_split(n, 256) * n
+        assert cur >= prev
+        prev = cur
+
+
+def _ensembles(specs):
+    arr = (_native.DemBatchEnsemble * len(specs))()
+    for e, (off, n, cap) in enumerate(specs):
+        arr[e] = _native.DemBatchEnsemble(off, n, cap, 10, 1e-3, 1e-3, 1.0, 1.0, 1.0)
+    return arr
+
+
+def test_batch_workspace_and_argument_checks():
+    lib = _native.load()
+    tune = _native.DemBatchTuning(256, 1024, 0, 0)
+    ok = _ensembles([(0, 500, 500), (500, 800, 1000)])
+    small = lib.dem_batch_workspace_bytes(ok, 2, ctypes.byref(tune))
+    big = lib.dem_batch_workspace_bytes(_ensembles([(0, 500, 500), (500, 800, 20000)]), 2,
+                                        ctypes.byref(tune))
+    assert 0 < small < big and small % 256 == 0
+    out = ctypes.c_void_p()
+    # overlapping reserved ranges
+    bad = _ensembles([(0, 500, 500), (400, 100, 100)])
+    rc = lib.dem_batch_create(bad, 2, ctypes.byref(tune), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                              ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 1 << 40,
+                              None, ctypes.byref(out))
+    assert rc == _native.DEM_ERR_INVALID and "overlap" in _native.last_error()
+    # n above capacity
+    bad = _ensembles([(0, 600, 500)])
+    assert lib.dem_batch_workspace_bytes(bad, 1, ctypes.byref(tune)) == 0
+    # unsupported block size
+    t2 = _native.DemBatchTuning(96, 1024, 0, 0)
+    rc = lib.dem_batch_create(ok, 2, ctypes.byref(t2), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                              ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 1 << 40,
+                              None, ctypes.byref(out))
+    assert rc == _native.DEM_ERR_INVALID and "block" in _native.last_error()
+    # workspace too small
+    rc = lib.dem_batch_create(ok, 2, ctypes.byref(tune), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                              ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 16,
+                              None, ctypes.byref(out))
+    assert rc == _native.DEM_ERR_WORKSPACE
+    assert out.value is None
+    assert lib.dem_batch_destroy(None) == 0
