@@ -165,6 +165,90 @@ def tree_step(reps: int = 5) -> dict:
     }
 
 
+def ensembles_leg(systems, steps: int, plain_steps: int, tune: bool = True) -> dict:
+    """256 uneven periodic systems: batched graph step vs the single-system
+    path run system after system.  Both include contact checks, host merges
+    and Euler steps; rates count Euler steps actually taken (a system that
+    merges does not advance in that global step)."""
+    import numpy as np
+    import torch
+
+    from dem import _native
+    from dem.batch import BatchedEnsembles, SequentialEnsembles, Tuning, autotune
+    from dem.force_calculator import DirectDropletForceCalculator
+
+    # dt: the fastest droplet moves 0.1 r_min in the first step (FP32 path)
+    dts = []
+    for sy in systems:
+        fc = DirectDropletForceCalculator(num_particles=sy.n, eps_oil=sy.eps_oil,
+                                          eta_oil=sy.eta_oil, eta_water=sy.eta_water, E=sy.E,
+                                          L=sy.L, boundary_mode="periodic")
+        F, V = fc.calculate_forces_and_convection(sy.positions, sy.radii)
+        v = np.linalg.norm(F / (sy.stokes_factor * sy.radii[:, None]) + V, axis=1)
+        dts.append(0.1 * sy.radii.min() / v.max())
+    ns = np.array([sy.n for sy in systems], dtype=np.float64)
+    big = 1 << 30
+    times = None
+    best = Tuning(256, 4096)
+    if tune:
+        best, times = autotune(systems, dts)
+    eng = BatchedEnsembles(systems, dts, [big] * len(systems), best, capture=True)
+    eng.run(2)
+    torch.cuda.synchronize()
+    d0 = eng.steps_done.cpu().numpy().astype(np.int64)
+    lib = _native.load()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    adv_b = int((eng.steps_done.cpu().numpy().astype(np.int64) - d0).sum())
+    merges_b = eng.merges
+    rounds_b = eng.merge_rounds
+    # per-phase wall time on a few more steps, each phase synchronised
+    eng.phase_ms = {}
+    prof_steps = max(1, min(5, steps))
+    eng.run(prof_steps)
+    phases = {k: v / prof_steps for k, v in eng.phase_ms.items()}
+    eng.close()
+    del eng
+
+    seq = SequentialEnsembles(systems, dts, [big] * len(systems))
+    seq.step_all()
+    torch.cuda.synchronize()
+    a0 = int(seq.steps_done.sum())
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        seq.step_all()
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    adv_p = int(seq.steps_done.sum()) - a0
+    rate_b = adv_b / tb
+    rate_p = adv_p / tp
+    pairs = float((ns * (ns - 1)).sum())
+    return {
+        "metric": "ensemble-steps/s batched vs plain",
+        "systems": len(systems),
+        "sum_n": int(ns.sum()),
+        "pairs_per_global_step": pairs,
+        "batched": {"ensemble_steps_per_s": rate_b, "global_steps": steps,
+                    "ms_per_global_step": tb / steps * 1e3, "ensemble_steps": adv_b,
+                    "merges": merges_b, "merge_rounds": rounds_b,
+                    "phase_ms_per_global_step": phases,
+                    "tuning": {"block": best.block, "jlen": best.jlen},
+                    "pair_interactions_per_s": pairs * adv_b / len(systems) / tb,
+                    "autotune_ms_per_step": ({f"{b}x{j}": v for (b, j), v in times.items()}
+                                             if times else None)},
+        "plain": {"ensemble_steps_per_s": rate_p, "global_steps": plain_steps,
+                  "ms_per_global_step": tp / plain_steps * 1e3, "ensemble_steps": adv_p,
+                  "merges": seq.merges,
+                  "path": "single-system device path per system (hash + count sync, "
+                          "dem_direct_forces_and_convection, dem_euler_step)"},
+        "speedup": rate_b / rate_p if rate_p > 0 else None,
+        "note": "dt_e = 0.1 r_min / max|v(0)| per system; sampled global steps of the "
+                "500-step run",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -174,12 +258,27 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the hash-pass and Barnes-Hut tree measurements")
+                    help="skip the hash-pass, Barnes-Hut tree and ensemble measurements")
+    ap.add_argument("--ens-steps", type=int, default=40,
+                    help="timed batched global steps of the 256-system workload")
+    ap.add_argument("--ens-plain-steps", type=int, default=4,
+                    help="timed steps of the system-after-system comparison")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--legs", default="hash,tree,ensembles",
+                    help="comma list of extra measurements: hash, tree, ensembles")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
+
+    ens_systems = None
+    legs = set() if a.no_extras else {x.strip() for x in a.legs.split(",") if x.strip()}
+    if "ensembles" in legs and a.ens_steps > 0:
+        from dem.workloads import ensembles
+
+        # placement runs in forked workers: before the GPU is initialised
+        ens_systems = ensembles(256, base_seed=0)
 
     import numpy as np
     import torch
@@ -234,12 +333,16 @@ def main() -> int:
     if not a.no_cpu_baseline:
         base = cpu_baseline(s, fc, Fn, a.cpu_seconds)
     extras = {}
-    if not a.no_extras:
+    if "hash" in legs:
         extras["hash"] = hash_pass(s)
+    if "tree" in legs:
         extras["tree"] = tree_step()
+    if ens_systems is not None:
+        extras["ensembles"] = ensembles_leg(ens_systems, a.ens_steps, a.ens_plain_steps,
+                                            tune=not a.no_autotune)
 
     out = {
-        "metric": "pair-interactions/s direct N=65536",
+        "metric": "pair-interactions/s direct N=65536; ensemble-steps/s batched vs plain",
         "value": value,
         "unit": "pair-interactions/s",
         "n_gpus": 1,

@@ -243,14 +243,23 @@ int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset
  * 43-159, with SpatialHashCollisionDetector.detect and
  * DirectDropletForceCalculator.calculate_forces_and_convection per step).
  * The batch keeps that per-system semantics — contact check first; on contact
- * the system does not advance and the caller merges it; otherwise force,
- * Oseen velocity and the Euler step with periodic wrap — and runs one global
- * step for every system in five launches (contact flags, force items, force
- * reduction, Oseen items, Oseen reduction + Euler + repack) whose sizes are
- * fixed at the capacities, so one captured HIP graph stays valid while merges
- * shrink the systems.  Each system's sums use the same source split as the
- * single-system kernels run with split dem_batch_split(n, jlen), so results are
- * bit-identical to dem_direct_forces_and_convection + dem_euler_step. */
+ * the caller merges and checks again without advancing; then force, Oseen
+ * velocity and the Euler step with periodic wrap — for all systems at once.
+ * One global step:
+ *   dem_batch_check(new_step=1)  hash pass of every active system: pair
+ *                                counts and the packed pair list;
+ *   dem_batch_advance            systems without contacts advance (this can
+ *                                run while the host plans the merges);
+ *   while some system has pairs: dem_batch_merge (the caller's merge plan),
+ *                                dem_batch_check(new_step=0) (systems not yet
+ *                                advanced);
+ *   dem_batch_advance            the remaining systems advance.
+ * Each system advances at most once per global step ("stepped" flag).  Launch
+ * sizes are fixed at the capacities, so the three captured HIP graphs stay
+ * valid while merges shrink the systems.  Each system's sums use the source
+ * split dem_batch_split(n, jlen); with that split passed to
+ * dem_direct_set_tuning, dem_direct_forces_and_convection + dem_euler_step give
+ * bit-identical results. */
 typedef struct dem_batch_ensemble {
   int32_t offset;   /* first droplet of the system in the concatenated arrays  */
   int32_t n;        /* droplets now (<= capacity)                              */
@@ -269,6 +278,9 @@ typedef struct dem_batch_tuning {
   int32_t jlen;       /* sources per work item (source split); <= 0: no split   */
   int32_t force_tile; /* FP32 sum length before the float64 hand-off; 0: 32     */
   int32_t conv_tile;  /* same for the Oseen sum; 0: 16                          */
+  int32_t contacts;   /* 1: contact pass; 0: every active system advances (the
+                         solver without a collision detector)                   */
+  int32_t reserved;
 } dem_batch_tuning;
 
 typedef struct dem_batch* dem_batch_t; /* opaque host handle */
@@ -281,30 +293,52 @@ size_t dem_batch_workspace_bytes(const dem_batch_ensemble* ens, int n_ens,
                                  const dem_batch_tuning* tuning);
 
 /* Creates the handle over caller arrays pos (sum capacity x 3), radii, F, V
- * (float64, device) and the workspace; uploads the plan and quantises every
- * system.  The handle owns host tables, a capture stream and the graph; the
- * caller owns all device memory. */
+ * (float64, device) and the workspace; uploads the work tables and quantises
+ * every system.  The handle owns host tables, a capture stream and the graphs;
+ * the caller owns all device memory. */
 int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_tuning* tuning,
                      double* pos, double* radii, double* forces, double* velocities,
                      void* ws, size_t ws_bytes, dem_stream_t stream, dem_batch_t* out);
 
-/* System e now holds n droplets (the caller rewrote its slice of pos/radii,
- * e.g. after a merge): rebuilds the work tables and re-quantises system e.
+/* Contact pass of every active system; new_step = 1 starts a global step
+ * (clears "stepped"), 0 re-checks systems not yet advanced.  Results: fields
+ * "pair_count", "pair_base" (exclusive scan of pair_count) and the packed
+ * "pairs" / "pair_values" (system after system, each lexicographically
+ * sorted; values are (x, y, z, r) of i then of j).  A total above the
+ * buffer's capacity is truncated. */
+int dem_batch_check(dem_batch_t b, int new_step, dem_stream_t stream);
+
+/* Force + Oseen + Euler of every active system whose last check found no
+ * contact and that has not advanced in this global step. */
+int dem_batch_advance(dem_batch_t b, dem_stream_t stream);
+
+/* Applies the caller's merges (droplet_solution.py:208-287 computed on the
+ * host): for systems[k], the droplets removed[removed_off[k] ..
+ * removed_off[k+1]) (local indices, ascending) leave, survivors keep their
+ * order, and the merged droplets merged[4*merged_off[k] .. 4*merged_off[k+1])
+ * ((x, y, z, r) each) are appended.  Host arrays; rebuilds the work tables,
+ * hash cells and records.  Synchronises `stream`. */
+int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int32_t* removed_off,
+                    const int32_t* removed, const int32_t* merged_off, const double* merged,
+                    dem_stream_t stream);
+
+/* Systems systems[k] now hold n[k] droplets (the caller rewrote their slices
+ * of pos/radii itself): rebuilds the work tables, hash cells and records.
  * Synchronises `stream`. */
-int dem_batch_update(dem_batch_t b, int e, int n, dem_stream_t stream);
+int dem_batch_resize(dem_batch_t b, int count, const int32_t* systems, const int32_t* n,
+                     dem_stream_t stream);
 
-/* One global step as five launches on `stream`. */
-int dem_batch_step(dem_batch_t b, dem_stream_t stream);
-
-/* Captures one global step into a HIP graph (on the handle's own stream). */
+/* Captures the first check, the re-check and the advance as HIP graphs (on
+ * the handle's own stream); later check/advance/run calls launch them. */
 int dem_batch_capture(dem_batch_t b);
 
-/* `steps` global steps: graph launches when captured, else dem_batch_step. */
+/* `steps` x (check, advance) without merging: systems with a contact stay. */
 int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream);
 
-/* Device address and size of a per-system array: "contact" (int32: the last
- * step found a contact), "advanced" (int32: the last step advanced),
- * "steps_done" (int32), "time" (float64, sum of dt over advanced steps). */
+/* Device address and size of "active", "stepped", "pair_count", "steps_done"
+ * (int32 per system), "pair_base" (int32, n_ens + 1), "time" (float64 per
+ * system: sum of dt over advanced steps), "pairs" (int32 (i, j), local
+ * indices), "pair_values" (float64 (x, y, z, r) x 2 per pair). */
 int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes);
 
 int dem_batch_destroy(dem_batch_t b);

@@ -1,12 +1,12 @@
-// Batched ensembles: one global step of many independent periodic droplet
-// systems in five launches, capturable as one HIP graph.
+// Batched ensembles: many independent periodic droplet systems stepped
+// together on one GPU, in two captured HIP graphs per global step.
 //
-// Reference semantics, per system (dem/solver/euler_droplet_solver.py:43-159
+// Reference semantics, per system (dem/solver/euler_droplet_solver.py:43-159,
 // driven system after system by main_multi.py:43-71):
 //   1. contact detection (SpatialHashCollisionDetector.detect,
-//      spatial_hash_collision_detector.py:233-316); on contact the step is
-//      abandoned, the caller merges (droplet_solution.py:208-287) and the
-//      step is retried;
+//      spatial_hash_collision_detector.py:233-316); on contact the caller
+//      merges (droplet_solution.py:208-287) and checks again, without
+//      advancing time;
 //   2. forces and Oseen velocities (direct_droplet_force_calculator.py:
 //      470-527);
 //   3. x += (F/(stokes_factor r) + v) dt, then x %= L (:113-131).
@@ -14,26 +14,38 @@
 // MI355X design:
 //   * All systems live in concatenated arrays (system e owns droplets
 //     [offset_e, offset_e + capacity_e)).  Work is described by two tables
-//     rebuilt on the host only when a merge changes a system's size: items
-//     (system, first target, source split) for the O(N^2) sums, and rows
-//     (system, first droplet) of 256 droplets for per-droplet work.  Launch
-//     grids are sized to the capacities and read the live counts from device
-//     memory, so one captured graph stays valid as systems shrink.
-//   * Launches: k_b_contact (one 1024-thread workgroup per system: bucket
-//     counts, block scan, scatter and the 27-cell search of the reference's
-//     hash, reduced to "any contact"), k_b_force (items), k_b_force_reduce
-//     (rows), k_b_conv (items), k_b_conv_euler (rows: Oseen reduction, Euler
-//     step, wrap and re-quantisation for the next step).
-//   * Items are ordered by decreasing cost so the long ones start first.
-//   * The per-target sums use the single-system functions (direct_core.h)
-//     with the same source split, so results equal the single-system path's.
+//     rebuilt on the host only when merges change sizes: items (system, first
+//     target, source split) for the O(N^2) sums and rows (system, first
+//     droplet) of 256 droplets for per-droplet work.  Grids are sized to the
+//     capacities and read live counts from device memory, so the captured
+//     graphs stay valid as systems shrink.
+//   * "check" graphs: the reference's hash pass for every active system at
+//     once — bucket counts (row-parallel atomics), one bucket scan per
+//     system, scatter (with a bucket-ordered copy of positions), the 27-cell
+//     search of every target (row-parallel), and for systems with contacts
+//     the lexicographically ordered pair list, packed system after system
+//     with both droplets' (x, y, z, r).  The first check of a global step
+//     clears the per-system "stepped" flags; later checks skip systems that
+//     already advanced.
+//   * "advance" graph: go = active, no contact, not yet stepped (fixed at the
+//     start of the launch); force items, force reduction, Oseen items, Oseen
+//     reduction + Euler step + wrap + re-quantisation for go systems.
+//   * dem_batch_merge applies host-computed merges on the device: survivors
+//     compacted in order, merged droplets appended.
+//   A global step is therefore: check; advance the clean systems while the
+//   host plans merges of the others; merge, check again ... ; advance the
+//   rest — each system goes through exactly the reference loop's states and
+//   advances once.
+//   * The per-target sums call the single-system functions (direct_core.h)
+//     with the same source split, so every system's result equals the
+//     single-system path's bit for bit.
 #include <math.h>
-
 #include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "dem_common.h"
@@ -48,8 +60,9 @@ using dem::DevScales;
 using dem::Rec;
 using dem::RecC;
 
-constexpr int kRow = 256;        // droplets per row
-constexpr int kContactWG = 1024; // threads of a contact workgroup
+constexpr int kRow = 256;       // droplets per row
+constexpr int kScanWG = 1024;   // threads of a per-system scan workgroup
+constexpr int kZeroBlocks = 1024;
 
 inline int split_max(int n) {  // same cap as the single-system kernels
   long long cap = (1ll << 21) / (n > 0 ? n : 1);
@@ -82,48 +95,77 @@ inline int next_pow2(int v) {
   return p;
 }
 
-// Device description of one system.
+inline int pair_capacity(int cap) { return cap > 64 ? cap : 64; }
+
+// Host-owned description of one system (uploaded whole).
 struct BEns {
-  DevScales sc;
-  long long part_off;  // doubles into the partial-sum array
-  long long hash_off;  // ints into the hash scratch
+  long long part_off;   // doubles into the partial-sum array
+  long long cnt_off;    // ints into the bucket counts (T per system)
+  long long start_off;  // ints into the bucket starts (T + 1 per system)
   int off, n, cap, S;
   int jchunk, steps, T, pad;
   double L, dt, sf, m_const, eta_const;
+};
+
+// Device-owned per-system values.
+struct BDev {
+  DevScales sc;
+  double cell;  // 2 max r
+  int g;        // cells per axis
+  int skip;     // cell <= 0: no contacts
 };
 
 struct BHead {
   int n_items;
   int n_rows;
   int n_ens;
-  int pad;
+  int contacts;  // 0: every active system advances (no contact pass)
+  int pair_cap;  // pairs the packed pair buffer holds
+  int pad[3];
 };
 
 struct BLayout {
   BEns* ens;
+  BDev* dev;
   BHead* head;
   int4* items;  // (e, i0, s, 0)
   int2* rows;   // (e, i0)
   Rec* rec;
   RecC* recc;
   double* part;
-  int* hash;    // per system: start[T+1], cnt[T], rank[cap], sorted[cap]
-  int* contact;
-  int* adv;
+  int* cnt;      // [sum T]
+  int* start;    // [sum T + 1]
+  int* bucket;   // [sum cap]
+  int* rank;     // [sum cap]
+  int* sorted;   // [sum cap] local droplet indices in bucket order
+  double4* spos; // [sum cap] (x, y, z, r) in bucket order
+  int* pc;       // [sum cap] pairs found for each target
+  int* poff;     // [sum cap] exclusive scan of pc within a system
+  int2* pairs;   // [pair_cap] packed system after system
+  double4* pvals;// [2 pair_cap] (x, y, z, r) of i and of j
+  int* pair_base;// [n_ens + 1] exclusive scan of pair_count
+  int* stage_i;  // merge lists uploaded by dem_batch_merge
+  double* stage_d;
+  int* active;
+  int* stepped;
+  int* go;
+  int* pair_count;
   int* steps_done;
   double* time;
   size_t bytes;
 };
 
 struct Plan {
-  long long items_cap = 0, rows_cap = 0, part_total = 0, hash_total = 0, n_total = 0;
-  std::vector<long long> part_off, hash_off;
+  long long items_cap = 0, rows_cap = 0, part_total = 0, cnt_total = 0, start_total = 0;
+  long long pair_total = 0, n_total = 0, cap_total = 0;
+  std::vector<long long> part_off, cnt_off, start_off;
 };
 
 Plan make_plan(const dem_batch_ensemble* ens, int n_ens, int block, int jlen) {
   Plan p;
   p.part_off.resize(n_ens);
-  p.hash_off.resize(n_ens);
+  p.cnt_off.resize(n_ens);
+  p.start_off.resize(n_ens);
   for (int e = 0; e < n_ens; ++e) {
     const int cap = ens[e].capacity;
     // a system only shrinks: bound its items and partials over every n <= cap
@@ -133,11 +175,16 @@ Plan make_plan(const dem_batch_ensemble* ens, int n_ens, int block, int jlen) {
     p.part_off[e] = p.part_total;
     p.part_total += (long long)S * 6 * cap;
     const int T = next_pow2(2 * (cap > 0 ? cap : 1));
-    p.hash_off[e] = p.hash_total;
-    p.hash_total += 2LL * T + 1 + 2LL * cap;
+    p.cnt_off[e] = p.cnt_total;
+    p.cnt_total += T;
+    p.start_off[e] = p.start_total;
+    p.start_total += T + 1;
+    p.pair_total += pair_capacity(cap);
+    p.cap_total += cap;
     long long end = (long long)ens[e].offset + cap;
     if (end > p.n_total) p.n_total = end;
   }
+  if (p.pair_total > 0x3fffffffLL) p.pair_total = 0x3fffffffLL;  // int offsets
   return p;
 }
 
@@ -145,16 +192,32 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   dem::Carver c(ws);
   BLayout l;
   const size_t ne = (size_t)(n_ens > 0 ? n_ens : 1);
+  auto at_least1 = [](long long v) { return (size_t)std::max<long long>(v, 1); };
   l.ens = c.take<BEns>(ne);
+  l.dev = c.take<BDev>(ne);
   l.head = c.take<BHead>(1);
-  l.items = c.take<int4>((size_t)std::max<long long>(p.items_cap, 1));
-  l.rows = c.take<int2>((size_t)std::max<long long>(p.rows_cap, 1));
-  l.rec = c.take<Rec>((size_t)std::max<long long>(p.n_total, 1));
-  l.recc = c.take<RecC>((size_t)std::max<long long>(p.n_total, 1));
-  l.part = c.take<double>((size_t)std::max<long long>(p.part_total, 1));
-  l.hash = c.take<int>((size_t)std::max<long long>(p.hash_total, 1));
-  l.contact = c.take<int>(ne);
-  l.adv = c.take<int>(ne);
+  l.items = c.take<int4>(at_least1(p.items_cap));
+  l.rows = c.take<int2>(at_least1(p.rows_cap));
+  l.rec = c.take<Rec>(at_least1(p.n_total));
+  l.recc = c.take<RecC>(at_least1(p.n_total));
+  l.part = c.take<double>(at_least1(p.part_total));
+  l.cnt = c.take<int>(at_least1(p.cnt_total));
+  l.start = c.take<int>(at_least1(p.start_total));
+  l.bucket = c.take<int>(at_least1(p.n_total));
+  l.rank = c.take<int>(at_least1(p.n_total));
+  l.sorted = c.take<int>(at_least1(p.n_total));
+  l.spos = c.take<double4>(at_least1(p.n_total));
+  l.pc = c.take<int>(at_least1(p.n_total));
+  l.poff = c.take<int>(at_least1(p.n_total));
+  l.pairs = c.take<int2>(at_least1(p.pair_total));
+  l.pvals = c.take<double4>(2 * at_least1(p.pair_total));
+  l.pair_base = c.take<int>(ne + 1);
+  l.stage_i = c.take<int>(at_least1(p.cap_total) + 3 * ne + 3);
+  l.stage_d = c.take<double>(4 * at_least1(p.cap_total));
+  l.active = c.take<int>(ne);
+  l.stepped = c.take<int>(ne);
+  l.go = c.take<int>(ne);
+  l.pair_count = c.take<int>(ne);
   l.steps_done = c.take<int>(ne);
   l.time = c.take<double>(ne);
   l.bytes = dem::align_up(c.used, 256);
@@ -163,190 +226,424 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
 
 // ---------------------------------------------------------------- set-up kernels
 
-__global__ void k_b_init(BEns* __restrict__ ens, int n_ens, int* __restrict__ contact,
-                         int* __restrict__ adv, int* __restrict__ steps_done,
-                         double* __restrict__ time) {
+__global__ void k_b_init(const BEns* __restrict__ ens, BDev* __restrict__ dev, int n_ens,
+                         int* __restrict__ active, int* __restrict__ stepped,
+                         int* __restrict__ go, int* __restrict__ pair_count,
+                         int* __restrict__ steps_done, double* __restrict__ time) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_ens) return;
-  ens[e].sc = dem::periodic_scales(ens[e].L, ens[e].m_const);
-  contact[e] = 0;
-  adv[e] = 0;
+  dev[e].sc = dem::periodic_scales(ens[e].L, ens[e].m_const);
+  active[e] = 0;
+  stepped[e] = 0;
+  go[e] = 0;
+  pair_count[e] = 0;
   steps_done[e] = 0;
   time[e] = 0.0;
 }
 
-// Quantise the droplets of every row (only_e < 0) or of system only_e.
+// Hash cell of every system: cell = 2 max r (spatial_hash...py:257-258; cell
+// <= 0 means no contacts, :259-260), g = max(1, ceil(L / cell)) (:262).
+__global__ __launch_bounds__(256) void k_b_cells(const BEns* __restrict__ ens,
+                                                 const BHead* __restrict__ head,
+                                                 const double* __restrict__ radii,
+                                                 BDev* __restrict__ dev) {
+  __shared__ double s_max[4];
+  const int e = blockIdx.x;
+  if (e >= head->n_ens) return;
+  const BEns& E = ens[e];
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < E.n; i += 256) m = fmax(m, radii[(size_t)E.off + i]);
+  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_down(m, d, 64));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  m = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+  const double cell = 2.0 * m;
+  const int skip = !(cell > 0.0);
+  double q = skip ? 1.0 : ceil(E.L / cell);
+  if (!(q >= 1.0)) q = 1.0;
+  if (q > 1073741824.0) q = 1073741824.0;
+  dev[e].cell = cell;
+  dev[e].g = (int)q;
+  dev[e].skip = skip;
+}
+
+// Quantise the droplets of every row.
 __global__ __launch_bounds__(kRow) void k_b_pack(const BEns* __restrict__ ens,
+                                                 const BDev* __restrict__ dev,
                                                  const int2* __restrict__ rows,
-                                                 const BHead* __restrict__ head, int only_e,
+                                                 const BHead* __restrict__ head,
                                                  const double* __restrict__ pos,
                                                  const double* __restrict__ radii,
                                                  Rec* __restrict__ rec, RecC* __restrict__ recc) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (only_e >= 0 && row.x != only_e) return;
   const BEns& E = ens[row.x];
   const int i = row.y + threadIdx.x;
   if (i >= E.n) return;
   const size_t g = (size_t)E.off + i;
   Rec o;
   RecC c;
-  dem::pack_one(pos + 3 * g, radii[g], nullptr, E.sc, o, c);
+  dem::pack_one(pos + 3 * g, radii[g], nullptr, dev[row.x].sc, o, c);
   rec[g] = o;
   recc[g] = c;
 }
 
-// ---------------------------------------------------------------- 1. contacts
-
-// Inclusive block scan of one int per thread (blockDim.x == kContactWG).
-__device__ int block_inclusive_scan(int v, int* sh) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int u = __shfl_up(v, d, 64);
-    if (lane >= d) v += u;
-  }
-  if (lane == 63) sh[wave] = v;
-  __syncthreads();
-  if (wave == 0) {
-    int w = lane < kContactWG / 64 ? sh[lane] : 0;
-#pragma unroll
-    for (int d = 1; d < kContactWG / 64; d <<= 1) {
-      int u = __shfl_up(w, d, 64);
-      if (lane >= d) w += u;
-    }
-    if (lane < kContactWG / 64) sh[lane] = w;
-  }
-  __syncthreads();
-  const int add = wave > 0 ? sh[wave - 1] : 0;
-  __syncthreads();
-  return v + add;
-}
-
-// One workgroup per system: does the reference's hash pass find any pair?
-// Decides whether the system advances this step (active and no contact).
-__global__ __launch_bounds__(kContactWG) void k_b_contact(const BEns* __restrict__ ens,
-                                                          const BHead* __restrict__ head,
-                                                          const double* __restrict__ pos,
-                                                          const double* __restrict__ radii,
-                                                          int* __restrict__ hash,
-                                                          int* __restrict__ contact,
-                                                          int* __restrict__ adv,
-                                                          const int* __restrict__ steps_done) {
-  __shared__ double s_max[kContactWG / 64];
-  __shared__ int s_scan[kContactWG / 64];
-  const int e = blockIdx.x;
-  if (e >= head->n_ens) return;
+// Merges computed by the caller, one workgroup per listed system.  Stage
+// layout: ints [0,c) systems, [c, 2c+1) removed offsets, [2c+1, 3c+2) merged
+// offsets, then the removed local indices (ascending per system); doubles:
+// (x, y, z, r) of every merged droplet.  Survivors keep their order and the
+// merged droplets follow them (droplet_solution.py:263-268).  `scratch`
+// holds the survivors between the two passes.
+__global__ __launch_bounds__(256) void k_b_compact(const BEns* __restrict__ ens, int c,
+                                                   const int* __restrict__ stage_i,
+                                                   const double* __restrict__ stage_d,
+                                                   double* __restrict__ pos,
+                                                   double* __restrict__ radii,
+                                                   double4* __restrict__ scratch) {
+  const int k = blockIdx.x;
+  if (k >= c) return;
+  const int e = stage_i[k];
+  const int* roff = stage_i + c;
+  const int* moff = stage_i + 2 * c + 1;
+  const int* R = stage_i + 3 * c + 2 + roff[k];
+  const int m = roff[k + 1] - roff[k];
   const BEns& E = ens[e];
   const int n = E.n;
-  const bool active = steps_done[e] < E.steps && n > 0;
-  if (!active) {
-    if (threadIdx.x == 0) {
-      contact[e] = 0;
-      adv[e] = 0;
+  const size_t off = (size_t)E.off;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    int lo = 0, hi = m;  // removed indices below i
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (R[mid] < i) lo = mid + 1;
+      else hi = mid;
     }
-    return;
-  }
-  const double* p = pos + 3 * (size_t)E.off;
-  const double* r = radii + E.off;
-  // cell = 2 max r (spatial_hash...py:257-258), g = max(1, ceil(L/cell)) (:262)
-  double m = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += kContactWG) m = fmax(m, r[i]);
-  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_down(m, d, 64));
-  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
-  __syncthreads();
-  m = s_max[0];
-  for (int w = 1; w < kContactWG / 64; ++w) m = fmax(m, s_max[w]);
-  const double cell = 2.0 * m;
-  if (!(cell > 0.0)) {  // no contacts (:259-260)
-    if (threadIdx.x == 0) {
-      contact[e] = 0;
-      adv[e] = 1;
-    }
-    return;
-  }
-  double q = ceil(E.L / cell);
-  if (!(q >= 1.0)) q = 1.0;
-  if (q > 1073741824.0) q = 1073741824.0;
-  const int g = (int)q;
-  const int T = E.T, mask = T - 1;
-  int* start = hash + E.hash_off;  // [T + 1]
-  int* cnt = start + T + 1;        // [T]
-  int* rank = cnt + T;             // [cap]
-  int* sorted = rank + E.cap;      // [cap]
-  for (int b = threadIdx.x; b < T; b += kContactWG) cnt[b] = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kContactWG) {
-    const int h = dem::hash_cell(dem::cell_of(p[3 * (size_t)i], cell, g, 1),
-                                 dem::cell_of(p[3 * (size_t)i + 1], cell, g, 1),
-                                 dem::cell_of(p[3 * (size_t)i + 2], cell, g, 1), mask);
-    rank[i] = atomicAdd(&cnt[h], 1);
+    if (lo < m && R[lo] == i) continue;
+    const size_t g = off + i;
+    scratch[off + (i - lo)] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
   }
   __syncthreads();
-  // exclusive scan of cnt into start: contiguous chunks per thread
-  const int chunk = (T + kContactWG - 1) / kContactWG;
-  const int b0 = min(T, (int)threadIdx.x * chunk), b1 = min(T, b0 + chunk);
-  int local = 0;
-  for (int b = b0; b < b1; ++b) local += cnt[b];
-  int run = block_inclusive_scan(local, s_scan) - local;
-  for (int b = b0; b < b1; ++b) {
-    start[b] = run;
-    run += cnt[b];
+  const int ns = n - m;
+  for (int q = threadIdx.x; q < ns; q += 256) {
+    const double4 v = scratch[off + q];
+    pos[3 * (off + q)] = v.x;
+    pos[3 * (off + q) + 1] = v.y;
+    pos[3 * (off + q) + 2] = v.z;
+    radii[off + q] = v.w;
   }
-  if (threadIdx.x == kContactWG - 1) start[T] = run;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += kContactWG) {
-    const int h = dem::hash_cell(dem::cell_of(p[3 * (size_t)i], cell, g, 1),
-                                 dem::cell_of(p[3 * (size_t)i + 1], cell, g, 1),
-                                 dem::cell_of(p[3 * (size_t)i + 2], cell, g, 1), mask);
-    sorted[start[h] + rank[i]] = i;
-  }
-  __syncthreads();
-  // 27-cell search over distinct neighbour buckets (:134-229), i < j
-  int found = 0;
-  for (int i = threadIdx.x; i < n && !found; i += kContactWG) {
-    const int ix = dem::cell_of(p[3 * (size_t)i], cell, g, 1);
-    const int iy = dem::cell_of(p[3 * (size_t)i + 1], cell, g, 1);
-    const int iz = dem::cell_of(p[3 * (size_t)i + 2], cell, g, 1);
-    int seen[27];
-    int nseen = 0;
-    for (int di = -1; di <= 1 && !found; ++di)
-      for (int dj = -1; dj <= 1 && !found; ++dj)
-        for (int dk = -1; dk <= 1 && !found; ++dk) {
-          const int nh = dem::hash_cell(dem::pymod(ix + di, g), dem::pymod(iy + dj, g),
-                                        dem::pymod(iz + dk, g), mask);
-          bool dup = false;
-          for (int k = 0; k < nseen; ++k) dup |= (seen[k] == nh);
-          if (dup) continue;
-          seen[nseen++] = nh;
-          for (int slot = start[nh]; slot < start[nh + 1]; ++slot) {
-            const int j = sorted[slot];
-            if (i < j && dem::contact(p, r, i, j, E.L, 1)) {
-              found = 1;
-              break;
-            }
-          }
-        }
-  }
-  const int any = __syncthreads_or(found);
-  if (threadIdx.x == 0) {
-    contact[e] = any ? 1 : 0;
-    adv[e] = any ? 0 : 1;
+  const int m0 = moff[k], mc = moff[k + 1] - moff[k];
+  for (int q = threadIdx.x; q < mc; q += 256) {
+    const double* v = stage_d + 4 * (size_t)(m0 + q);
+    const size_t g = off + ns + q;
+    pos[3 * g] = v[0];
+    pos[3 * g + 1] = v[1];
+    pos[3 * g + 2] = v[2];
+    radii[g] = v[3];
   }
 }
 
-// ---------------------------------------------------------------- 2-3. forces
+// ---------------------------------------------------------------- check graph
+
+// new_step: the first check of a global step (clears "stepped").  Active:
+// steps left, droplets present, not advanced yet in this global step.
+__global__ void k_b_begin(const BEns* __restrict__ ens, const BHead* __restrict__ head,
+                          int new_step, const int* __restrict__ steps_done,
+                          int* __restrict__ stepped, int* __restrict__ active,
+                          int* __restrict__ pair_count) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= head->n_ens) return;
+  if (new_step) stepped[e] = 0;
+  active[e] = (steps_done[e] < ens[e].steps && ens[e].n > 0 && !stepped[e]) ? 1 : 0;
+  pair_count[e] = 0;
+}
+
+__global__ void k_b_zero(int* __restrict__ p, long long count) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < count;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = 0;
+}
+
+__device__ __forceinline__ bool hashed(const BHead* head, const int* active, const BDev* dev,
+                                       int e) {
+  return head->contacts && active[e] && !dev[e].skip;
+}
+
+__device__ __forceinline__ int bucket_of(const double* p, const BDev& D, int mask) {
+  return dem::hash_cell(dem::cell_of(p[0], D.cell, D.g, 1), dem::cell_of(p[1], D.cell, D.g, 1),
+                        dem::cell_of(p[2], D.cell, D.g, 1), mask);
+}
+
+// Bucket of every droplet and its rank among the bucket's droplets.
+__global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
+                                                   const BDev* __restrict__ dev,
+                                                   const int2* __restrict__ rows,
+                                                   const BHead* __restrict__ head,
+                                                   const int* __restrict__ active,
+                                                   const double* __restrict__ pos,
+                                                   int* __restrict__ cnt, int* __restrict__ bucket,
+                                                   int* __restrict__ rank) {
+  if ((int)blockIdx.x >= head->n_rows) return;
+  const int2 row = rows[blockIdx.x];
+  if (!hashed(head, active, dev, row.x)) return;
+  const BEns& E = ens[row.x];
+  const int i = row.y + threadIdx.x;
+  if (i >= E.n) return;
+  const size_t g = (size_t)E.off + i;
+  const int h = bucket_of(pos + 3 * g, dev[row.x], E.T - 1);
+  bucket[g] = h;
+  rank[g] = atomicAdd(&cnt[E.cnt_off + h], 1);
+}
+
+// Exclusive scan of in[0, len) into out[0, len) (and out[len] = total when
+// `total`) by one kScanWG-thread workgroup, in contiguous per-thread chunks.
+__device__ void block_scan(const int* __restrict__ in, int* __restrict__ out, int len,
+                           bool total) {
+  __shared__ int s_w[kScanWG / 64];
+  const int chunk = (len + kScanWG - 1) / kScanWG;
+  const int b0 = min(len, (int)threadIdx.x * chunk), b1 = min(len, b0 + chunk);
+  int local = 0;
+  for (int b = b0; b < b1; ++b) local += in[b];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int v = local;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) s_w[wave] = v;
+  __syncthreads();
+  if (wave == 0) {
+    int w = lane < kScanWG / 64 ? s_w[lane] : 0;
+#pragma unroll
+    for (int d = 1; d < kScanWG / 64; d <<= 1) {
+      const int u = __shfl_up(w, d, 64);
+      if (lane >= d) w += u;
+    }
+    if (lane < kScanWG / 64) s_w[lane] = w;
+  }
+  __syncthreads();
+  int run = v - local + (wave > 0 ? s_w[wave - 1] : 0);
+  for (int b = b0; b < b1; ++b) {
+    out[b] = run;
+    run += in[b];
+  }
+  if (total && threadIdx.x == kScanWG - 1) out[len] = run;
+}
+
+__global__ __launch_bounds__(kScanWG) void k_b_bucket_scan(const BEns* __restrict__ ens,
+                                                           const BDev* __restrict__ dev,
+                                                           const BHead* __restrict__ head,
+                                                           const int* __restrict__ active,
+                                                           const int* __restrict__ cnt,
+                                                           int* __restrict__ start) {
+  const int e = blockIdx.x;
+  if (e >= head->n_ens || !hashed(head, active, dev, e)) return;
+  const BEns& E = ens[e];
+  block_scan(cnt + E.cnt_off, start + E.start_off, E.T, true);  // T + 1 slots per system
+}
+
+__global__ __launch_bounds__(kRow) void k_b_scatter(const BEns* __restrict__ ens,
+                                                    const BDev* __restrict__ dev,
+                                                    const int2* __restrict__ rows,
+                                                    const BHead* __restrict__ head,
+                                                    const int* __restrict__ active,
+                                                    const int* __restrict__ start,
+                                                    const int* __restrict__ bucket,
+                                                    const int* __restrict__ rank,
+                                                    const double* __restrict__ pos,
+                                                    const double* __restrict__ radii,
+                                                    int* __restrict__ sorted,
+                                                    double4* __restrict__ spos) {
+  if ((int)blockIdx.x >= head->n_rows) return;
+  const int2 row = rows[blockIdx.x];
+  if (!hashed(head, active, dev, row.x)) return;
+  const BEns& E = ens[row.x];
+  const int i = row.y + threadIdx.x;
+  if (i >= E.n) return;
+  const size_t g = (size_t)E.off + i;
+  const size_t slot = (size_t)E.off + start[E.start_off + bucket[g]] + rank[g];
+  sorted[slot] = i;
+  spos[slot] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
+}
+
+// Pairs (i, j), i < j, of target i over its distinct neighbour buckets
+// (spatial_hash...py:134-229); EMIT writes them to out[0, room) sorted by j.
+// The 27 bucket ranges are loaded up front and candidates are read from the
+// bucket-ordered copy (sorted index and (x, y, z, r) of the same slot), so a
+// candidate costs one memory round trip.
+template <bool EMIT>
+__device__ int search_target(const BEns& E, const BDev& D, const double* __restrict__ p,
+                             const double* __restrict__ r, const int* __restrict__ start,
+                             const int* __restrict__ sorted, const double4* __restrict__ spos,
+                             int i, int2* __restrict__ out, double4* __restrict__ vals,
+                             int room) {
+  const int mask = E.T - 1, g = D.g;
+  const double xi = p[3 * (size_t)i], yi = p[3 * (size_t)i + 1], zi = p[3 * (size_t)i + 2];
+  const double ri = r[i];
+  const int ix = dem::cell_of(xi, D.cell, g, 1);
+  const int iy = dem::cell_of(yi, D.cell, g, 1);
+  const int iz = dem::cell_of(zi, D.cell, g, 1);
+  int nb[27];
+  int nseen = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int di = k / 9 - 1, dj = (k / 3) % 3 - 1, dk = k % 3 - 1;
+    const int nh = dem::hash_cell(dem::pymod(ix + di, g), dem::pymod(iy + dj, g),
+                                  dem::pymod(iz + dk, g), mask);
+    bool dup = false;
+#pragma unroll
+    for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
+    if (!dup) nb[nseen++] = nh;
+  }
+  int s0[27], s1[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int h = k < nseen ? nb[k] : 0;
+    s0[k] = start[h];
+    s1[k] = k < nseen ? start[h + 1] : s0[k];
+  }
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    for (int slot = s0[k]; slot < s1[k]; ++slot) {
+      const int j = sorted[slot];
+      const double4 q = spos[slot];
+      if (i < j && dem::contact_pair(xi, yi, zi, ri, q.x, q.y, q.z, q.w, E.L, 1)) {
+        if (EMIT && cnt < room) {
+          out[cnt] = make_int2(i, j);
+          vals[2 * cnt] = make_double4(xi, yi, zi, ri);
+          vals[2 * cnt + 1] = q;
+        }
+        ++cnt;
+      }
+    }
+  }
+  if (EMIT) {  // in-segment insertion sort by j (a handful of pairs)
+    const int hi = min(cnt, room);
+    for (int a = 1; a < hi; ++a) {
+      const int2 v = out[a];
+      const double4 vj = vals[2 * a + 1];
+      int b = a - 1;
+      while (b >= 0 && out[b].y > v.y) {
+        out[b + 1] = out[b];
+        vals[2 * (b + 1) + 1] = vals[2 * b + 1];
+        --b;
+      }
+      out[b + 1] = v;
+      vals[2 * (b + 1) + 1] = vj;
+    }
+  }
+  return cnt;
+}
+
+__global__ __launch_bounds__(kRow) void k_b_search(const BEns* __restrict__ ens,
+                                                   const BDev* __restrict__ dev,
+                                                   const int2* __restrict__ rows,
+                                                   const BHead* __restrict__ head,
+                                                   const int* __restrict__ active,
+                                                   const double* __restrict__ pos,
+                                                   const double* __restrict__ radii,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ sorted,
+                                                   const double4* __restrict__ spos,
+                                                   int* __restrict__ pc,
+                                                   int* __restrict__ pair_count) {
+  __shared__ int s_sum[kRow / 64];
+  if ((int)blockIdx.x >= head->n_rows) return;
+  const int2 row = rows[blockIdx.x];
+  if (!hashed(head, active, dev, row.x)) return;  // uniform over the block
+  const BEns& E = ens[row.x];
+  const int i = row.y + threadIdx.x;
+  int c = 0;
+  if (i < E.n) {
+    c = search_target<false>(E, dev[row.x], pos + 3 * (size_t)E.off, radii + E.off,
+                             start + E.start_off, sorted + E.off, spos + E.off, i, nullptr,
+                             nullptr, 0);
+    pc[(size_t)E.off + i] = c;
+  }
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
+  if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+    if (tot) atomicAdd(&pair_count[row.x], tot);
+  }
+}
+
+// poff of a system: exclusive scan of its per-target pair counts (no total:
+// poff[off + n] may be the next system's first slot).
+__global__ __launch_bounds__(kScanWG) void k_b_pair_scan(const BEns* __restrict__ ens,
+                                                         const BHead* __restrict__ head,
+                                                         const int* __restrict__ pair_count,
+                                                         const int* __restrict__ pc,
+                                                         int* __restrict__ poff) {
+  const int e = blockIdx.x;
+  if (e >= head->n_ens || pair_count[e] == 0) return;
+  const BEns& E = ens[e];
+  block_scan(pc + E.off, poff + E.off, E.n, false);
+}
+
+// Where each system's pairs start in the packed pair buffer.
+__global__ __launch_bounds__(kScanWG) void k_b_pair_base(const BHead* __restrict__ head,
+                                                         const int* __restrict__ pair_count,
+                                                         int* __restrict__ pair_base) {
+  block_scan(pair_count, pair_base, head->n_ens, true);
+}
+
+__global__ __launch_bounds__(kRow) void k_b_emit(const BEns* __restrict__ ens,
+                                                 const BDev* __restrict__ dev,
+                                                 const int2* __restrict__ rows,
+                                                 const BHead* __restrict__ head,
+                                                 const int* __restrict__ pair_count,
+                                                 const double* __restrict__ pos,
+                                                 const double* __restrict__ radii,
+                                                 const int* __restrict__ start,
+                                                 const int* __restrict__ sorted,
+                                                 const double4* __restrict__ spos,
+                                                 const int* __restrict__ pc,
+                                                 const int* __restrict__ poff,
+                                                 const int* __restrict__ pair_base,
+                                                 int2* __restrict__ pairs,
+                                                 double4* __restrict__ pvals) {
+  if ((int)blockIdx.x >= head->n_rows) return;
+  const int2 row = rows[blockIdx.x];
+  if (pair_count[row.x] == 0) return;
+  const BEns& E = ens[row.x];
+  const int i = row.y + threadIdx.x;
+  if (i >= E.n) return;
+  const size_t g = (size_t)E.off + i;
+  if (pc[g] == 0) return;
+  const long long at = (long long)pair_base[row.x] + poff[g];
+  const long long room = min((long long)pc[g], (long long)head->pair_cap - at);
+  if (room <= 0) return;
+  search_target<true>(E, dev[row.x], pos + 3 * (size_t)E.off, radii + E.off, start + E.start_off,
+                      sorted + E.off, spos + E.off, i, pairs + at, pvals + 2 * at, (int)room);
+}
+
+// ---------------------------------------------------------------- advance graph
+
+// go = active, no contact found, not advanced yet in this global step; the
+// rest of the advance launch reads only go, so marking "stepped" here is safe.
+__global__ void k_b_go(const BHead* __restrict__ head, const int* __restrict__ active,
+                       const int* __restrict__ pair_count, int* __restrict__ stepped,
+                       int* __restrict__ go) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= head->n_ens) return;
+  const int g = (active[e] && pair_count[e] == 0 && !stepped[e]) ? 1 : 0;
+  go[e] = g;
+  if (g) stepped[e] = 1;
+}
 
 template <int BW>
 __global__ __launch_bounds__(BW) void k_b_force(const BEns* __restrict__ ens,
                                                 const int4* __restrict__ items,
                                                 const BHead* __restrict__ head,
-                                                const int* __restrict__ adv,
+                                                const int* __restrict__ go,
                                                 const Rec* __restrict__ rec, int tile,
                                                 double* __restrict__ part) {
   if ((int)blockIdx.x >= head->n_items) return;
   const int4 it = items[blockIdx.x];
-  if (!adv[it.x]) return;
+  if (!go[it.x]) return;
   const BEns& E = ens[it.x];
   const int n = E.n;
   const Rec* r = rec + E.off;
@@ -366,16 +663,17 @@ __global__ __launch_bounds__(BW) void k_b_force(const BEns* __restrict__ ens,
 }
 
 __global__ __launch_bounds__(kRow) void k_b_force_reduce(const BEns* __restrict__ ens,
+                                                         const BDev* __restrict__ dev,
                                                          const int2* __restrict__ rows,
                                                          const BHead* __restrict__ head,
-                                                         const int* __restrict__ adv,
+                                                         const int* __restrict__ go,
                                                          const double* __restrict__ part,
                                                          const double* __restrict__ radii,
                                                          double* __restrict__ F,
                                                          RecC* __restrict__ recc) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (!adv[row.x]) return;
+  if (!go[row.x]) return;
   const BEns& E = ens[row.x];
   const int n = E.n;
   const int i = row.y + threadIdx.x;
@@ -387,25 +685,23 @@ __global__ __launch_bounds__(kRow) void k_b_force_reduce(const BEns* __restrict_
   const size_t g = (size_t)E.off + i;
   double Fi[3];
   float fr[3];
-  dem::force_finish(sum, radii[g], E.sc, E.m_const, Fi, fr);
+  dem::force_finish(sum, radii[g], dev[row.x].sc, E.m_const, Fi, fr);
   for (int a = 0; a < 3; ++a) F[3 * g + a] = Fi[a];
   recc[g].fx = fr[0];
   recc[g].fy = fr[1];
   recc[g].fz = fr[2];
 }
 
-// ---------------------------------------------------------------- 4-5. Oseen + Euler
-
 template <int BW>
 __global__ __launch_bounds__(BW) void k_b_conv(const BEns* __restrict__ ens,
                                                const int4* __restrict__ items,
                                                const BHead* __restrict__ head,
-                                               const int* __restrict__ adv,
+                                               const int* __restrict__ go,
                                                const RecC* __restrict__ recc, int tile,
                                                double* __restrict__ part) {
   if ((int)blockIdx.x >= head->n_items) return;
   const int4 it = items[blockIdx.x];
-  if (!adv[it.x]) return;
+  if (!go[it.x]) return;
   const BEns& E = ens[it.x];
   const int n = E.n;
   const RecC* r = recc + E.off;
@@ -426,23 +722,17 @@ __global__ __launch_bounds__(BW) void k_b_conv(const BEns* __restrict__ ens,
   }
 }
 
-__global__ __launch_bounds__(kRow) void k_b_conv_euler(const BEns* __restrict__ ens,
-                                                       const int2* __restrict__ rows,
-                                                       const BHead* __restrict__ head,
-                                                       const int* __restrict__ adv,
-                                                       const double* __restrict__ part,
-                                                       const double* __restrict__ radii,
-                                                       const double* __restrict__ F,
-                                                       double* __restrict__ V,
-                                                       double* __restrict__ pos,
-                                                       Rec* __restrict__ rec,
-                                                       RecC* __restrict__ recc,
-                                                       int* __restrict__ steps_done,
-                                                       double* __restrict__ time) {
+__global__ __launch_bounds__(kRow) void k_b_conv_euler(
+    const BEns* __restrict__ ens, const BDev* __restrict__ dev, const int2* __restrict__ rows,
+    const BHead* __restrict__ head, const int* __restrict__ go, const double* __restrict__ part,
+    const double* __restrict__ radii, const double* __restrict__ F, double* __restrict__ V,
+    double* __restrict__ pos, Rec* __restrict__ rec, RecC* __restrict__ recc,
+    int* __restrict__ steps_done, double* __restrict__ time) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (!adv[row.x]) return;
+  if (!go[row.x]) return;
   const BEns& E = ens[row.x];
+  const DevScales sc = dev[row.x].sc;
   const int n = E.n;
   const int i = row.y + threadIdx.x;
   if (row.y == 0 && threadIdx.x == 0) {
@@ -457,9 +747,9 @@ __global__ __launch_bounds__(kRow) void k_b_conv_euler(const BEns* __restrict__ 
   for (int s = 0; s < E.S; ++s)
     for (int a = 0; a < 3; ++a) m[a] += pp[(size_t)(6 * s + a) * n + i];
   const size_t g = (size_t)E.off + i;
-  dem::CorrScalars cs{};
+  const dem::CorrScalars cs{};
   double Vi[3];
-  dem::conv_finish(m, c, E.sc, E.eta_const, cs, F + 3 * g, Vi);
+  dem::conv_finish(m, c, sc, E.eta_const, cs, F + 3 * g, Vi);
   const double ri = radii[g];
   double x[3];
   for (int a = 0; a < 3; ++a) {
@@ -469,7 +759,7 @@ __global__ __launch_bounds__(kRow) void k_b_conv_euler(const BEns* __restrict__ 
   }
   Rec o;
   RecC cc;
-  dem::pack_one(x, ri, nullptr, E.sc, o, cc);
+  dem::pack_one(x, ri, nullptr, sc, o, cc);
   rec[g] = o;
   recc[g] = cc;
 }
@@ -482,14 +772,18 @@ struct dem_batch {
   int n_ens = 0;
   dem_batch_tuning tune{};
   std::vector<dem_batch_ensemble> desc;
+  std::vector<int> order;  // systems by decreasing capacity (item order)
   Plan plan;
   BLayout l{};
   double *pos = nullptr, *radii = nullptr, *F = nullptr, *V = nullptr;
+  std::vector<BEns> ens_h;
   std::vector<int4> items_h;
   std::vector<int2> rows_h;
+  std::vector<int> stage_h;
   hipStream_t cap_stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // 0: first check of a global step, 1: check again, 2: advance
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -505,6 +799,7 @@ int check_tuning(const dem_batch_tuning* t) {
 
 int check_ensembles(const dem_batch_ensemble* ens, int n_ens) {
   if (n_ens <= 0 || !ens) return dem::set_error(DEM_ERR_INVALID, "need at least one ensemble");
+  std::vector<int> idx(n_ens);
   for (int e = 0; e < n_ens; ++e) {
     const dem_batch_ensemble& d = ens[e];
     if (d.offset < 0 || d.capacity < 0 || d.n < 0 || d.n > d.capacity)
@@ -513,33 +808,33 @@ int check_ensembles(const dem_batch_ensemble* ens, int n_ens) {
       return dem::set_error(DEM_ERR_CAPACITY, "ensemble %d: capacity %d too large", e, d.capacity);
     if (!(d.L > 0.0)) return dem::set_error(DEM_ERR_INVALID, "ensemble %d: L must be > 0", e);
     if (d.steps < 0) return dem::set_error(DEM_ERR_INVALID, "ensemble %d: steps < 0", e);
-    for (int f = 0; f < e; ++f) {  // reserved ranges must not overlap
-      const dem_batch_ensemble& o = ens[f];
-      const long long a0 = d.offset, a1 = (long long)d.offset + d.capacity;
-      const long long b0 = o.offset, b1 = (long long)o.offset + o.capacity;
-      if (a0 < b1 && b0 < a1 && d.capacity > 0 && o.capacity > 0)
-        return dem::set_error(DEM_ERR_INVALID, "ensembles %d and %d overlap", f, e);
-    }
+    idx[e] = e;
+  }
+  // reserved ranges must not overlap
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return ens[a].offset < ens[b].offset; });
+  for (int k = 1; k < n_ens; ++k) {
+    const dem_batch_ensemble& a = ens[idx[k - 1]];
+    const dem_batch_ensemble& b = ens[idx[k]];
+    if (a.capacity > 0 && b.capacity > 0 && (long long)a.offset + a.capacity > b.offset)
+      return dem::set_error(DEM_ERR_INVALID, "ensembles %d and %d overlap", idx[k - 1], idx[k]);
   }
   return DEM_OK;
 }
 
-// Rebuilds items / rows / device system table from the host descriptions.
-int upload_tables(dem_batch* b, hipStream_t st, bool full) {
+// Rebuilds the system table, items and rows from the host descriptions and
+// uploads them (synchronising `st` so the host tables can be reused).
+int upload_tables(dem_batch* b, hipStream_t st) {
   const int bw = b->tune.block;
-  std::vector<BEns> eh(b->n_ens);
-  struct Cost {
-    long long c;
-    int4 it;
-  };
-  std::vector<Cost> items;
+  b->ens_h.resize(b->n_ens);
+  b->items_h.clear();
   b->rows_h.clear();
   for (int e = 0; e < b->n_ens; ++e) {
     const dem_batch_ensemble& d = b->desc[e];
-    BEns& E = eh[e];
+    BEns& E = b->ens_h[e];
     E = BEns{};
     E.part_off = b->plan.part_off[e];
-    E.hash_off = b->plan.hash_off[e];
+    E.cnt_off = b->plan.cnt_off[e];
+    E.start_off = b->plan.start_off[e];
     E.off = d.offset;
     E.n = d.n;
     E.cap = d.capacity;
@@ -552,34 +847,27 @@ int upload_tables(dem_batch* b, hipStream_t st, bool full) {
     E.sf = d.stokes_factor;
     E.m_const = d.m_const;
     E.eta_const = d.eta_const;
-    for (int i0 = 0; i0 < d.n; i0 += bw)
-      for (int s = 0; s < E.S; ++s) {
-        const int j0 = s * E.jchunk, j1 = std::min(d.n, j0 + E.jchunk);
-        if (j1 <= j0) continue;
-        const long long cost = (long long)std::min(bw, d.n - i0) * (j1 - j0);
-        items.push_back({cost, make_int4(e, i0, s, 0)});
-      }
-    for (int i0 = 0; i0 < d.n; i0 += kRow) b->rows_h.push_back(make_int2(e, i0));
   }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Cost& a, const Cost& c) { return a.c > c.c; });
-  b->items_h.resize(items.size());
-  for (size_t k = 0; k < items.size(); ++k) b->items_h[k] = items[k].it;
-  if ((long long)b->items_h.size() > b->plan.items_cap || (long long)b->rows_h.size() > b->plan.rows_cap)
+  // items of the largest systems first (their items take longest)
+  for (int e : b->order) {
+    const BEns& E = b->ens_h[e];
+    for (int i0 = 0; i0 < E.n; i0 += bw)
+      for (int s = 0; s < E.S; ++s)
+        if (s * E.jchunk < E.n) b->items_h.push_back(make_int4(e, i0, s, 0));
+  }
+  for (int e = 0; e < b->n_ens; ++e)
+    for (int i0 = 0; i0 < b->ens_h[e].n; i0 += kRow) b->rows_h.push_back(make_int2(e, i0));
+  if ((long long)b->items_h.size() > b->plan.items_cap ||
+      (long long)b->rows_h.size() > b->plan.rows_cap)
     return dem::set_error(DEM_ERR_CAPACITY, "work tables exceed their capacity");
-  BHead h{(int)b->items_h.size(), (int)b->rows_h.size(), b->n_ens, 0};
-  if (full) {
-    DEM_CHECK_HIP(hipMemcpyAsync(b->l.ens, eh.data(), sizeof(BEns) * eh.size(),
-                                 hipMemcpyHostToDevice, st));
-  } else {
-    // keep the device-computed scales: copy everything after them
-    for (int e = 0; e < b->n_ens; ++e) {
-      const size_t skip = offsetof(BEns, part_off);
-      DEM_CHECK_HIP(hipMemcpyAsync(reinterpret_cast<char*>(b->l.ens + e) + skip,
-                                   reinterpret_cast<const char*>(&eh[e]) + skip,
-                                   sizeof(BEns) - skip, hipMemcpyHostToDevice, st));
-    }
-  }
+  BHead h{};
+  h.n_items = (int)b->items_h.size();
+  h.n_rows = (int)b->rows_h.size();
+  h.n_ens = b->n_ens;
+  h.contacts = b->tune.contacts ? 1 : 0;
+  h.pair_cap = (int)b->plan.pair_total;
+  DEM_CHECK_HIP(hipMemcpyAsync(b->l.ens, b->ens_h.data(), sizeof(BEns) * b->ens_h.size(),
+                               hipMemcpyHostToDevice, st));
   if (!b->items_h.empty())
     DEM_CHECK_HIP(hipMemcpyAsync(b->l.items, b->items_h.data(), sizeof(int4) * b->items_h.size(),
                                  hipMemcpyHostToDevice, st));
@@ -587,81 +875,160 @@ int upload_tables(dem_batch* b, hipStream_t st, bool full) {
     DEM_CHECK_HIP(hipMemcpyAsync(b->l.rows, b->rows_h.data(), sizeof(int2) * b->rows_h.size(),
                                  hipMemcpyHostToDevice, st));
   DEM_CHECK_HIP(hipMemcpyAsync(b->l.head, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  // host tables are reused by the next update: wait for the copies
   DEM_CHECK_HIP(hipStreamSynchronize(st));
+  return DEM_OK;
+}
+
+// Hash cells and records of every system (after any change of radii/positions).
+int refresh(dem_batch* b, hipStream_t st) {
+  const BLayout& l = b->l;
+  hipLaunchKernelGGL(k_b_cells, dim3(b->n_ens), dim3(256), 0, st, (const BEns*)l.ens,
+                     (const BHead*)l.head, (const double*)b->radii, l.dev);
+  DEM_CHECK_LAUNCH("k_b_cells");
+  hipLaunchKernelGGL(k_b_pack, dim3((unsigned)std::max<long long>(b->plan.rows_cap, 1)),
+                     dim3(kRow), 0, st, (const BEns*)l.ens, (const BDev*)l.dev,
+                     (const int2*)l.rows, (const BHead*)l.head, (const double*)b->pos,
+                     (const double*)b->radii, l.rec, l.recc);
+  DEM_CHECK_LAUNCH("k_b_pack");
   return DEM_OK;
 }
 
 int tile_force(const dem_batch* b) { return b->tune.force_tile > 0 ? b->tune.force_tile : 32; }
 int tile_conv(const dem_batch* b) { return b->tune.conv_tile > 0 ? b->tune.conv_tile : 16; }
 
-template <int BW>
-int launch_items(dem_batch* b, hipStream_t st, bool prof) {
+int enqueue_check(dem_batch* b, hipStream_t st, int new_step) {
   const BLayout& l = b->l;
-  const int gi = (int)std::max<long long>(b->plan.items_cap, 1);
-  const int gr = (int)std::max<long long>(b->plan.rows_cap, 1);
-  if (prof) {
-    DEM_PROF_LAUNCH("k_b_force", st,
-                    hipLaunchKernelGGL(k_b_force<BW>, dim3(gi), dim3(BW), 0, st,
-                                       (const BEns*)l.ens, (const int4*)l.items,
-                                       (const BHead*)l.head, (const int*)l.adv,
-                                       (const Rec*)l.rec, tile_force(b), l.part));
-  } else {
+  const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
+  hipLaunchKernelGGL(k_b_begin, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
+                     (const BEns*)l.ens, (const BHead*)l.head, new_step,
+                     (const int*)l.steps_done, l.stepped, l.active, l.pair_count);
+  DEM_CHECK_LAUNCH("k_b_begin");
+  if (!b->tune.contacts) return DEM_OK;
+  hipLaunchKernelGGL(k_b_zero, dim3(kZeroBlocks), dim3(256), 0, st, l.cnt, b->plan.cnt_total);
+  DEM_CHECK_LAUNCH("k_b_zero");
+  hipLaunchKernelGGL(k_b_assign, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.active, (const double*)b->pos, l.cnt, l.bucket, l.rank);
+  DEM_CHECK_LAUNCH("k_b_assign");
+  hipLaunchKernelGGL(k_b_bucket_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active,
+                     (const int*)l.cnt, l.start);
+  DEM_CHECK_LAUNCH("k_b_bucket_scan");
+  hipLaunchKernelGGL(k_b_scatter, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.active, (const int*)l.start, (const int*)l.bucket,
+                     (const int*)l.rank, (const double*)b->pos, (const double*)b->radii, l.sorted,
+                     l.spos);
+  DEM_CHECK_LAUNCH("k_b_scatter");
+  hipLaunchKernelGGL(k_b_search, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.active, (const double*)b->pos, (const double*)b->radii,
+                     (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos, l.pc,
+                     l.pair_count);
+  DEM_CHECK_LAUNCH("k_b_search");
+  hipLaunchKernelGGL(k_b_pair_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
+                     (const BHead*)l.head, (const int*)l.pair_count, (const int*)l.pc, l.poff);
+  DEM_CHECK_LAUNCH("k_b_pair_scan");
+  hipLaunchKernelGGL(k_b_pair_base, dim3(1), dim3(kScanWG), 0, st, (const BHead*)l.head,
+                     (const int*)l.pair_count, l.pair_base);
+  DEM_CHECK_LAUNCH("k_b_pair_base");
+  hipLaunchKernelGGL(k_b_emit, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.pair_count, (const double*)b->pos, (const double*)b->radii,
+                     (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos,
+                     (const int*)l.pc, (const int*)l.poff, (const int*)l.pair_base, l.pairs,
+                     l.pvals);
+  DEM_CHECK_LAUNCH("k_b_emit");
+  return DEM_OK;
+}
+
+template <int BW>
+int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
+  const BLayout& l = b->l;
+  const unsigned gi = (unsigned)std::max<long long>(b->plan.items_cap, 1);
+  const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
+  hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
+                     (const BHead*)l.head, (const int*)l.active, (const int*)l.pair_count,
+                     l.stepped, l.go);
+  DEM_CHECK_LAUNCH("k_b_go");
+  auto force = [&] {
     hipLaunchKernelGGL(k_b_force<BW>, dim3(gi), dim3(BW), 0, st, (const BEns*)l.ens,
-                       (const int4*)l.items, (const BHead*)l.head, (const int*)l.adv,
+                       (const int4*)l.items, (const BHead*)l.head, (const int*)l.go,
                        (const Rec*)l.rec, tile_force(b), l.part);
+  };
+  auto conv = [&] {
+    hipLaunchKernelGGL(k_b_conv<BW>, dim3(gi), dim3(BW), 0, st, (const BEns*)l.ens,
+                       (const int4*)l.items, (const BHead*)l.head, (const int*)l.go,
+                       (const RecC*)l.recc, tile_conv(b), l.part);
+  };
+  if (prof) {
+    DEM_PROF_LAUNCH("k_b_force", st, force());
+  } else {
+    force();
   }
   DEM_CHECK_LAUNCH("k_b_force");
   hipLaunchKernelGGL(k_b_force_reduce, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
-                     (const int2*)l.rows, (const BHead*)l.head, (const int*)l.adv,
-                     (const double*)l.part, (const double*)b->radii, b->F, l.recc);
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.go, (const double*)l.part, (const double*)b->radii, b->F,
+                     l.recc);
   DEM_CHECK_LAUNCH("k_b_force_reduce");
   if (prof) {
-    DEM_PROF_LAUNCH("k_b_conv", st,
-                    hipLaunchKernelGGL(k_b_conv<BW>, dim3(gi), dim3(BW), 0, st,
-                                       (const BEns*)l.ens, (const int4*)l.items,
-                                       (const BHead*)l.head, (const int*)l.adv,
-                                       (const RecC*)l.recc, tile_conv(b), l.part));
+    DEM_PROF_LAUNCH("k_b_conv", st, conv());
   } else {
-    hipLaunchKernelGGL(k_b_conv<BW>, dim3(gi), dim3(BW), 0, st, (const BEns*)l.ens,
-                       (const int4*)l.items, (const BHead*)l.head, (const int*)l.adv,
-                       (const RecC*)l.recc, tile_conv(b), l.part);
+    conv();
   }
   DEM_CHECK_LAUNCH("k_b_conv");
   hipLaunchKernelGGL(k_b_conv_euler, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
-                     (const int2*)l.rows, (const BHead*)l.head, (const int*)l.adv,
-                     (const double*)l.part, (const double*)b->radii, (const double*)b->F, b->V,
-                     b->pos, l.rec, l.recc, l.steps_done, l.time);
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.go, (const double*)l.part, (const double*)b->radii,
+                     (const double*)b->F, b->V, b->pos, l.rec, l.recc, l.steps_done, l.time);
   DEM_CHECK_LAUNCH("k_b_conv_euler");
   return DEM_OK;
 }
 
-int enqueue_step(dem_batch* b, hipStream_t st, bool prof) {
-  const BLayout& l = b->l;
-  if (prof) {
-    DEM_PROF_LAUNCH("k_b_contact", st,
-                    hipLaunchKernelGGL(k_b_contact, dim3(b->n_ens), dim3(kContactWG), 0, st,
-                                       (const BEns*)l.ens, (const BHead*)l.head,
-                                       (const double*)b->pos, (const double*)b->radii, l.hash,
-                                       l.contact, l.adv, (const int*)l.steps_done));
-  } else {
-    hipLaunchKernelGGL(k_b_contact, dim3(b->n_ens), dim3(kContactWG), 0, st, (const BEns*)l.ens,
-                       (const BHead*)l.head, (const double*)b->pos, (const double*)b->radii,
-                       l.hash, l.contact, l.adv, (const int*)l.steps_done);
-  }
-  DEM_CHECK_LAUNCH("k_b_contact");
+int enqueue_advance(dem_batch* b, hipStream_t st, bool prof) {
   switch (b->tune.block) {
-    case 64: return launch_items<64>(b, st, prof);
-    case 128: return launch_items<128>(b, st, prof);
-    default: return launch_items<256>(b, st, prof);
+    case 64: return enqueue_advance_bw<64>(b, st, prof);
+    case 128: return enqueue_advance_bw<128>(b, st, prof);
+    default: return enqueue_advance_bw<256>(b, st, prof);
   }
 }
 
-void drop_graph(dem_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+int enqueue(dem_batch* b, int k, hipStream_t st, bool prof) {
+  return k == 2 ? enqueue_advance(b, st, prof) : enqueue_check(b, st, k == 0 ? 1 : 0);
+}
+
+void drop_graphs(dem_batch* b) {
+  for (int k = 0; k < 3; ++k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
+  }
+}
+
+int capture_one(dem_batch* b, int k) {
+  DEM_CHECK_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed));
+  int rc = enqueue(b, k, b->cap_stream, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return dem::set_error(DEM_ERR_HIP, "capture: %s", hipGetErrorString(e));
+  b->graph[k] = g;
+  e = hipGraphInstantiate(&b->exec[k], g, nullptr, nullptr, 0);
+  if (e != hipSuccess) return dem::set_error(DEM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  return DEM_OK;
+}
+
+int launch(dem_batch* b, int k, hipStream_t st) {
+  if (b->exec[k]) {
+    DEM_CHECK_HIP(hipGraphLaunch(b->exec[k], st));
+    return DEM_OK;
+  }
+  return enqueue(b, k, st, true);
 }
 
 }  // namespace
@@ -698,6 +1065,10 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   b->n_ens = n_ens;
   b->tune = *tuning;
   b->desc.assign(ens, ens + n_ens);
+  b->order.resize(n_ens);
+  std::iota(b->order.begin(), b->order.end(), 0);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return ens[x].capacity > ens[y].capacity; });
   b->plan = p;
   b->l = l;
   b->pos = pos;
@@ -705,20 +1076,15 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   b->F = forces;
   b->V = velocities;
   hipStream_t st = dem::as_stream(stream);
-  rc = upload_tables(b, st, true);
+  rc = upload_tables(b, st);
   if (!rc) {
-    hipLaunchKernelGGL(k_b_init, dim3((n_ens + 255) / 256), dim3(256), 0, st, l.ens, n_ens,
-                       l.contact, l.adv, l.steps_done, l.time);
+    hipLaunchKernelGGL(k_b_init, dim3((n_ens + 255) / 256), dim3(256), 0, st,
+                       (const BEns*)l.ens, l.dev, n_ens, l.active, l.stepped, l.go,
+                       l.pair_count, l.steps_done, l.time);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = dem::set_error(DEM_ERR_HIP, "k_b_init: %s", hipGetErrorString(e));
   }
-  if (!rc) {
-    hipLaunchKernelGGL(k_b_pack, dim3((unsigned)std::max<long long>(p.rows_cap, 1)), dim3(kRow), 0,
-                       st, (const BEns*)l.ens, (const int2*)l.rows, (const BHead*)l.head, -1,
-                       (const double*)pos, (const double*)radii, l.rec, l.recc);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = dem::set_error(DEM_ERR_HIP, "k_b_pack: %s", hipGetErrorString(e));
-  }
+  if (!rc) rc = refresh(b, st);
   if (rc) {
     delete b;
     return rc;
@@ -727,50 +1093,97 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   return DEM_OK;
 }
 
-int dem_batch_update(dem_batch_t b, int e, int n, dem_stream_t stream) {
+int dem_batch_resize(dem_batch_t b, int count, const int32_t* systems, const int32_t* n,
+                     dem_stream_t stream) {
   if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
-  if (e < 0 || e >= b->n_ens) return dem::set_error(DEM_ERR_INVALID, "no ensemble %d", e);
-  if (n < 0 || n > b->desc[e].capacity)
-    return dem::set_error(DEM_ERR_CAPACITY, "ensemble %d: n=%d outside [0, %d]", e, n,
-                          b->desc[e].capacity);
+  if (count < 0 || (count > 0 && (!systems || !n)))
+    return dem::set_error(DEM_ERR_INVALID, "bad system list");
+  for (int k = 0; k < count; ++k) {
+    const int e = systems[k];
+    if (e < 0 || e >= b->n_ens) return dem::set_error(DEM_ERR_INVALID, "no ensemble %d", e);
+    if (n[k] < 0 || n[k] > b->desc[e].capacity)
+      return dem::set_error(DEM_ERR_CAPACITY, "ensemble %d: n=%d outside [0, %d]", e, n[k],
+                            b->desc[e].capacity);
+  }
   hipStream_t st = dem::as_stream(stream);
   DEM_CHECK_HIP(hipStreamSynchronize(st));
-  b->desc[e].n = n;
-  int rc = upload_tables(b, st, false);
+  for (int k = 0; k < count; ++k) b->desc[systems[k]].n = n[k];
+  int rc = upload_tables(b, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_b_pack, dim3((unsigned)std::max<long long>(b->plan.rows_cap, 1)),
-                     dim3(kRow), 0, st, (const BEns*)b->l.ens, (const int2*)b->l.rows,
-                     (const BHead*)b->l.head, e, (const double*)b->pos, (const double*)b->radii,
-                     b->l.rec, b->l.recc);
-  DEM_CHECK_LAUNCH("k_b_pack");
-  return DEM_OK;
+  return refresh(b, st);
 }
 
-int dem_batch_step(dem_batch_t b, dem_stream_t stream) {
+int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int32_t* removed_off,
+                    const int32_t* removed, const int32_t* merged_off, const double* merged,
+                    dem_stream_t stream) {
   if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
-  return enqueue_step(b, dem::as_stream(stream), true);
+  if (count < 0 || count > b->n_ens) return dem::set_error(DEM_ERR_INVALID, "bad count %d", count);
+  if (count == 0) return DEM_OK;
+  if (!systems || !removed_off || !merged_off || (removed_off[count] > 0 && !removed) ||
+      (merged_off[count] > 0 && !merged))
+    return dem::set_error(DEM_ERR_INVALID, "null merge list");
+  if (removed_off[0] != 0 || merged_off[0] != 0)
+    return dem::set_error(DEM_ERR_INVALID, "merge lists must start at offset 0");
+  std::vector<int> seen(b->n_ens, 0);
+  std::vector<int> new_n(count);
+  for (int k = 0; k < count; ++k) {
+    const int e = systems[k];
+    if (e < 0 || e >= b->n_ens || seen[e]++)
+      return dem::set_error(DEM_ERR_INVALID, "bad or repeated system %d", e);
+    const int n = b->desc[e].n;
+    const int r0 = removed_off[k], r1 = removed_off[k + 1];
+    const int m0 = merged_off[k], m1 = merged_off[k + 1];
+    if (r1 < r0 || m1 < m0) return dem::set_error(DEM_ERR_INVALID, "offsets must not decrease");
+    for (int q = r0; q < r1; ++q)
+      if (removed[q] < 0 || removed[q] >= n || (q > r0 && removed[q] <= removed[q - 1]))
+        return dem::set_error(DEM_ERR_INVALID,
+                              "system %d: removed indices must be ascending in [0, %d)", e, n);
+    new_n[k] = n - (r1 - r0) + (m1 - m0);
+    if (new_n[k] < 0 || new_n[k] > b->desc[e].capacity)
+      return dem::set_error(DEM_ERR_CAPACITY, "system %d: %d droplets after merging", e, new_n[k]);
+  }
+  const long long nr = removed_off[count], nm = merged_off[count];
+  if (nr > b->plan.cap_total || nm > b->plan.cap_total)
+    return dem::set_error(DEM_ERR_CAPACITY, "merge lists exceed the staging area");
+  hipStream_t st = dem::as_stream(stream);
+  DEM_CHECK_HIP(hipStreamSynchronize(st));  // stage_h may still feed a copy
+  b->stage_h.assign(systems, systems + count);
+  b->stage_h.insert(b->stage_h.end(), removed_off, removed_off + count + 1);
+  b->stage_h.insert(b->stage_h.end(), merged_off, merged_off + count + 1);
+  if (nr) b->stage_h.insert(b->stage_h.end(), removed, removed + nr);
+  DEM_CHECK_HIP(hipMemcpyAsync(b->l.stage_i, b->stage_h.data(), sizeof(int) * b->stage_h.size(),
+                               hipMemcpyHostToDevice, st));
+  if (nm)
+    DEM_CHECK_HIP(hipMemcpyAsync(b->l.stage_d, merged, sizeof(double) * 4 * nm,
+                                 hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_b_compact, dim3(count), dim3(256), 0, st, (const BEns*)b->l.ens, count,
+                     (const int*)b->l.stage_i, (const double*)b->l.stage_d, b->pos, b->radii,
+                     b->l.spos);
+  DEM_CHECK_LAUNCH("k_b_compact");
+  for (int k = 0; k < count; ++k) b->desc[systems[k]].n = new_n[k];
+  int rc = upload_tables(b, st);  // synchronises: the host lists may be freed after return
+  if (rc) return rc;
+  return refresh(b, st);
+}
+
+int dem_batch_check(dem_batch_t b, int new_step, dem_stream_t stream) {
+  if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
+  return launch(b, new_step ? 0 : 1, dem::as_stream(stream));
+}
+
+int dem_batch_advance(dem_batch_t b, dem_stream_t stream) {
+  if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
+  return launch(b, 2, dem::as_stream(stream));
 }
 
 int dem_batch_capture(dem_batch_t b) {
   if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
-  drop_graph(b);
+  drop_graphs(b);
   if (!b->cap_stream) DEM_CHECK_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
-  DEM_CHECK_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed));
-  int rc = enqueue_step(b, b->cap_stream, false);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-  if (rc) {
-    if (g) hipGraphDestroy(g);
-    return rc;
-  }
-  if (e != hipSuccess) return dem::set_error(DEM_ERR_HIP, "capture: %s", hipGetErrorString(e));
-  b->graph = g;
-  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    drop_graph(b);
-    return dem::set_error(DEM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
-  }
-  return DEM_OK;
+  int rc = DEM_OK;
+  for (int k = 0; k < 3 && !rc; ++k) rc = capture_one(b, k);
+  if (rc) drop_graphs(b);
+  return rc;
 }
 
 int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream) {
@@ -778,12 +1191,9 @@ int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream) {
   if (steps < 0) return dem::set_error(DEM_ERR_INVALID, "steps < 0");
   hipStream_t st = dem::as_stream(stream);
   for (int k = 0; k < steps; ++k) {
-    if (b->exec) {
-      DEM_CHECK_HIP(hipGraphLaunch(b->exec, st));
-    } else {
-      int rc = enqueue_step(b, st, true);
-      if (rc) return rc;
-    }
+    int rc = launch(b, 0, st);
+    if (!rc) rc = launch(b, 2, st);
+    if (rc) return rc;
   }
   return DEM_OK;
 }
@@ -791,14 +1201,19 @@ int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream) {
 int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes) {
   if (!b || !name || !ptr || !bytes) return dem::set_error(DEM_ERR_INVALID, "null argument");
   const size_t ne = (size_t)b->n_ens;
+  const size_t pc = (size_t)std::max<long long>(b->plan.pair_total, 1);
   struct F {
     const char* n;
     void* p;
     size_t sz;
-  } fields[] = {{"contact", b->l.contact, ne * sizeof(int)},
-                {"advanced", b->l.adv, ne * sizeof(int)},
+  } fields[] = {{"active", b->l.active, ne * sizeof(int)},
+                {"stepped", b->l.stepped, ne * sizeof(int)},
+                {"pair_count", b->l.pair_count, ne * sizeof(int)},
+                {"pair_base", b->l.pair_base, (ne + 1) * sizeof(int)},
                 {"steps_done", b->l.steps_done, ne * sizeof(int)},
-                {"time", b->l.time, ne * sizeof(double)}};
+                {"time", b->l.time, ne * sizeof(double)},
+                {"pairs", b->l.pairs, pc * sizeof(int2)},
+                {"pair_values", b->l.pvals, 2 * pc * sizeof(double4)}};
   for (const F& f : fields)
     if (strcmp(f.n, name) == 0) {
       *ptr = f.p;
@@ -810,8 +1225,8 @@ int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes) 
 
 int dem_batch_destroy(dem_batch_t b) {
   if (!b) return DEM_OK;
-  drop_graph(b);
-  if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+  drop_graphs(b);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   delete b;
   return DEM_OK;
 }

@@ -32,7 +32,7 @@
 namespace {
 
 using dem::cell_of;
-using dem::contact;
+using dem::contact_pair;
 using dem::hash_cell;
 using dem::pymod;
 
@@ -150,15 +150,20 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ pos,
     return;
   }
   const int g = hp.g;
-  const int ix = cell_of(pos[3 * (size_t)i], hp.cell, g, periodic);
-  const int iy = cell_of(pos[3 * (size_t)i + 1], hp.cell, g, periodic);
-  const int iz = cell_of(pos[3 * (size_t)i + 2], hp.cell, g, periodic);
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  const double ri = r[i];
+  const int ix = cell_of(xi, hp.cell, g, periodic);
+  const int iy = cell_of(yi, hp.cell, g, periodic);
+  const int iz = cell_of(zi, hp.cell, g, periodic);
   int seen[27];
   int nseen = 0;
   int cnt = 0;
   const int base = EMIT ? pair_off[i] : 0;
+#pragma unroll 1
   for (int di = -1; di <= 1; ++di)
+#pragma unroll 1
     for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll 1
       for (int dk = -1; dk <= 1; ++dk) {
         int nx = ix + di, ny = iy + dj, nz = iz + dk;
         if (periodic) {
@@ -177,7 +182,8 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ pos,
         const int end = start + counts[nh];
         for (int slot = start; slot < end; ++slot) {
           const int j = sorted[slot];
-          if (i < j && contact(pos, r, i, j, L, periodic)) {
+          if (i < j && contact_pair(xi, yi, zi, ri, pos[3 * (size_t)j], pos[3 * (size_t)j + 1],
+                                    pos[3 * (size_t)j + 2], r[j], L, periodic)) {
             if (EMIT) {
               const int at = base + cnt;
               if (at < cap) {

@@ -27,13 +27,16 @@ __device__ __forceinline__ int hash_cell(int ix, int iy, int iz, int mask) {
   return (int)(h & (uint32_t)mask);
 }
 
-// Contact predicate in float64, element-wise rounding (no contraction).
-__device__ __noinline__ bool contact(const double* __restrict__ pos, const double* __restrict__ r,
-                                     int i, int j, double L, int periodic) {
+// Contact predicate in float64 with the reference's element-wise rounding
+// (no contraction): d = x_j - x_i per axis, minimum image |d| > L/2 ->
+// d -= sign(d) L, then sqrt((dx^2 + dy^2) + dz^2) <= r_i + r_j (:209-225).
+__device__ __forceinline__ bool contact_pair(double xi, double yi, double zi, double ri,
+                                             double xj, double yj, double zj, double rj,
+                                             double L, int periodic) {
 #pragma clang fp contract(off)
-  double dx = pos[3 * (size_t)j] - pos[3 * (size_t)i];
-  double dy = pos[3 * (size_t)j + 1] - pos[3 * (size_t)i + 1];
-  double dz = pos[3 * (size_t)j + 2] - pos[3 * (size_t)i + 2];
+  double dx = xj - xi;
+  double dy = yj - yi;
+  double dz = zj - zi;
   if (periodic) {
     const double h = L * 0.5;
     if (fabs(dx) > h) dx -= copysign(L, dx);
@@ -43,8 +46,8 @@ __device__ __noinline__ bool contact(const double* __restrict__ pos, const doubl
   double d2 = dx * dx;
   d2 = d2 + dy * dy;
   d2 = d2 + dz * dz;
-  double dist = sqrt(d2);
-  double rs = r[i] + r[j];
+  const double dist = sqrt(d2);
+  const double rs = ri + rj;
   return dist <= rs;
 }
 

@@ -99,6 +99,8 @@ class DemBatchTuning(ctypes.Structure):
         ("jlen", ctypes.c_int32),
         ("force_tile", ctypes.c_int32),
         ("conv_tile", ctypes.c_int32),
+        ("contacts", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -132,8 +134,10 @@ SIGNATURES: dict[str, tuple] = {
     "dem_batch_split": (_I, [_I, _I]),
     "dem_batch_workspace_bytes": (_S, [_P, _I, _P]),
     "dem_batch_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _S, _P, ctypes.POINTER(_P)]),
-    "dem_batch_update": (_I, [_P, _I, _I, _P]),
-    "dem_batch_step": (_I, [_P, _P]),
+    "dem_batch_resize": (_I, [_P, _I, _P, _P, _P]),
+    "dem_batch_check": (_I, [_P, _I, _P]),
+    "dem_batch_advance": (_I, [_P, _P]),
+    "dem_batch_merge": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "dem_batch_capture": (_I, [_P]),
     "dem_batch_run": (_I, [_P, _I, _P]),
     "dem_batch_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_S)]),

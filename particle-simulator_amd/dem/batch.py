@@ -35,7 +35,7 @@ import numpy as np
 from . import _device, _native
 from .collision_detector import SpatialHashCollisionDetector
 from .force_calculator import DirectDropletForceCalculator
-from .solution.droplet_solution import merge_collided
+from .solution.droplet_solution import merge_collided, merge_plan
 
 BLOCKS = (64, 128, 256)
 JLENS = (1024, 2048, 4096, 8192)
@@ -47,9 +47,11 @@ class Tuning:
     jlen: int = 4096   # sources per work item
     force_tile: int = 0
     conv_tile: int = 0
+    contacts: bool = True  # False: no contact pass (solver without a detector)
 
     def native(self) -> _native.DemBatchTuning:
-        return _native.DemBatchTuning(self.block, self.jlen, self.force_tile, self.conv_tile)
+        return _native.DemBatchTuning(self.block, self.jlen, self.force_tile, self.conv_tile,
+                                      1 if self.contacts else 0, 0)
 
 
 def split_for(n: int, jlen: int) -> int:
@@ -64,12 +66,19 @@ def _consts(s):
 
 
 class BatchedEnsembles:
-    """Independent periodic systems stepped by one captured graph per global step.
+    """Independent periodic systems stepped together.
 
     ``systems``: objects with ``positions`` (N,3), ``radii`` (N,), ``L``,
     ``m_const``, ``eta_const``, ``eta_oil`` and ``eta_water`` (e.g.
     ``dem.workloads.System``); ``dts``/``steps``: per-system time step and
     Euler step count.
+
+    One global step (``step``): the contact pass of every active system; while
+    any system has contacts, merge all of them on the host (float64, the
+    reference's greedy merge) and check again; then force + Oseen + Euler for
+    every active system.  Each system therefore goes through exactly the
+    states of the reference loop, and every active system advances once per
+    global step.
     """
 
     def __init__(self, systems, dts, steps, tuning: Tuning | None = None, capture: bool = True):
@@ -111,17 +120,30 @@ class BatchedEnsembles:
                      _device.ptr(self.pos), _device.ptr(self.radii), _device.ptr(self.F),
                      _device.ptr(self.V), _device.ptr(self.ws), self.ws.numel(),
                      _device.stream(), ctypes.byref(self._h))
-        self.contact = self._field("contact", t.int32)
-        self.advanced = self._field("advanced", t.int32)
+        self.active = self._field("active", t.int32)
+        self.stepped = self._field("stepped", t.int32)
+        self.pair_count = self._field("pair_count", t.int32)
+        self.pair_base = self._field("pair_base", t.int32)
         self.steps_done = self._field("steps_done", t.int32)
         self.time = self._field("time", t.float64)
+        self.pairs = self._field("pairs", t.int32).view(-1, 2)
+        self.pair_values = self._field("pair_values", t.float64).view(-1, 8)
+        self.pair_capacity = int(self.pairs.shape[0])
+        # pinned staging of the check results, copied before the advance
+        k = min(self.pair_capacity, 1 << 16)
+        self._h_counts = t.empty(self.n_ens, dtype=t.int32, pin_memory=True)
+        self._h_pairs = t.empty((k, 2), dtype=t.int32, pin_memory=True)
+        self._h_vals = t.empty((k, 8), dtype=t.float64, pin_memory=True)
+        self._staged = t.cuda.Event()
         self.captured = False
         if capture:
             _native.call("dem_batch_capture", self._h)
             self.captured = True
         self._detectors: dict[int, SpatialHashCollisionDetector] = {}
         self.merges = 0
+        self.merge_rounds = 0
         self.global_steps = 0
+        self.phase_ms: dict | None = None
 
     def _field(self, name, dtype):
         p = ctypes.c_void_p()
@@ -132,53 +154,139 @@ class BatchedEnsembles:
 
     # ------------------------------------------------------------------ stepping
 
-    def run(self, global_steps: int, merge: bool = True) -> int:
-        """``global_steps`` global steps, merging after each one; returns the
-        number of systems merged."""
+    def check(self, new_step: bool = True):
+        _native.call("dem_batch_check", self._h, 1 if new_step else 0, _device.stream())
+
+    def advance(self):
+        _native.call("dem_batch_advance", self._h, _device.stream())
+
+    def _stage(self):
+        """Queue the copies of the last check's results to pinned memory."""
+        k = self._h_pairs.shape[0]
+        self._h_counts.copy_(self.pair_count, non_blocking=True)
+        self._h_pairs.copy_(self.pairs[:k], non_blocking=True)
+        self._h_vals.copy_(self.pair_values[:k], non_blocking=True)
+        self._staged.record()
+
+    def _tick(self, name, t0):
+        if self.phase_ms is not None:
+            _device.torch().cuda.synchronize()
+            now = time.perf_counter()
+            self.phase_ms[name] = self.phase_ms.get(name, 0.0) + (now - t0) * 1e3
+            return now
+        return t0
+
+    def step(self, merge: bool = True) -> int:
+        """One global step; returns the number of system merges in it.
+
+        check -> (advance the systems without contacts | host merge plans of
+        the others) -> merge, check again, ... -> advance the rest.  With
+        ``self.phase_ms`` set to a dict, phases are synchronised and their
+        wall times (ms) accumulated under "check", "advance", "merge"."""
+        t0 = time.perf_counter()
+        self.check(new_step=True)
+        if merge:
+            self._stage()
+        t0 = self._tick("check", t0)
+        self.advance()
+        t0 = self._tick("advance", t0)
         merged = 0
-        st = _device.stream()
-        for _ in range(int(global_steps)):
-            _native.call("dem_batch_run", self._h, 1, st)
-            self.global_steps += 1
-            if merge:
-                hits = np.nonzero(self.contact.cpu().numpy())[0]
-                for e in hits:
-                    self._merge(int(e))
-                    merged += 1
+        while merge:
+            self._staged.synchronize()
+            k = self._merge_staged()
+            t0 = self._tick("merge", t0)
+            if k == 0:
+                break
+            merged += k
+            self.check(new_step=False)
+            self._stage()
+            t0 = self._tick("check", t0)
+        if merged:
+            self.advance()
+            self._tick("advance", t0)
+        self.global_steps += 1
         return merged
+
+    def run(self, global_steps: int, merge: bool = True) -> int:
+        return sum(self.step(merge) for _ in range(int(global_steps)))
 
     def run_until_done(self, max_global_steps: int = 1_000_000) -> int:
         k = 0
         while not self.done() and k < max_global_steps:
-            self.run(1)
+            self.step()
             k += 1
         return k
 
     def done(self) -> bool:
         return bool(np.all(self.steps_done.cpu().numpy() >= self.steps))
 
-    def _merge(self, e: int):
-        """Merge the contacts of system e on the host (float64, reference order)."""
-        off, n = int(self.offset[e]), int(self.n[e])
+    def _system_pairs(self, e, a, c, pairs_h, vals_h):
+        """(pairs, values) of system e from the packed list, or — when the
+        packed buffer overflowed — from the single-system pass."""
+        if a + c <= self.pair_capacity:
+            return pairs_h[a: a + c], vals_h[a: a + c]
         det = self._detectors.get(e)
         if det is None:
             det = SpatialHashCollisionDetector(num_particles=int(self.capacity[e]),
                                                L=float(self.L[e]), boundary_mode="periodic")
             self._detectors[e] = det
-        pos_e = self.pos[off: off + n]
-        r_e = self.radii[off: off + n]
-        k, pairs = det.detect_device(pos_e, r_e, float(self.L[e]), True)
-        if k == 0:
-            raise RuntimeError(f"system {e}: batched contact flag without a contact pair")
-        new_p, new_r, _ = merge_collided(pos_e.cpu().numpy(), r_e.cpu().numpy(),
-                                         pairs.cpu().numpy())
-        m = int(new_r.shape[0])
         t = _device.torch()
-        self.pos[off: off + m] = t.from_numpy(np.ascontiguousarray(new_p)).to(self.pos.device)
-        self.radii[off: off + m] = t.from_numpy(np.ascontiguousarray(new_r)).to(self.pos.device)
-        self.n[e] = m
-        _native.call("dem_batch_update", self._h, e, m, _device.stream())
-        self.merges += 1
+        off, n = int(self.offset[e]), int(self.n[e])
+        k, pd = det.detect_device(self.pos[off: off + n], self.radii[off: off + n],
+                                  float(self.L[e]), True)
+        pairs = pd.cpu().numpy()
+        g = t.from_numpy(off + pairs.reshape(-1).astype(np.int64)).to(self.pos.device)
+        v = t.cat([self.pos.index_select(0, g), self.radii.index_select(0, g)[:, None]], 1)
+        return pairs, v.cpu().numpy().reshape(-1, 8)
+
+    def _merge_staged(self) -> int:
+        """Merge every system the last check found in contact; returns how
+        many systems were merged.
+
+        The merge decisions and merged droplets come from ``merge_plan`` (the
+        reference's float64 arithmetic) on the droplets named in the pairs,
+        whose values the check packed next to them; ``dem_batch_merge`` then
+        compacts each system on the device (survivors in order, merged
+        droplets appended) — exactly ``merge_collided``'s result."""
+        counts = self._h_counts.numpy().astype(np.int64)
+        hit = np.nonzero(counts)[0]
+        if len(hit) == 0:
+            return 0
+        base = np.concatenate([[0], np.cumsum(counts)])
+        k = self._h_pairs.shape[0]
+        total = int(min(base[-1], self.pair_capacity))
+        pairs_h = self._h_pairs.numpy()
+        vals_h = self._h_vals.numpy()
+        if total > k:  # rare: more pairs than the pinned staging holds
+            pairs_h = self.pairs[:total].cpu().numpy()
+            vals_h = self.pair_values[:total].cpu().numpy()
+        roff, moff, removed, merged = [0], [0], [], []
+        for e in hit:
+            pe, ve = self._system_pairs(int(e), int(base[e]), int(counts[e]), pairs_h, vals_h)
+            idx = np.concatenate([pe[:, 0], pe[:, 1]])
+            val = np.concatenate([ve[:, 0:4], ve[:, 4:8]])
+            rows = dict(zip(idx.tolist(), range(len(idx))))
+            rem, nr, npos, _ = merge_plan(pe, lambda i: val[rows[int(i)], 3],
+                                          lambda i: val[rows[int(i)], :3])
+            removed.extend(sorted(int(i) for i in rem))
+            roff.append(len(removed))
+            for r_, p_ in zip(nr, npos):
+                merged.append((p_[0], p_[1], p_[2], r_))
+            moff.append(len(merged))
+        es = np.ascontiguousarray(hit, dtype=np.int32)
+        ro = np.asarray(roff, dtype=np.int32)
+        mo = np.asarray(moff, dtype=np.int32)
+        rm = np.asarray(removed, dtype=np.int32)
+        mg = np.ascontiguousarray(np.asarray(merged, dtype=np.float64).reshape(-1))
+        vp = ctypes.c_void_p
+        _native.call("dem_batch_merge", self._h, len(es), es.ctypes.data_as(vp),
+                     ro.ctypes.data_as(vp), rm.ctypes.data_as(vp), mo.ctypes.data_as(vp),
+                     mg.ctypes.data_as(vp), _device.stream())
+        self.n[hit] -= ro[1:] - ro[:-1]
+        self.n[hit] += mo[1:] - mo[:-1]
+        self.merges += len(hit)
+        self.merge_rounds += 1
+        return len(hit)
 
     # ------------------------------------------------------------------ state
 
@@ -197,6 +305,12 @@ class BatchedEnsembles:
     def velocities(self, e: int) -> np.ndarray:
         off = int(self.offset[e])
         return self.V[off: off + int(self.n[e])].cpu().numpy()
+
+    def contact_pairs(self, e: int) -> np.ndarray:
+        """Pairs of system e found by the last check (lexicographic order)."""
+        b = int(self.pair_base[e].item())
+        c = int(self.pair_count[e].item())
+        return self.pairs[b: min(b + c, self.pair_capacity)].cpu().numpy()
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -221,9 +335,10 @@ class SequentialEnsembles:
     ``BatchedEnsembles`` with the same ``jlen``.
     """
 
-    def __init__(self, systems, dts, steps, jlen: int | None = None):
+    def __init__(self, systems, dts, steps, jlen: int | None = None, contacts: bool = True):
         self.n_ens = len(systems)
         self.jlen = jlen
+        self.contacts = contacts
         self.dt = np.asarray(dts, dtype=np.float64).reshape(self.n_ens)
         self.steps = np.asarray(steps, dtype=np.int64).reshape(self.n_ens)
         self.steps_done = np.zeros(self.n_ens, dtype=np.int64)
@@ -249,7 +364,7 @@ class SequentialEnsembles:
         if self.steps_done[e] >= self.steps[e]:
             return
         L = self.L[e]
-        while True:
+        while self.contacts:
             k, pairs = self.det[e].detect_device(self.pos[e], self.radii[e], L, True)
             if k == 0:
                 break
@@ -291,14 +406,16 @@ class SequentialEnsembles:
 def autotune(systems, dts, blocks=BLOCKS, jlens=JLENS, warmup: int = 1, reps: int = 3,
              verbose: bool = False):
     """Fastest (block, jlen) for these systems: each candidate runs ``warmup``
-    + ``reps`` captured global steps on a throw-away copy of the state (no
-    merges).  Returns (Tuning, {(block, jlen): ms per global step})."""
+    + ``reps`` captured force/Oseen/Euler steps of every system (contact pass
+    off, so every system advances) on a throw-away copy of the state.
+    Returns (Tuning with contacts on, {(block, jlen): ms per step})."""
     t = _device.torch()
     times = {}
     big = 1 << 30
     for b in blocks:
         for j in jlens:
-            eng = BatchedEnsembles(systems, dts, [big] * len(systems), Tuning(b, j), capture=True)
+            eng = BatchedEnsembles(systems, dts, [big] * len(systems),
+                                   Tuning(b, j, contacts=False), capture=True)
             st = _device.stream()
             _native.call("dem_batch_run", eng._h, warmup, st)
             t.cuda.synchronize()

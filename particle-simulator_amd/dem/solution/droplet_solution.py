@@ -26,29 +26,27 @@ from ..particle_state import DropletState
 from .solution_base import Solution
 
 
-def merge_collided(positions, radii, collided):
-    """Volume-conserving greedy merge (reference droplet_solution.py:208-271).
+def merge_plan(collided, radius_of, position_of):
+    """The greedy merge decisions of the reference (droplet_solution.py:
+    208-262), reading droplet values through ``radius_of(i)`` (float64
+    scalar) and ``position_of(i)`` ((3,) float64).
 
-    Returns (new_positions, new_radii, collided sorted by first index).  Pairs
-    are visited in ``np.argsort(collided[:, 0])`` order; a pair is skipped when
-    either droplet already merged; r = (r1^3 + r2^3) ** (1/3); the centre is
-    the r^3-weighted mean of the raw coordinates; survivors keep their order
-    (``np.delete``) and merged droplets are appended in processing order.
+    Pairs are visited in ``np.argsort(collided[:, 0])`` order; a pair is
+    skipped when either droplet already merged; r = (r1^3 + r2^3) ** (1/3)
+    (float power); the centre is the r^3-weighted mean of the raw coordinates
+    (no minimum image).  Returns (indices to remove, descending; merged radii;
+    merged positions; collided sorted by first index).
     """
-    new_radii = np.copy(radii)
-    new_positions = np.copy(positions)
+    sorted_indices = np.argsort(collided[:, 0])
+    collided = collided[sorted_indices]
     new_droplets_radii = []
     new_droplets_positions = []
     collided_indices = set()
-
-    sorted_indices = np.argsort(collided[:, 0])
-    collided = collided[sorted_indices]
-
     for idx1, idx2 in collided:
         if idx1 in collided_indices or idx2 in collided_indices:
             continue
-        radius1, radius2 = new_radii[idx1], new_radii[idx2]
-        position1, position2 = new_positions[idx1], new_positions[idx2]
+        radius1, radius2 = radius_of(idx1), radius_of(idx2)
+        position1, position2 = position_of(idx1), position_of(idx2)
         radius1_cube = radius1**3
         radius2_cube = radius2**3
         new_radius = (radius1_cube + radius2_cube) ** (1 / 3)
@@ -59,8 +57,20 @@ def merge_collided(positions, radii, collided):
         new_droplets_positions.append(new_position)
         collided_indices.add(idx1)
         collided_indices.add(idx2)
+    return sorted(collided_indices, reverse=True), new_droplets_radii, new_droplets_positions, collided
 
-    indices_to_remove = sorted(collided_indices, reverse=True)
+
+def merge_collided(positions, radii, collided):
+    """Volume-conserving greedy merge (reference droplet_solution.py:208-271).
+
+    Returns (new_positions, new_radii, collided sorted by first index):
+    survivors keep their order (``np.delete``) and merged droplets are
+    appended in processing order (see ``merge_plan``).
+    """
+    new_radii = np.copy(radii)
+    new_positions = np.copy(positions)
+    indices_to_remove, new_droplets_radii, new_droplets_positions, collided = merge_plan(
+        collided, lambda i: new_radii[i], lambda i: new_positions[i])
     new_radii = np.delete(new_radii, indices_to_remove)
     new_positions = np.delete(new_positions, indices_to_remove, axis=0)
     new_radii = np.append(new_radii, new_droplets_radii)

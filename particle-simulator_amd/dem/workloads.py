@@ -122,6 +122,22 @@ def ensemble_specs(count: int = 256, base_seed: int = 0):
     return [(int(ns[e]), float(Es[e]), float(phis[e])) for e in range(count)]
 
 
-def ensembles(count: int = 256, base_seed: int = 0) -> list[System]:
-    return [make_system(n, seed=e, phi=phi, E=E)
-            for e, (n, E, phi) in enumerate(ensemble_specs(count, base_seed))]
+def _make_ensemble(args) -> System:
+    e, (n, E, phi) = args
+    return make_system(n, seed=e, phi=phi, E=E)
+
+
+def ensembles(count: int = 256, base_seed: int = 0, workers: int | None = None) -> list[System]:
+    """The ensemble batch; system e is seeded with e, so the result does not
+    depend on ``workers`` (forked processes used for the overlap-free
+    placement; call before the GPU is initialised)."""
+    import multiprocessing as mp
+    import os
+
+    jobs = list(enumerate(ensemble_specs(count, base_seed)))
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers <= 1 or count < 4:
+        return [_make_ensemble(j) for j in jobs]
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_make_ensemble, jobs, chunksize=1)

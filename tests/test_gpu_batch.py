@@ -1,15 +1,21 @@
 """Batched ensembles (``dem_batch_*``) on the GPU.
 
-* Bit-for-bit: N uneven periodic systems stepped together (contacts, merges,
-  systems finishing at different step counts) end in exactly the states the
-  single-system path reaches system after system with the same source split.
+* Bit-for-bit: uneven periodic systems stepped together (contacts, merge
+  rounds, systems finishing at different step counts) end in exactly the
+  states the single-system path reaches system after system with the same
+  source split; also without the contact pass.
+* Pair lists of the batched contact pass equal the CPU oracle's restatement
+  of the reference hash (bucket aliases and periodic misses included), and a
+  system whose pairs overflow its slot still merges exactly.
 * Against the FP64 oracle: the first batched step's forces, velocities and
   displacements, under the direct kernels' accuracy contract.
-* Graph replay equals eager launches; finished systems stay put; the update
-  call rejects sizes beyond the capacity.
+* Graph replay equals eager launches; finished systems stay put; resizing
+  beyond the capacity is rejected.
 """
 
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 import pytest
@@ -68,7 +74,7 @@ def test_batched_equals_sequential_bit_for_bit(gpu, setup, block, jlen):
     assert b.done()
     assert np.array_equal(b.steps_done.cpu().numpy(), np.array(STEPS))
     assert b.merges == s.merges and b.merges > 0, "the run should contain merges"
-    assert g >= max(STEPS)
+    assert g == max(STEPS), "every active system advances once per global step"
     for e in range(len(systems)):
         assert np.array_equal(b.radii_of(e), s.radii_of(e)), f"system {e} radii"
         assert np.array_equal(b.positions(e), s.positions(e)), f"system {e} positions"
@@ -86,7 +92,7 @@ def test_first_step_against_fp64_oracle(gpu, setup):
     sdt = [dts[0], dts[2], dts[5]]
     b = BatchedEnsembles(sub, sdt, [1, 1, 1], Tuning(256, 512), capture=False)
     b.run(1, merge=False)
-    assert np.array_equal(b.advanced.cpu().numpy(), np.array([1, 1, 1]))
+    assert np.array_equal(b.steps_done.cpu().numpy(), np.array([1, 1, 1]))
     for e, sy in enumerate(sub):
         F_ref = oracle.direct_forces(sy.positions, sy.radii, sy.m_const, sy.L, True)
         V_ref = oracle.direct_convection(sy.positions, F_ref, sy.eta_const, sy.L, True)
@@ -127,8 +133,82 @@ def test_finished_and_empty_step_budgets(gpu, setup):
     assert np.array_equal(b.positions(0), p0)
     assert list(b.steps_done.cpu().numpy()) == [0, 3]
     np.testing.assert_allclose(b.time.cpu().numpy()[1], 3 * dts[1], rtol=1e-15)
+    es = np.array([1], dtype=np.int32)
+    ns = np.array([int(b.capacity[1]) + 1], dtype=np.int32)
     with pytest.raises(_native.NativeError):
-        _native.call("dem_batch_update", b._h, 1, int(b.capacity[1]) + 1, None)
+        _native.call("dem_batch_resize", b._h, 1, es.ctypes.data_as(ctypes.c_void_p),
+                     ns.ctypes.data_as(ctypes.c_void_p), None)
+    b.close()
+
+
+def _overlapping(n, seed, phi, L_scale=1.0):
+    """Uniform positions without overlap rejection: many contacts."""
+    from dem.workloads import System, box_edge, lognormal_radii
+
+    rng = np.random.default_rng(seed)
+    r = lognormal_radii(rng, n)
+    L = box_edge(r, phi) * L_scale
+    return System(positions=rng.uniform(0, L, size=(n, 3)), radii=r, L=L, phi=phi, seed=seed)
+
+
+def test_pair_lists_equal_the_oracle_hash(gpu):
+    systems = [_overlapping(n, 50 + k, phi) for k, (n, phi) in
+               enumerate([(300, 0.1), (1200, 0.05), (64, 0.2), (2500, 0.03), (700, 0.08)])]
+    b = BatchedEnsembles(systems, [1e-3] * len(systems), [1] * len(systems), Tuning(128, 512))
+    b.check()
+    cnt = b.pair_count.cpu().numpy()
+    assert cnt.sum() <= b.pair_capacity
+    base = b.pair_base.cpu().numpy()
+    vals = b.pair_values.cpu().numpy()
+    for e, sy in enumerate(systems):
+        T = 1 << int(np.ceil(np.log2(2 * sy.n)))
+        hit, ref = oracle.hash_pairs(sy.positions, sy.radii, sy.L, True, T)
+        assert cnt[e] == len(ref) and hit == (len(ref) > 0)
+        got = b.contact_pairs(e)
+        assert np.array_equal(got, ref.astype(np.int32)), f"system {e}"
+        # the packed values are the droplets' own coordinates and radii
+        v = vals[base[e]: base[e] + cnt[e]]
+        np.testing.assert_array_equal(v[:, 0:3], sy.positions[got[:, 0]])
+        np.testing.assert_array_equal(v[:, 3], sy.radii[got[:, 0]])
+        np.testing.assert_array_equal(v[:, 4:7], sy.positions[got[:, 1]])
+        np.testing.assert_array_equal(v[:, 7], sy.radii[got[:, 1]])
+    b.close()
+
+
+def test_overflowing_pair_buffer_merges_exactly(gpu):
+    # 80 droplets of 8 um in a 40 um box: ~850 pairs against a packed buffer
+    # of 480 (sum of max(64, capacity))
+    from dem.workloads import System
+
+    rng = np.random.default_rng(3)
+    r = np.full(80, 8e-6)
+    L = 4e-5
+    crowd = System(positions=rng.uniform(0, L, size=(80, 3)), radii=r, L=L)
+    calm = _systems([(400, 0.02, 3e5)])[0]
+    systems = [crowd, calm]
+    dts = [1e-4, _dts([calm])[0]]
+    b = BatchedEnsembles(systems, dts, [3, 3], Tuning(64, 256))
+    b.check()
+    assert b.pair_count.cpu().numpy().sum() > b.pair_capacity
+    b.run_until_done(20)
+    s = SequentialEnsembles(systems, dts, [3, 3], jlen=256)
+    s.run_until_done()
+    for e in range(2):
+        assert np.array_equal(b.radii_of(e), s.radii_of(e))
+        assert np.array_equal(b.positions(e), s.positions(e))
+    b.close()
+
+
+def test_without_contact_pass_equals_sequential_without_detector(gpu, setup):
+    systems, dts = setup
+    steps = [4, 2, 3]
+    b = BatchedEnsembles(systems[:3], dts[:3], steps, Tuning(256, 1024, contacts=False))
+    b.run_until_done(10)
+    s = SequentialEnsembles(systems[:3], dts[:3], steps, jlen=1024, contacts=False)
+    s.run_until_done()
+    assert b.merges == 0 and s.merges == 0
+    for e in range(3):
+        assert np.array_equal(b.positions(e), s.positions(e))
     b.close()
 
 

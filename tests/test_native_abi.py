@@ -87,7 +87,7 @@ def _ensembles(specs):
 
 def test_batch_workspace_and_argument_checks():
     lib = _native.load()
-    tune = _native.DemBatchTuning(256, 1024, 0, 0)
+    tune = _native.DemBatchTuning(256, 1024, 0, 0, 1, 0)
     ok = _ensembles([(0, 500, 500), (500, 800, 1000)])
     small = lib.dem_batch_workspace_bytes(ok, 2, ctypes.byref(tune))
     big = lib.dem_batch_workspace_bytes(_ensembles([(0, 500, 500), (500, 800, 20000)]), 2,
@@ -104,7 +104,7 @@ def test_batch_workspace_and_argument_checks():
     bad = _ensembles([(0, 600, 500)])
     assert lib.dem_batch_workspace_bytes(bad, 1, ctypes.byref(tune)) == 0
     # unsupported block size
-    t2 = _native.DemBatchTuning(96, 1024, 0, 0)
+    t2 = _native.DemBatchTuning(96, 1024, 0, 0, 1, 0)
     rc = lib.dem_batch_create(ok, 2, ctypes.byref(t2), ctypes.c_void_p(8), ctypes.c_void_p(8),
                               ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 1 << 40,
                               None, ctypes.byref(out))
