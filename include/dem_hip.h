@@ -251,8 +251,8 @@ int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset
  *   dem_batch_advance            systems without contacts advance (this can
  *                                run while the host plans the merges);
  *   while some system has pairs: dem_batch_merge (the caller's merge plan),
- *                                dem_batch_check(new_step=0) (systems not yet
- *                                advanced);
+ *                                dem_batch_check(new_step=0) (only the
+ *                                systems the previous check found in contact);
  *   dem_batch_advance            the remaining systems advance.
  * Each system advances at most once per global step ("stepped" flag).  Launch
  * sizes are fixed at the capacities, so the three captured HIP graphs stay
@@ -300,8 +300,9 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
                      double* pos, double* radii, double* forces, double* velocities,
                      void* ws, size_t ws_bytes, dem_stream_t stream, dem_batch_t* out);
 
-/* Contact pass of every active system; new_step = 1 starts a global step
- * (clears "stepped"), 0 re-checks systems not yet advanced.  Results: fields
+/* Contact pass; new_step = 1 starts a global step (clears "stepped") and
+ * checks every system with steps left, 0 re-checks only the systems the
+ * previous check found in contact (merged by the caller since).  Results: fields
  * "pair_count", "pair_base" (exclusive scan of pair_count) and the packed
  * "pairs" / "pair_values" (system after system, each lexicographically
  * sorted; values are (x, y, z, r) of i then of j).  A total above the

@@ -25,11 +25,12 @@
 //     search of every target (row-parallel), and for systems with contacts
 //     the lexicographically ordered pair list, packed system after system
 //     with both droplets' (x, y, z, r).  The first check of a global step
-//     clears the per-system "stepped" flags; later checks skip systems that
-//     already advanced.
-//   * "advance" graph: go = active, no contact, not yet stepped (fixed at the
-//     start of the launch); force items, force reduction, Oseen items, Oseen
-//     reduction + Euler step + wrap + re-quantisation for go systems.
+//     clears the per-system "stepped" flags and covers every system with
+//     steps left; a re-check covers only the systems just merged.
+//   * "advance" graph: go = steps left, no contact in the last check, not yet
+//     stepped (fixed at the start of the launch); force items, force
+//     reduction, Oseen items, Oseen reduction + Euler step + wrap +
+//     re-quantisation for go systems.
 //   * dem_batch_merge applies host-computed merges on the device: survivors
 //     compacted in order, merged droplets appended.
 //   A global step is therefore: check; advance the clean systems while the
@@ -344,8 +345,15 @@ __global__ __launch_bounds__(256) void k_b_compact(const BEns* __restrict__ ens,
 
 // ---------------------------------------------------------------- check graph
 
-// new_step: the first check of a global step (clears "stepped").  Active:
-// steps left, droplets present, not advanced yet in this global step.
+__device__ __forceinline__ bool eligible(const BEns& E, const int* steps_done,
+                                         const int* stepped, int e) {
+  return steps_done[e] < E.steps && E.n > 0 && !stepped[e];
+}
+
+// Which systems this check hashes ("active").  new_step (the first check of
+// a global step) clears "stepped" and checks every system with steps left; a
+// re-check only covers the systems the previous check found in contact (the
+// caller has merged them since); the others keep pair_count = 0.
 __global__ void k_b_begin(const BEns* __restrict__ ens, const BHead* __restrict__ head,
                           int new_step, const int* __restrict__ steps_done,
                           int* __restrict__ stepped, int* __restrict__ active,
@@ -353,7 +361,8 @@ __global__ void k_b_begin(const BEns* __restrict__ ens, const BHead* __restrict_
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= head->n_ens) return;
   if (new_step) stepped[e] = 0;
-  active[e] = (steps_done[e] < ens[e].steps && ens[e].n > 0 && !stepped[e]) ? 1 : 0;
+  const bool want = new_step ? true : pair_count[e] > 0;
+  active[e] = (want && eligible(ens[e], steps_done, stepped, e)) ? 1 : 0;
   pair_count[e] = 0;
 }
 
@@ -622,14 +631,15 @@ __global__ __launch_bounds__(kRow) void k_b_emit(const BEns* __restrict__ ens,
 
 // ---------------------------------------------------------------- advance graph
 
-// go = active, no contact found, not advanced yet in this global step; the
-// rest of the advance launch reads only go, so marking "stepped" here is safe.
-__global__ void k_b_go(const BHead* __restrict__ head, const int* __restrict__ active,
-                       const int* __restrict__ pair_count, int* __restrict__ stepped,
-                       int* __restrict__ go) {
+// go = steps left, not advanced yet in this global step, and no contact in
+// its last check; the rest of the advance launch reads only go, so marking
+// "stepped" here is safe.
+__global__ void k_b_go(const BEns* __restrict__ ens, const BHead* __restrict__ head,
+                       const int* __restrict__ steps_done, const int* __restrict__ pair_count,
+                       int* __restrict__ stepped, int* __restrict__ go) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= head->n_ens) return;
-  const int g = (active[e] && pair_count[e] == 0 && !stepped[e]) ? 1 : 0;
+  const int g = (eligible(ens[e], steps_done, stepped, e) && pair_count[e] == 0) ? 1 : 0;
   go[e] = g;
   if (g) stepped[e] = 1;
 }
@@ -948,8 +958,8 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
   const unsigned gi = (unsigned)std::max<long long>(b->plan.items_cap, 1);
   const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
   hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
-                     (const BHead*)l.head, (const int*)l.active, (const int*)l.pair_count,
-                     l.stepped, l.go);
+                     (const BEns*)l.ens, (const BHead*)l.head, (const int*)l.steps_done,
+                     (const int*)l.pair_count, l.stepped, l.go);
   DEM_CHECK_LAUNCH("k_b_go");
   auto force = [&] {
     hipLaunchKernelGGL(k_b_force<BW>, dim3(gi), dim3(BW), 0, st, (const BEns*)l.ens,

@@ -35,7 +35,7 @@ import numpy as np
 from . import _device, _native
 from .collision_detector import SpatialHashCollisionDetector
 from .force_calculator import DirectDropletForceCalculator
-from .solution.droplet_solution import merge_collided, merge_plan
+from .solution.droplet_solution import merge_collided, merge_plan_rows
 
 BLOCKS = (64, 128, 256)
 JLENS = (1024, 2048, 4096, 8192)
@@ -243,9 +243,9 @@ class BatchedEnsembles:
         """Merge every system the last check found in contact; returns how
         many systems were merged.
 
-        The merge decisions and merged droplets come from ``merge_plan`` (the
-        reference's float64 arithmetic) on the droplets named in the pairs,
-        whose values the check packed next to them; ``dem_batch_merge`` then
+        The merge decisions and merged droplets come from ``merge_plan_rows``
+        (the reference's float64 arithmetic) on the droplets named in the
+        pairs, whose values the check packed next to them; ``dem_batch_merge`` then
         compacts each system on the device (survivors in order, merged
         droplets appended) — exactly ``merge_collided``'s result."""
         counts = self._h_counts.numpy().astype(np.int64)
@@ -263,15 +263,10 @@ class BatchedEnsembles:
         roff, moff, removed, merged = [0], [0], [], []
         for e in hit:
             pe, ve = self._system_pairs(int(e), int(base[e]), int(counts[e]), pairs_h, vals_h)
-            idx = np.concatenate([pe[:, 0], pe[:, 1]])
-            val = np.concatenate([ve[:, 0:4], ve[:, 4:8]])
-            rows = dict(zip(idx.tolist(), range(len(idx))))
-            rem, nr, npos, _ = merge_plan(pe, lambda i: val[rows[int(i)], 3],
-                                          lambda i: val[rows[int(i)], :3])
-            removed.extend(sorted(int(i) for i in rem))
+            rem, mg = merge_plan_rows(pe, ve)
+            removed.extend(rem)
             roff.append(len(removed))
-            for r_, p_ in zip(nr, npos):
-                merged.append((p_[0], p_[1], p_[2], r_))
+            merged.extend(mg)
             moff.append(len(merged))
         es = np.ascontiguousarray(hit, dtype=np.int32)
         ro = np.asarray(roff, dtype=np.int32)

@@ -60,6 +60,43 @@ def merge_plan(collided, radius_of, position_of):
     return sorted(collided_indices, reverse=True), new_droplets_radii, new_droplets_positions, collided
 
 
+def merge_plan_rows(pairs, values):
+    """``merge_plan`` for one system whose pair list carries the droplets'
+    values: pairs (k, 2) local indices, values (k, 8) = (x, y, z, r) of the
+    first droplet then of the second.
+
+    Same decisions and the same IEEE operations in the same order as
+    ``merge_plan`` — argsort by first index, greedy skip, r^3 = r ** 3,
+    r = (r1^3 + r2^3) ** (1/3), centre (x1 r1^3 + x2 r2^3) / (r1^3 + r2^3) per
+    axis — on Python floats (C ``pow`` and unfused multiply/add/divide, as
+    numpy's float64 scalars and element-wise array operations do), which is
+    what makes it cheap for thousands of systems.  Returns (indices to remove,
+    ascending; merged (x, y, z, r) tuples in processing order).
+    """
+    order = np.argsort(pairs[:, 0])
+    pl = pairs[order].tolist()
+    vl = values[order].tolist()
+    value = {}
+    for (i, j), v in zip(pl, vl):
+        value[i] = v[0:4]
+        value[j] = v[4:8]
+    done = set()
+    merged = []
+    for i, j in pl:
+        if i in done or j in done:
+            continue
+        x1, y1, z1, r1 = value[i]
+        x2, y2, z2, r2 = value[j]
+        c1 = r1**3
+        c2 = r2**3
+        s = c1 + c2
+        merged.append(((x1 * c1 + x2 * c2) / s, (y1 * c1 + y2 * c2) / s,
+                       (z1 * c1 + z2 * c2) / s, s ** (1 / 3)))
+        done.add(i)
+        done.add(j)
+    return sorted(done), merged
+
+
 def merge_collided(positions, radii, collided):
     """Volume-conserving greedy merge (reference droplet_solution.py:208-271).
 

@@ -176,3 +176,31 @@ def test_generator_produces_non_overlapping_droplets():
     d = np.linalg.norm(st.positions[:, None] - st.positions[None], axis=2)
     np.fill_diagonal(d, np.inf)
     assert np.all(d >= st.radii[:, None] + st.radii[None] + 1e-6 - 1e-18)
+
+
+def test_merge_plan_rows_matches_the_reference_merge_bit_for_bit():
+    """The batch's float merge plan equals merge_collided (the reference's
+    numpy code) on random systems with chains of shared droplets."""
+    from dem.solution.droplet_solution import merge_collided, merge_plan_rows
+
+    rng = np.random.default_rng(7)
+    for trial in range(200):
+        n = int(rng.integers(2, 60))
+        pos = rng.uniform(0, 1e-3, size=(n, 3))
+        rad = np.exp(rng.uniform(np.log(2e-6), np.log(2e-5), size=n))
+        k = int(rng.integers(1, 3 * n))
+        i = rng.integers(0, n, size=k)
+        j = rng.integers(0, n, size=k)
+        pairs = np.stack([np.minimum(i, j), np.maximum(i, j)], 1)
+        pairs = np.unique(pairs[pairs[:, 0] < pairs[:, 1]], axis=0).astype(np.int32)
+        if len(pairs) == 0:
+            continue
+        ref_p, ref_r, _ = merge_collided(pos, rad, pairs)
+        vals = np.concatenate([pos[pairs[:, 0]], rad[pairs[:, 0], None], pos[pairs[:, 1]],
+                               rad[pairs[:, 1], None]], 1)
+        removed, merged = merge_plan_rows(pairs, vals)
+        keep = np.delete(np.arange(n), removed)
+        m = np.asarray(merged, dtype=np.float64).reshape(-1, 4)
+        got_p = np.vstack([pos[keep], m[:, :3]])
+        got_r = np.concatenate([rad[keep], m[:, 3]])
+        assert np.array_equal(got_p, ref_p) and np.array_equal(got_r, ref_r), trial
