@@ -345,6 +345,37 @@ int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes);
 int dem_batch_destroy(dem_batch_t b);
 
 /* ------------------------------------------------------------------------ */
+/* Differentiation: adjoint of one direct Euler step, near-contact loss      */
+/* ------------------------------------------------------------------------ */
+
+/* The reference has no gradients; these support differentiating trajectories
+ * of the direct path (forces direct_droplet_force_calculator.py:183-253,
+ * Oseen :354-411, Euler euler_droplet_solver.py:113-131) through a
+ * hand-written adjoint.  Periodic boxes only. */
+size_t dem_adjoint_workspace_bytes(int n);
+
+/* For positions x (the state of one step), the forward forces F at x and an
+ * adjoint vector a (all (n,3) float64), with v = F/(sf r) + V(x, F):
+ *   grad_x = (dv/dx)^T a                                   ((n,3) float64)
+ *   sums[0] = sum_j c_j . F_j   (dJ/dm_const  = sums[0] / m_const)
+ *   sums[1] = sum_j F_j . u_j   (dJ/deta_const = sums[1] / eta_const)
+ *   sums[2] = sum_i b_i . F_i   (dJ/dstokes   = -sums[2] / stokes_factor)
+ * for J = sum_i a_i . v_i, where b_i = a_i/(sf r_i), u = eta sum G a and
+ * c = b + u.  sums is a device array of 3 doubles.  The contact mask is taken
+ * as locally constant.  FP32 pair arithmetic, float64 accumulation. */
+int dem_direct_step_vjp(const double* pos, const double* radii, const double* forces,
+                        const double* adj, int n, const dem_phys* phys, double stokes_factor,
+                        double* grad_x, double* sums, void* ws, size_t ws_bytes,
+                        dem_stream_t stream);
+
+/* loss = sum_{i<j} softplus(unit * (factor (r_i + r_j) - |d_ij|)) (minimum
+ * image) into *loss (device double) and its gradient d loss / d x into grad
+ * ((n,3) float64, per metre).  unit = 1e6 measures lengths in micrometres. */
+int dem_contact_loss(const double* pos, const double* radii, int n, const dem_phys* phys,
+                     double factor, double unit, double* loss, double* grad, void* ws,
+                     size_t ws_bytes, dem_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing (measurement support; no counterpart in the reference)       */
 /* ------------------------------------------------------------------------ */
 

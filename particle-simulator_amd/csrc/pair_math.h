@@ -24,6 +24,14 @@ __device__ __forceinline__ bool separated(float r2w, float rs) {
   return INCLUSIVE ? (r2w >= rs * rs) : (r2w > rs * rs);
 }
 
+// |d|^2 in work units from fixed-unit differences, in the order the dipole
+// kernels evaluate it (so contact decisions agree wherever it is used).
+__device__ __forceinline__ float r2_work(float dx, float dy, float dz) {
+  const float dz2 = dz * dz;
+  const float dxy2 = dx * dx + dy * dy;
+  return (dxy2 + dz2) * kSigma2f;
+}
+
 // Dipole force of one source; excluded pairs (contact, j == i) add nothing.
 template <bool INCLUSIVE>
 __device__ __forceinline__ void dipole_pair(float dx, float dy, float dz, float ri, float rj,

@@ -142,6 +142,9 @@ SIGNATURES: dict[str, tuple] = {
     "dem_batch_run": (_I, [_P, _I, _P]),
     "dem_batch_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_S)]),
     "dem_batch_destroy": (_I, [_P]),
+    "dem_adjoint_workspace_bytes": (_S, [_I]),
+    "dem_direct_step_vjp": (_I, [_P, _P, _P, _P, _I, _P, _D, _P, _P, _P, _S, _P]),
+    "dem_contact_loss": (_I, [_P, _P, _I, _P, _D, _D, _P, _P, _P, _S, _P]),
     "dem_profile_start": (_I, []),
     "dem_profile_stop": (_I, []),
     "dem_profile_read": (_I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
