@@ -112,3 +112,60 @@ def contact_loss(x, radii, L, factor: float = 1.05, unit: float = 1e6):
     sp = torch.nn.functional.softplus(s)
     upper = torch.triu(torch.ones(n, n, dtype=torch.bool, device=x.device), 1)
     return torch.where(upper, sp, torch.zeros_like(sp)).sum()
+
+
+def central_difference(x0, radii, E, mu, eps_oil, mu_w, dt, steps, L, direction=None,
+                       eps=0.0, dE=0.0, dmu=0.0):
+    """(loss(+) - loss(-)) / 2 in float64 for the perturbed inputs
+    (x0 +- eps v, E +- dE, mu +- dmu), and whether the two trajectories keep
+    identical contact masks and periodic images of every pair at every step
+    (the loss jumps where either switches, so a difference quotient across a
+    switch is not a derivative)."""
+    xp = x0 + eps * direction if direction is not None else x0.clone()
+    xm = x0 - eps * direction if direction is not None else x0.clone()
+    par = []
+    for sgn in (1.0, -1.0):
+        Ev, muv = E + sgn * dE, mu + sgn * dmu
+        par.append((m_const(eps_oil, Ev), eta_const(muv), stokes_factor(muv, mu_w)))
+    thr = radii[:, None] + radii[None, :]
+    same = True
+    with torch.no_grad():
+        for _ in range(steps):
+            if same:
+                da = xp[None, :, :] - xp[:, None, :]
+                db = xm[None, :, :] - xm[:, None, :]
+                if not torch.equal(da.abs() > L / 2, db.abs() > L / 2):
+                    same = False
+                elif not torch.equal((_mic(da, L) ** 2).sum(-1).sqrt() > thr,
+                                     (_mic(db, L) ** 2).sum(-1).sqrt() > thr):
+                    same = False
+                del da, db
+            xp = torch.remainder(xp + velocity(xp, radii, *par[0], L) * dt, L)
+            xm = torch.remainder(xm + velocity(xm, radii, *par[1], L) * dt, L)
+        q = (contact_loss(xp, radii, L) - contact_loss(xm, radii, L)).item() / 2
+    return q, same
+
+
+def switch_free_derivative(x0, radii, E, mu, eps_oil, mu_w, dt, steps, L, which,
+                           direction=None,
+                           hs=(1e-6, 1e-7, 1e-8, 1e-9, 1e-10, 1e-11, 1e-12)):
+    """Central difference of the loss along `which` ("E", "mu" or "x0" with
+    `direction`), with the largest relative step h in `hs` (positions: h L)
+    whose two trajectories switch no contact mask or periodic image.
+    Returns (derivative, h) or (None, None) if every step straddles a switch."""
+    for h in hs:
+        if which == "E":
+            q, same = central_difference(x0, radii, E, mu, eps_oil, mu_w, dt, steps, L,
+                                         dE=h * E)
+            step = h * E
+        elif which == "mu":
+            q, same = central_difference(x0, radii, E, mu, eps_oil, mu_w, dt, steps, L,
+                                         dmu=h * mu)
+            step = h * mu
+        else:
+            q, same = central_difference(x0, radii, E, mu, eps_oil, mu_w, dt, steps, L,
+                                         direction=direction, eps=h * L)
+            step = h * L
+        if same:
+            return q / step, h
+    return None, None

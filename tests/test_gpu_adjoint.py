@@ -124,23 +124,6 @@ def _fp64_loss(x0, r, E, mu, dt, T, L):
     return r64.contact_loss(xT, r, L)
 
 
-def _switch_pattern(x0, r, E, mu, dt, T, L):
-    """Contact masks and periodic images of every pair at every step of the
-    FP64 trajectory: the discontinuities a difference quotient must not
-    straddle."""
-    m, eta, sf = r64.m_const(EPS, E), r64.eta_const(mu), r64.stokes_factor(mu, MU_W)
-    x = x0.clone()
-    pats = []
-    for _ in range(T):
-        d = x[None, :, :] - x[:, None, :]
-        img = (d.abs() > L / 2)
-        dm = r64._mic(d, L)
-        contact = (dm * dm).sum(-1).sqrt() > r[:, None] + r[None, :]
-        pats.append(torch.cat([img.reshape(-1), contact.reshape(-1)]))
-        x = torch.remainder(x + r64.velocity(x, r, m, eta, sf, L) * dt, L)
-    return torch.stack(pats)
-
-
 def test_trajectory_gradients_match_autograd_and_fd(gpu):
     s = make_system(512, seed=0, phi=0.03)
     dev = torch.device("cuda")
@@ -166,29 +149,18 @@ def test_trajectory_gradients_match_autograd_and_fd(gpu):
     assert _rel(mu.grad.item(), mu64.grad.item()) <= 1e-4
     assert _field_err(x0.grad.cpu().numpy(), x64.grad.cpu().numpy()) <= 1e-4
 
-    # FP64 central differences, h = 1e-6 relative
-    with torch.no_grad():
-        x0d = torch.tensor(s.positions, device=dev)
-        h = 1e-6
-        fE = (_fp64_loss(x0d, r, E0 * (1 + h), mu0, dt, T, s.L)
-              - _fp64_loss(x0d, r, E0 * (1 - h), mu0, dt, T, s.L)).item() / (2 * h * E0)
-        fmu = (_fp64_loss(x0d, r, E0, mu0 * (1 + h), dt, T, s.L)
-               - _fp64_loss(x0d, r, E0, mu0 * (1 - h), dt, T, s.L)).item() / (2 * h * mu0)
-        g = torch.Generator(dev).manual_seed(3)
-        v = torch.randn(s.n, 3, dtype=torch.float64, device=dev, generator=g)
-        # Positions: a step of 1e-6 L moves ~N^2 pair distances, so contact
-        # masks and periodic images flip along the trajectory (jumps of the
-        # loss); take the largest step h L, h in 1e-6 ... 1e-9, whose two
-        # perturbed trajectories keep every mask and image.
-        fx = None
-        for hx in (1e-6, 1e-7, 1e-8, 1e-9):
-            eps = hx * s.L
-            if torch.equal(_switch_pattern(x0d + eps * v, r, E0, mu0, dt, T, s.L),
-                           _switch_pattern(x0d - eps * v, r, E0, mu0, dt, T, s.L)):
-                fx = (_fp64_loss(x0d + eps * v, r, E0, mu0, dt, T, s.L)
-                      - _fp64_loss(x0d - eps * v, r, E0, mu0, dt, T, s.L)).item() / (2 * eps)
-                break
+    # FP64 central differences: h = 1e-6 relative (positions h L), or the
+    # largest smaller h whose two perturbed trajectories switch no contact
+    # mask or periodic image (the loss jumps there)
+    x0d = torch.tensor(s.positions, device=dev)
+    g = torch.Generator(dev).manual_seed(3)
+    v = torch.randn(s.n, 3, dtype=torch.float64, device=dev, generator=g)
+    args = (x0d, r, E0, mu0, EPS, MU_W, dt, T, s.L)
+    fE, hE = r64.switch_free_derivative(*args, "E")
+    fmu, hmu = r64.switch_free_derivative(*args, "mu")
+    fx, hx = r64.switch_free_derivative(*args, "x0", direction=v)
+    assert None not in (fE, fmu, fx), "every step straddles a contact or image switch"
+    assert hE == 1e-6 and hmu == 1e-6
     assert _rel(E.grad.item(), fE) <= 1e-4
     assert _rel(mu.grad.item(), fmu) <= 1e-4
-    assert fx is not None, "every position step straddles a contact or image switch"
     assert _rel(float((x0.grad * v).sum()), fx) <= 1e-4
