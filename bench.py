@@ -36,6 +36,11 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector (packed) FLOP/s
 PEAK_FP32_UNPACKED = 78.6e12
 FLOP_FORCE = 33  # algorithmic flops per ordered pair, force phase
 FLOP_CONV = 31   # Oseen phase
+# HBM bytes per launch at N = 65536 from two rocprofv3 --pmc runs (FETCH_SIZE
+# scaled by 2.0, WRITE_SIZE by 1.0: the factors a plain 1 GiB copy reads in
+# the same program); profiles/direct_force_oseen_n65536.md
+TRAFFIC_N65536 = {"k_force_tile": (15384.9 * 2.0 + 12288.0) * 1024,
+                  "k_conv_tile": (23247.7 * 2.0 + 12288.0) * 1024}
 
 
 def _errors(x, ref, floor=1e-3) -> dict:
@@ -353,6 +358,8 @@ def main() -> int:
     ap.add_argument("--ens-plain-steps", type=int, default=4,
                     help="timed steps of the system-after-system comparison")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (scale for PMC byte counters)")
     ap.add_argument("--legs", default="hash,tree,ensembles,adjoint",
                     help="comma list of extra measurements: hash, tree, ensembles, adjoint")
     a = ap.parse_args()
@@ -394,6 +401,14 @@ def main() -> int:
     t1 = time.perf_counter()
     lib.dem_profile_stop()
     ms_step = (t1 - t0) / a.steps * 1e3
+    if a.copy_calibration:
+        # a copy of known size: 1 GiB read + 1 GiB written per launch
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     npairs = s.n * (s.n - 1)
     value = npairs / (ms_step * 1e-3)
 
@@ -405,9 +420,13 @@ def main() -> int:
                       "tflops": flop * npairs / (avg * 1e-3) / 1e12 if avg > 0 else 0.0}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     achieved = kern[dom]["flops"] / (kern[dom]["avg_ms"] * 1e-3)
-    roof = {"bound": "valu_fp32", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+    roof = {"bound": "mfma", "compute": "FP32 VALU (pair sums are not a contraction: no MFMA "
+                                       "instructions; peak = MI355X FP32 vector, packed)",
+            "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-            "frac_unpacked_peak": achieved / PEAK_FP32_UNPACKED, "traffic": None,
+            "frac_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
+            "traffic": TRAFFIC_N65536.get(dom) if s.n == 65536 else None,
+            "traffic_unit": "bytes per launch (rocprofv3 PMC, recorded)",
             "kernels": kern}
 
     F64, V64 = fc.forces_and_convection_f64_device(pd, rd)
