@@ -429,6 +429,13 @@ def main() -> int:
             "traffic_unit": "bytes per launch (rocprofv3 PMC, recorded)",
             "kernels": kern}
 
+    # the drop-in API: numpy float64 in and out (host<->device copies included)
+    fc.calculate_forces_and_convection(s.positions, s.radii)
+    th0 = time.perf_counter()
+    for _ in range(3):
+        fc.calculate_forces_and_convection(s.positions, s.radii)
+    host_ms = (time.perf_counter() - th0) / 3 * 1e3
+
     F64, V64 = fc.forces_and_convection_f64_device(pd, rd)
     torch.cuda.synchronize()
     Fn, Vn = F.cpu().numpy(), V.cpu().numpy()
@@ -466,6 +473,10 @@ def main() -> int:
         "data": "synthetic (seeded log-normal droplets, periodic cube)",
         "config": {"workload": f"direct DEM force+Oseen, N={s.n}, phi=0.02, periodic, seed 0",
                    "n": s.n, "L_m": s.L, "parallelism": "single GPU"},
+        "drop_in_host_arrays": {"ms_per_step": host_ms, "pair_interactions_per_s":
+                                npairs / (host_ms * 1e-3),
+                                "note": "calculate_forces_and_convection on numpy arrays: "
+                                        "PCIe copies in and out included (not the metric)"},
         "roofline": roof,
         "cpu_baseline": base,
         "parity": parity,
