@@ -36,6 +36,11 @@ PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector (packed) FLOP/s
 PEAK_FP32_UNPACKED = 78.6e12
 FLOP_FORCE = 33  # algorithmic flops per ordered pair, force phase
 FLOP_CONV = 31   # Oseen phase
+# flops the pair-once kernels execute per ordered pair: one evaluation per
+# unordered pair plus the reversed side's products and accumulations
+# (force 33 + 6, Oseen 31 + 18, halved)
+EXEC_FORCE_SYM = 19.5
+EXEC_CONV_SYM = 24.5
 # HBM bytes per launch at N = 65536 from two rocprofv3 --pmc runs (FETCH_SIZE
 # scaled by 2.0, WRITE_SIZE by 1.0: the factors a plain 1 GiB copy reads in
 # the same program); profiles/direct_force_oseen_n65536.md
@@ -413,11 +418,17 @@ def main() -> int:
     value = npairs / (ms_step * 1e-3)
 
     kern = {}
-    for name, flop in (("k_force_tile", FLOP_FORCE), ("k_conv_tile", FLOP_CONV)):
+    for name, flop, ex in (("k_force_tile", FLOP_FORCE, FLOP_FORCE),
+                           ("k_conv_tile", FLOP_CONV, FLOP_CONV),
+                           ("k_force_sym", FLOP_FORCE, EXEC_FORCE_SYM),
+                           ("k_conv_sym", FLOP_CONV, EXEC_CONV_SYM)):
         tot, cnt = _native.profile_read(name)
-        avg = tot / max(cnt, 1)
+        if cnt == 0:
+            continue
+        avg = tot / cnt
         kern[name] = {"avg_ms": avg, "launches": cnt, "flops": flop * npairs,
-                      "tflops": flop * npairs / (avg * 1e-3) / 1e12 if avg > 0 else 0.0}
+                      "tflops": flop * npairs / (avg * 1e-3) / 1e12 if avg > 0 else 0.0,
+                      "executed_tflops": ex * npairs / (avg * 1e-3) / 1e12 if avg > 0 else 0.0}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     achieved = kern[dom]["flops"] / (kern[dom]["avg_ms"] * 1e-3)
     roof = {"bound": "mfma", "compute": "FP32 VALU (pair sums are not a contraction: no MFMA "
@@ -425,6 +436,10 @@ def main() -> int:
             "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
             "frac_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
+            "executed": kern[dom]["executed_tflops"],
+            "executed_frac_unpacked_peak": kern[dom]["executed_tflops"] * 1e12 / PEAK_FP32_UNPACKED,
+            "note": "achieved counts the algorithmic flops of every ordered pair; the "
+                    "pair-once kernels evaluate each unordered pair once (executed)",
             "traffic": TRAFFIC_N65536.get(dom) if s.n == 65536 else None,
             "traffic_unit": "bytes per launch (rocprofv3 PMC, recorded)",
             "kernels": kern}

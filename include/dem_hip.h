@@ -129,6 +129,14 @@ int dem_direct_f64_targets(const double* pos, const double* radii, int n,
  * (split << 16) | tile. */
 int dem_direct_set_tuning(int split, int tile);
 
+/* Pair-once (Newton's third law) evaluation of the periodic force sum and of
+ * the Oseen sum without the COMSOL correction, for 16384 <= n <= 131072 and
+ * no explicit split: each unordered pair is evaluated once and credited to
+ * both droplets (same accuracy contract, different rounding than the
+ * one-sided kernels).  on = 0 keeps the one-sided kernels.  Returns the
+ * previous setting.  No counterpart in the reference (a tuning switch). */
+int dem_direct_set_symmetric(int on);
+
 /* ------------------------------------------------------------------------ */
 /* Spatial-hash contact detection                                            */
 /* ------------------------------------------------------------------------ */

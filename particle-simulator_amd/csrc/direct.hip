@@ -45,6 +45,33 @@ constexpr int WG = 256;
 
 int g_split = 0;  // 0: heuristic
 int g_tile = 0;   // 0: heuristic
+int g_sym = 1;    // 1: symmetric kernels where they apply, 0: one-sided only
+
+// Symmetric (pair-once) evaluation: blocks of kSymB droplets; block b pairs
+// with the next sym_partners(b, B) blocks cyclically, which covers every
+// unordered block pair exactly once.  Its float64 partials of the reversed
+// pairs take B * B/2 * 256 * 3 doubles, so it runs for mid-size N only.
+constexpr int kSymB = 256;
+constexpr int kSymLds = kSymB + 64;  // sources lane + k, k < 256, without a wrap
+constexpr int kSymMin = 16384;
+constexpr int kSymMax = 131072;
+
+inline int sym_blocks(int n) { return (n + kSymB - 1) / kSymB; }
+__host__ __device__ inline int sym_partners(int b, int B) {
+  if (B & 1) return (B - 1) / 2;
+  return b < B / 2 ? B / 2 : B / 2 - 1;
+}
+inline bool sym_fits(int n) { return n >= kSymMin && n <= kSymMax; }
+// An explicit split (dem_direct_set_tuning) selects the one-sided kernels.
+inline bool use_sym(int n) { return g_sym != 0 && g_split == 0 && sym_fits(n); }
+inline int sym_chunks(int n) {  // grid.y: about 2048 workgroups
+  const int B = sym_blocks(n);
+  int S = (2048 + B - 1) / B;
+  const int E = 1 + B / 2;
+  if (S > E) S = E;
+  if (S > 64) S = 64;
+  return S < 1 ? 1 : S;
+}
 
 inline int split_max(int n) {
   long long cap = (1ll << 21) / (n > 0 ? n : 1);
@@ -75,20 +102,35 @@ constexpr int kBBoxBlocks = 256;
 struct Layout {
   DevScales* sc;
   double* bbox;  // [kBBoxBlocks][6]
-  Rec* rec;
+  Rec* rec;      // [npad]: records beyond n are zero (no radius, no force)
   RecC* recc;
   double* part;  // [split][6][n]
+  double* part_j;  // symmetric kernels: [B][B/2][3][kSymB]
+  int npad;
   size_t bytes;
 };
 
 Layout carve(void* ws, int n) {
   dem::Carver c(ws);
   Layout l;
+  const int nn = n > 0 ? n : 1;
+  l.npad = sym_fits(n) ? sym_blocks(n) * kSymB : nn;
   l.sc = c.take<DevScales>(1);
   l.bbox = c.take<double>(kBBoxBlocks * 6);
-  l.rec = c.take<Rec>((size_t)(n > 0 ? n : 1));
-  l.recc = c.take<RecC>((size_t)(n > 0 ? n : 1));
-  l.part = c.take<double>((size_t)split_max(n) * 6 * (size_t)(n > 0 ? n : 1));
+  l.rec = c.take<Rec>((size_t)l.npad);
+  l.recc = c.take<RecC>((size_t)l.npad);
+  size_t parts = (size_t)split_max(n) * 6 * (size_t)nn;
+  if (sym_fits(n)) {
+    const size_t ps = (size_t)sym_chunks(n) * 3 * (size_t)nn;
+    if (ps > parts) parts = ps;
+  }
+  l.part = c.take<double>(parts);
+  if (sym_fits(n)) {
+    const size_t B = (size_t)sym_blocks(n);
+    l.part_j = c.take<double>(B * (B / 2) * 3 * kSymB);
+  } else {
+    l.part_j = nullptr;
+  }
   l.bytes = dem::align_up(c.used, 256);
   return l;
 }
@@ -148,11 +190,17 @@ __global__ void k_setup(const double* __restrict__ bbox, int nblk, double L, int
 // Quantise positions, scale radii, and (optionally) load caller forces as sources.
 __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
                                               const double* __restrict__ radii,
-                                              const double* __restrict__ forces, int n,
+                                              const double* __restrict__ forces, int n, int npad,
                                               const DevScales* __restrict__ scp,
                                               Rec* __restrict__ rec, RecC* __restrict__ recc) {
   int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n) {
+    if (i < npad) {  // padding: no radius, no force, at the origin
+      rec[i] = Rec{};
+      recc[i] = RecC{};
+    }
+    return;
+  }
   const DevScales sc = *scp;
   Rec o;
   RecC c;
@@ -189,6 +237,180 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
   double sum[3] = {0.0, 0.0, 0.0};
   for (int s = 0; s < S; ++s)
     for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(6 * s + a) * n + i];
+  const DevScales sc = *scp;
+  double Fi[3];
+  float fr[3];
+  dem::force_finish(sum, radii[i], sc, m_const, Fi, fr);
+  for (int a = 0; a < 3; ++a) F[3 * (size_t)i + a] = Fi[a];
+  recc[i].fx = fr[0];
+  recc[i].fy = fr[1];
+  recc[i].fz = fr[2];
+}
+
+// ---------------------------------------------------------------- phase 1, pair-once
+
+// Block b (256 targets, 4 waves) works through entries e of its partner list
+// in [e0, e1): e = 0 is b itself (one-sided, every ordered pair of the
+// block), e >= 1 is block c = (b + e) mod B, whose pairs with b are evaluated
+// once and credited to both sides.  Sources sit in LDS as arrays; lane l at
+// step k takes source slot l + k (the first 64 sources are repeated after
+// 256, so no wrap).  The reversed pair's force is added into the wave's own
+// LDS accumulator of that slot by a plain read-add-write: one instruction's
+// lanes touch distinct slots, and steps run in groups k = kk + 64u
+// (u = 0..3) whose four slot windows [k, k + 63] are disjoint, so only
+// consecutive groups share slots and wave_sync() orders them.  Per (b, c) the
+// four waves' accumulators are summed in fixed order into float64
+// part_j[b][e-1][3][256]; the target side keeps float64 accumulators as the
+// one-sided kernel does.
+
+// Orders one wave's LDS accesses across its lanes: the compiler keeps program
+// order across it, and the LDS executes a wave's instructions in order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ void sym_stage(T* a, T v, int t) {
+  a[t] = v;
+  if (t < 64) a[t + kSymB] = v;
+}
+
+__device__ __forceinline__ void sym_zero(float (*acc)[kSymLds], int lane) {
+  for (int q = lane; q < kSymLds; q += 64) {
+    acc[0][q] = 0.f;
+    acc[1][q] = 0.f;
+    acc[2][q] = 0.f;
+  }
+}
+
+// Reversed-pair sums of source t of the current partner block, all waves.
+__device__ __forceinline__ void sym_fold(const float (*acc)[3][kSymLds], int t,
+                                         double* __restrict__ dst) {
+  for (int a = 0; a < 3; ++a) {
+    double v = 0.0;
+    for (int w = 0; w < 4; ++w) {
+      v += (double)acc[w][a][t];
+      if (t < 64) v += (double)acc[w][a][t + kSymB];
+    }
+    dst[a * kSymB + t] = v;
+  }
+}
+
+__global__ __launch_bounds__(kSymB) void k_force_sym(const Rec* __restrict__ rec, int n, int B,
+                                                     int ec, int tile, double* __restrict__ part,
+                                                     double* __restrict__ part_j) {
+  __shared__ uint32_t sx[kSymLds], sy[kSymLds], sz[kSymLds];
+  __shared__ float sr[kSymLds], sr3[kSymLds];
+  __shared__ float acc[4][3][kSymLds];
+  const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const Rec me = rec[(size_t)b * kSymB + t];
+  const float ri = me.r, nri3 = -me.r3;
+  const int E = 1 + sym_partners(b, B);
+  const int e0 = s * ec, e1 = min(E, e0 + ec);
+  const int gt = max(1, tile / 4);  // step groups per FP32 partial
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    const int c = e == 0 ? b : (b + e) % B;
+    __syncthreads();  // the previous entry's LDS reads are done
+    const Rec rc = rec[(size_t)c * kSymB + t];
+    sym_stage(sx, rc.x, t);
+    sym_stage(sy, rc.y, t);
+    sym_stage(sz, rc.z, t);
+    sym_stage(sr, rc.r, t);
+    sym_stage(sr3, rc.r3, t);
+    sym_zero(acc[w], lane);
+    __syncthreads();
+    if (e == 0) {
+      for (int kt = 0; kt < kSymB; kt += tile) {
+        const int ke = min(kt + tile, kSymB);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 4
+        for (int k = kt; k < ke; ++k) {
+          const int q = lane + k;
+          dem::dipole_pair<false>(dem::diff32(sx[q], me.x), dem::diff32(sy[q], me.y),
+                                  dem::diff32(sz[q], me.z), ri, sr[q], sr3[q], fx, fy, fz);
+        }
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
+      }
+      continue;
+    }
+    for (int g0 = 0; g0 < 64; g0 += gt) {
+      const int ge = min(g0 + gt, 64);
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      for (int kk = g0; kk < ge; ++kk) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = lane + kk + 64 * u;
+          const float dx = dem::diff32(sx[q], me.x);
+          const float dy = dem::diff32(sy[q], me.y);
+          const float dz = dem::diff32(sz[q], me.z);
+          const float dz2 = dz * dz;
+          const float dxy2 = dx * dx + dy * dy;
+          const float r2f = dxy2 + dz2;
+          const float r2 = r2f * dem::kSigma2f;
+          const float ir = __builtin_amdgcn_rsqf(r2);
+          const float ir2 = ir * ir;
+          const float ir7 = (ir2 * ir2) * (ir2 * ir);
+          const float sm = dem::separated<false>(r2, ri + sr[q]) ? ir7 : 0.f;  // select
+          const float A = 4.f * dz2 - dxy2;
+          const float Bz = A - 2.f * r2f;  // 2 dz^2 - 3 (dx^2 + dy^2)
+          const float cxy = sm * A, cz = sm * Bz;
+          const float gx = cxy * dx, gy = cxy * dy, gz = cz * dz;
+          const float rj3 = sr3[q];
+          fx += rj3 * gx;
+          fy += rj3 * gy;
+          fz += rj3 * gz;
+          // g is odd in d: the reversed pair adds -r_i^3 g to source q
+          acc[w][0][q] = fmaf(nri3, gx, acc[w][0][q]);
+          acc[w][1][q] = fmaf(nri3, gy, acc[w][1][q]);
+          acc[w][2][q] = fmaf(nri3, gz, acc[w][2][q]);
+        }
+        wave_sync();
+      }
+      ax += (double)fx;
+      ay += (double)fy;
+      az += (double)fz;
+    }
+    __syncthreads();
+    sym_fold(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
+  }
+  const int i = b * kSymB + t;
+  if (i < n) {
+    part[(size_t)(3 * s + 0) * n + i] = ax;
+    part[(size_t)(3 * s + 1) * n + i] = ay;
+    part[(size_t)(3 * s + 2) * n + i] = az;
+  }
+}
+
+// Sum of a target's pair-once partials: its own chunks, then the reversed
+// pairs from every block that has its block as a partner (ascending b).
+__device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, int n,
+                                        const double* __restrict__ part_j, int B, int i,
+                                        double* sum) {
+  for (int s = 0; s < S; ++s)
+    for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
+  const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
+  for (int b = 0; b < B; ++b) {
+    if (b == bi) continue;
+    const int p = (bi - b - 1 + B) % B;
+    if (p >= sym_partners(b, B)) continue;
+    for (int a = 0; a < 3; ++a)
+      sum[a] += part_j[(((size_t)b * Pmax + p) * 3 + a) * kSymB + il];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_force_sym_reduce(
+    const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
+    const double* __restrict__ radii, const DevScales* __restrict__ scp, double m_const,
+    double* __restrict__ F, RecC* __restrict__ recc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double sum[3] = {0.0, 0.0, 0.0};
+  sym_sum(part, S, n, part_j, B, i, sum);
   const DevScales sc = *scp;
   double Fi[3];
   float fr[3];
@@ -251,6 +473,117 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
       if (cs.enabled) c[a] += part[(size_t)(6 * s + 3 + a) * n + i];
     }
   const DevScales sc = *scp;
+  double Vi[3];
+  dem::conv_finish(m, c, sc, eta_const, cs, F + 3 * (size_t)i, Vi);
+  for (int a = 0; a < 3; ++a) V[3 * (size_t)i + a] = Vi[a];
+}
+
+// Oseen sum, pair-once (no periodic correction): the Oseen tensor is even in
+// the separation, so the pair's tensor serves both sides; same schedule, slot
+// order and partial layout as k_force_sym.  Separations are the high words of
+// the 64-bit ring differences, as in the one-sided kernel: a pair whose
+// separation lies within one 32-bit unit of half the box along an axis still
+// takes the image the exact positions give (32-bit positions could round it
+// to the other image, which flips that component of the pair's term).
+__global__ __launch_bounds__(kSymB) void k_conv_sym(const RecC* __restrict__ rec, int n, int B,
+                                                    int ec, int tile, double* __restrict__ part,
+                                                    double* __restrict__ part_j) {
+  __shared__ unsigned long long sx[kSymLds], sy[kSymLds], sz[kSymLds];
+  __shared__ float sfx[kSymLds], sfy[kSymLds], sfz[kSymLds];
+  __shared__ float acc[4][3][kSymLds];
+  const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const RecC me = rec[(size_t)b * kSymB + t];
+  const unsigned long long nxi = 0ull - (((unsigned long long)me.xh << 32) | me.xl);
+  const unsigned long long nyi = 0ull - (((unsigned long long)me.yh << 32) | me.yl);
+  const unsigned long long nzi = 0ull - (((unsigned long long)me.zh << 32) | me.zl);
+  const float fxi = me.fx, fyi = me.fy, fzi = me.fz;
+  const int E = 1 + sym_partners(b, B);
+  const int e0 = s * ec, e1 = min(E, e0 + ec);
+  const int gt = max(1, tile / 4);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    const int c = e == 0 ? b : (b + e) % B;
+    __syncthreads();
+    const RecC rc = rec[(size_t)c * kSymB + t];
+    sym_stage(sx, ((unsigned long long)rc.xh << 32) | rc.xl, t);
+    sym_stage(sy, ((unsigned long long)rc.yh << 32) | rc.yl, t);
+    sym_stage(sz, ((unsigned long long)rc.zh << 32) | rc.zl, t);
+    sym_stage(sfx, rc.fx, t);
+    sym_stage(sfy, rc.fy, t);
+    sym_stage(sfz, rc.fz, t);
+    sym_zero(acc[w], lane);
+    __syncthreads();
+    if (e == 0) {
+      for (int kt = 0; kt < kSymB; kt += tile) {
+        const int ke = min(kt + tile, kSymB);
+        float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll 4
+        for (int k = kt; k < ke; ++k) {
+          const int q = lane + k;
+          const float dx = dem::diff64n(sx[q], nxi);
+          const float dy = dem::diff64n(sy[q], nyi);
+          const float dz = dem::diff64n(sz[q], nzi);
+          dem::oseen_pair(dx, dy, dz, sfx[q], sfy[q], sfz[q],
+                          dx * dx + dy * dy + dz * dz > 0.f,  // drops j == i
+                          vx, vy, vz);
+        }
+        ax += (double)vx;
+        ay += (double)vy;
+        az += (double)vz;
+      }
+      continue;
+    }
+    for (int g0 = 0; g0 < 64; g0 += gt) {
+      const int ge = min(g0 + gt, 64);
+      float vx = 0.f, vy = 0.f, vz = 0.f;
+      for (int kk = g0; kk < ge; ++kk) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = lane + kk + 64 * u;
+          const float dx = dem::diff64n(sx[q], nxi);
+          const float dy = dem::diff64n(sy[q], nyi);
+          const float dz = dem::diff64n(sz[q], nzi);
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          const float ir = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;  // coincident: nothing
+          const float ir2 = ir * ir;
+          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
+          const float tj = (dx * Fx + dy * Fy + dz * Fz) * ir2;
+          vx += ir * (Fx + tj * dx);
+          vy += ir * (Fy + tj * dy);
+          vz += ir * (Fz + tj * dz);
+          const float ti = (dx * fxi + dy * fyi + dz * fzi) * ir2;
+          acc[w][0][q] = fmaf(ir, fxi + ti * dx, acc[w][0][q]);
+          acc[w][1][q] = fmaf(ir, fyi + ti * dy, acc[w][1][q]);
+          acc[w][2][q] = fmaf(ir, fzi + ti * dz, acc[w][2][q]);
+        }
+        wave_sync();
+      }
+      ax += (double)vx;
+      ay += (double)vy;
+      az += (double)vz;
+    }
+    __syncthreads();
+    sym_fold(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
+  }
+  const int i = b * kSymB + t;
+  if (i < n) {
+    part[(size_t)(3 * s + 0) * n + i] = ax;
+    part[(size_t)(3 * s + 1) * n + i] = ay;
+    part[(size_t)(3 * s + 2) * n + i] = az;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_conv_sym_reduce(
+    const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
+    const DevScales* __restrict__ scp, double eta_const, const double* __restrict__ F,
+    double* __restrict__ V) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double m[3] = {0.0, 0.0, 0.0};
+  const double c[3] = {0.0, 0.0, 0.0};
+  sym_sum(part, S, n, part_j, B, i, m);
+  const DevScales sc = *scp;
+  const dem::CorrScalars cs{};
   double Vi[3];
   dem::conv_finish(m, c, sc, eta_const, cs, F + 3 * (size_t)i, Vi);
   for (int a = 0; a < 3; ++a) V[3 * (size_t)i + a] = Vi[a];
@@ -414,14 +747,28 @@ int prepare(const Layout& l, const double* pos, const double* radii, const doubl
   hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, l.bbox, nblk, ph->L, ph->periodic,
                      ph->m_const, l.sc);
   DEM_CHECK_LAUNCH("k_setup");
-  hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, st, pos, radii, forces, n, l.sc,
-                     l.rec, l.recc);
+  hipLaunchKernelGGL(k_pack, dim3((l.npad + 255) / 256), dim3(256), 0, st, pos, radii, forces, n,
+                     l.npad, l.sc, l.rec, l.recc);
   DEM_CHECK_LAUNCH("k_pack");
   return DEM_OK;
 }
 
 int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, double* F,
                hipStream_t st) {
+  if (use_sym(n)) {
+    const int B = sym_blocks(n), S = sym_chunks(n);
+    const int ec = (1 + B / 2 + S - 1) / S;
+    DEM_PROF_LAUNCH("k_force_sym", st,
+                    hipLaunchKernelGGL(k_force_sym, dim3(B, S), dim3(kSymB), 0, st,
+                                       (const Rec*)l.rec, n, B, ec, choose_tile_force(), l.part,
+                                       l.part_j));
+    DEM_CHECK_LAUNCH("k_force_sym");
+    hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 255) / 256), dim3(256), 0, st,
+                       (const double*)l.part, S, n, (const double*)l.part_j, B, radii,
+                       (const DevScales*)l.sc, ph->m_const, F, l.recc);
+    DEM_CHECK_LAUNCH("k_force_sym_reduce");
+    return DEM_OK;
+  }
   const int S = choose_split(n);
   const int jchunk = (n + S - 1) / S;
   dim3 grid((n + WG - 1) / WG, S);
@@ -443,6 +790,20 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
   CorrArgs ca{};
   dem::CorrScalars cs{};
   const bool on = corr && corr->enabled;
+  if (!on && use_sym(n)) {
+    const int B = sym_blocks(n), S = sym_chunks(n);
+    const int ec = (1 + B / 2 + S - 1) / S;
+    DEM_PROF_LAUNCH("k_conv_sym", st,
+                    hipLaunchKernelGGL(k_conv_sym, dim3(B, S), dim3(kSymB), 0, st,
+                                       (const RecC*)l.recc, n, B, ec, choose_tile_conv(),
+                                       l.part, l.part_j));
+    DEM_CHECK_LAUNCH("k_conv_sym");
+    hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 255) / 256), dim3(256), 0, st,
+                       (const double*)l.part, S, n, (const double*)l.part_j, B,
+                       (const DevScales*)l.sc, ph->eta_const, F, V);
+    DEM_CHECK_LAUNCH("k_conv_sym_reduce");
+    return DEM_OK;
+  }
   if (on) {
     ca.grid = reinterpret_cast<const float4*>(corr->grid);
     ca.n = corr->n;
@@ -475,6 +836,12 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
 extern "C" {
 
 size_t dem_direct_workspace_bytes(int n) { return carve(nullptr, n < 0 ? 0 : n).bytes; }
+
+int dem_direct_set_symmetric(int on) {
+  const int prev = g_sym;
+  g_sym = on ? 1 : 0;
+  return prev;
+}
 
 int dem_direct_set_tuning(int split, int tile) {
   int prev = (g_split << 16) | (g_tile & 0xffff);

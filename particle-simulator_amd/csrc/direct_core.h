@@ -153,6 +153,7 @@ __device__ __forceinline__ void conv_partial(const RecC* __restrict__ rec, unsig
                                              unsigned long long yi, unsigned long long zi,
                                              const CorrGrid& cg, int j0, int j1, int tile,
                                              double* m, double* c) {
+  const unsigned long long nxi = 0ull - xi, nyi = 0ull - yi, nzi = 0ull - zi;
   for (int jt = j0; jt < j1; jt += tile) {
     const int je = min(j1, jt + tile);
     float vx = 0.f, vy = 0.f, vz = 0.f;
@@ -160,9 +161,9 @@ __device__ __forceinline__ void conv_partial(const RecC* __restrict__ rec, unsig
 #pragma unroll 4
     for (int j = jt; j < je; ++j) {
       const RecC sj = rec[j];
-      const float dx = diff64(sj.xh, sj.xl, xi);
-      const float dy = diff64(sj.yh, sj.yl, yi);
-      const float dz = diff64(sj.zh, sj.zl, zi);
+      const float dx = diff64n(((unsigned long long)sj.xh << 32) | sj.xl, nxi);
+      const float dy = diff64n(((unsigned long long)sj.yh << 32) | sj.yl, nyi);
+      const float dz = diff64n(((unsigned long long)sj.zh << 32) | sj.zl, nzi);
       const bool other = dx * dx + dy * dy + dz * dz > 0.f;  // drops j == i
       oseen_pair(dx, dy, dz, sj.fx, sj.fy, sj.fz, other, vx, vy, vz);
       if (CORR)

@@ -80,6 +80,13 @@ __device__ __forceinline__ float diff64(uint32_t jh, uint32_t jl, unsigned long 
   return (float)(int32_t)(((((unsigned long long)jh << 32) | jl) - qi) >> 32);
 }
 
+// The same difference from the 64-bit source position and the negated target
+// position: qj + (-qi) is one 64-bit add (v_lshl_add_u64) instead of a
+// subtract with borrow; bit-identical to diff64.
+__device__ __forceinline__ float diff64n(unsigned long long qj, unsigned long long nqi) {
+  return (float)(int32_t)((qj + nqi) >> 32);
+}
+
 // 32-bit minimum-image difference qj - qi, as float.
 __device__ __forceinline__ float diff32(uint32_t qj, uint32_t qi) {
   return (float)(int32_t)(qj - qi);
