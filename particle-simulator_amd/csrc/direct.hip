@@ -53,6 +53,8 @@ int g_sym = 1;    // 1: symmetric kernels where they apply, 0: one-sided only
 // pairs take B * B/2 * 256 * 3 doubles, so it runs for mid-size N only.
 constexpr int kSymB = 256;
 constexpr int kSymLds = kSymB + 64;  // sources lane + k, k < 256, without a wrap
+constexpr int kSymTplForce = 2;  // targets per thread, force / Oseen kernels
+constexpr int kSymTplConv = 2;
 constexpr int kSymMin = 16384;
 constexpr int kSymMax = 131072;
 
@@ -64,12 +66,15 @@ __host__ __device__ inline int sym_partners(int b, int B) {
 inline bool sym_fits(int n) { return n >= kSymMin && n <= kSymMax; }
 // An explicit split (dem_direct_set_tuning) selects the one-sided kernels.
 inline bool use_sym(int n) { return g_sym != 0 && g_split == 0 && sym_fits(n); }
-inline int sym_chunks(int n) {  // grid.y: about 2048 workgroups
+constexpr int kSymMaxChunks = 64;
+// grid.y for `slots` co-resident workgroups: one round of B x S workgroups,
+// at most one chunk per partner entry.
+inline int sym_chunks(int n, int slots) {
   const int B = sym_blocks(n);
-  int S = (2048 + B - 1) / B;
+  int S = slots / B;
   const int E = 1 + B / 2;
   if (S > E) S = E;
-  if (S > 64) S = 64;
+  if (S > kSymMaxChunks) S = kSymMaxChunks;
   return S < 1 ? 1 : S;
 }
 
@@ -121,7 +126,7 @@ Layout carve(void* ws, int n) {
   l.recc = c.take<RecC>((size_t)l.npad);
   size_t parts = (size_t)split_max(n) * 6 * (size_t)nn;
   if (sym_fits(n)) {
-    const size_t ps = (size_t)sym_chunks(n) * 3 * (size_t)nn;
+    const size_t ps = (size_t)kSymMaxChunks * 3 * (size_t)nn;
     if (ps > parts) parts = ps;
   }
   l.part = c.take<double>(parts);
@@ -249,17 +254,18 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
 
 // ---------------------------------------------------------------- phase 1, pair-once
 
-// Block b (256 targets, 4 waves) works through entries e of its partner list
-// in [e0, e1): e = 0 is b itself (one-sided, every ordered pair of the
-// block), e >= 1 is block c = (b + e) mod B, whose pairs with b are evaluated
-// once and credited to both sides.  Sources sit in LDS as arrays; lane l at
-// step k takes source slot l + k (the first 64 sources are repeated after
-// 256, so no wrap).  The reversed pair's force is added into the wave's own
-// LDS accumulator of that slot by a plain read-add-write: one instruction's
-// lanes touch distinct slots, and steps run in groups k = kk + 64u
-// (u = 0..3) whose four slot windows [k, k + 63] are disjoint, so only
-// consecutive groups share slots and wave_sync() orders them.  Per (b, c) the
-// four waves' accumulators are summed in fixed order into float64
+// Block b (256 targets; 256 / TPL threads, thread t holding targets
+// t + (256 / TPL) h, h < TPL) works through entries e of its partner list in
+// [e0, e1): e = 0 is b itself (one-sided, every ordered pair of the block),
+// e >= 1 is block c = (b + e) mod B, whose pairs with b are evaluated once
+// and credited to both sides.  Sources sit in LDS as arrays; lane l at step
+// k takes source slot l + k (the first 64 sources are repeated after 256, so
+// no wrap) for each of its targets.  The reversed pairs' sums are added into
+// the wave's own LDS accumulator of that slot by a plain read-add-write: one
+// instruction's lanes touch distinct slots, and steps run in groups
+// k = kk + 64u (u = 0..3) whose four slot windows [k, k + 63] are disjoint, so
+// only consecutive groups share slots and wave_sync() orders them.  Per
+// (b, c) the waves' accumulators are summed in fixed order into float64
 // part_j[b][e-1][3][256]; the target side keeps float64 accumulators as the
 // one-sided kernel does.
 
@@ -271,10 +277,11 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Source slot s (0..255) and, for s < 64, its repeat at s + 256.
 template <typename T>
-__device__ __forceinline__ void sym_stage(T* a, T v, int t) {
-  a[t] = v;
-  if (t < 64) a[t + kSymB] = v;
+__device__ __forceinline__ void sym_stage(T* a, T v, int s) {
+  a[s] = v;
+  if (s < 64) a[s + kSymB] = v;
 }
 
 __device__ __forceinline__ void sym_zero(float (*acc)[kSymLds], int lane) {
@@ -285,104 +292,139 @@ __device__ __forceinline__ void sym_zero(float (*acc)[kSymLds], int lane) {
   }
 }
 
-// Reversed-pair sums of source t of the current partner block, all waves.
+// Reversed-pair sums of the partner block's sources t + T h, all W waves.
+template <int T, int W>
 __device__ __forceinline__ void sym_fold(const float (*acc)[3][kSymLds], int t,
                                          double* __restrict__ dst) {
-  for (int a = 0; a < 3; ++a) {
-    double v = 0.0;
-    for (int w = 0; w < 4; ++w) {
-      v += (double)acc[w][a][t];
-      if (t < 64) v += (double)acc[w][a][t + kSymB];
+  for (int slot = t; slot < kSymB; slot += T) {
+    for (int a = 0; a < 3; ++a) {
+      double v = 0.0;
+      for (int w = 0; w < W; ++w) {
+        v += (double)acc[w][a][slot];
+        if (slot < 64) v += (double)acc[w][a][slot + kSymB];
+      }
+      dst[a * kSymB + slot] = v;
     }
-    dst[a * kSymB + t] = v;
   }
 }
 
-__global__ __launch_bounds__(kSymB) void k_force_sym(const Rec* __restrict__ rec, int n, int B,
-                                                     int ec, int tile, double* __restrict__ part,
-                                                     double* __restrict__ part_j) {
+// Dipole pair of one target with a source (radii rif, rjf in fixed units, rj3
+// in work units): adds to the target's FP32 sums and returns g (the pair's
+// force per r_i^3 r_j^3, odd in d) for the reversed side.  The contact mask
+// |d| > r_i + r_j is h = |d|^2 - (r_i + r_j)^2 > 0 in fixed units, where a
+// positive h is at least 2^20 (an ulp of |d|^2 there) and so exceeds 1/|d|
+// in work units: med3(h, 0, 1/|d|) is 1/|d| for separated pairs and 0 for
+// touching ones and for j == i.
+__device__ __forceinline__ void force_pair_sym(float dx, float dy, float dz, float rif, float rjf,
+                                               float rj3, float* f, float* g) {
+  const float dz2 = dz * dz;
+  const float dxy2 = dx * dx + dy * dy;
+  const float r2f = dxy2 + dz2;
+  const float rsf = rif + rjf;
+  const float h = fmaf(-rsf, rsf, r2f);
+  const float ir = __builtin_amdgcn_fmed3f(h, 0.f, __builtin_amdgcn_rsqf(r2f * dem::kSigma2f));
+  const float ir2 = ir * ir;
+  const float sm = (ir2 * ir2) * (ir2 * ir);
+  const float A = 4.f * dz2 - dxy2;
+  const float Bz = A - 2.f * r2f;  // 2 dz^2 - 3 (dx^2 + dy^2)
+  const float cxy = sm * A, cz = sm * Bz;
+  g[0] = cxy * dx;
+  g[1] = cxy * dy;
+  g[2] = cz * dz;
+  f[0] += rj3 * g[0];
+  f[1] += rj3 * g[1];
+  f[2] += rj3 * g[2];
+}
+
+template <int TPL>
+__global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec, int n,
+                                                           int B, int ec, int tile,
+                                                           double* __restrict__ part,
+                                                           double* __restrict__ part_j) {
+  constexpr int T = kSymB / TPL, W = T / 64;
   __shared__ uint32_t sx[kSymLds], sy[kSymLds], sz[kSymLds];
-  __shared__ float sr[kSymLds], sr3[kSymLds];
-  __shared__ float acc[4][3][kSymLds];
+  __shared__ float sr[kSymLds], sr3[kSymLds];  // radius in fixed units, r^3 in work units
+  __shared__ float acc[W][3][kSymLds];
   const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const Rec me = rec[(size_t)b * kSymB + t];
-  const float ri = me.r, nri3 = -me.r3;
+  Rec m[TPL];
+  float nr3[TPL], rf[TPL];
+  double ai[TPL][3];
+  for (int h = 0; h < TPL; ++h) {
+    m[h] = rec[(size_t)b * kSymB + t + T * h];
+    nr3[h] = -m[h].r3;
+    rf[h] = m[h].r * 0x1p22f;
+    ai[h][0] = ai[h][1] = ai[h][2] = 0.0;
+  }
   const int E = 1 + sym_partners(b, B);
   const int e0 = s * ec, e1 = min(E, e0 + ec);
   const int gt = max(1, tile / 4);  // step groups per FP32 partial
-  double ax = 0.0, ay = 0.0, az = 0.0;
   for (int e = e0; e < e1; ++e) {
     const int c = e == 0 ? b : (b + e) % B;
     __syncthreads();  // the previous entry's LDS reads are done
-    const Rec rc = rec[(size_t)c * kSymB + t];
-    sym_stage(sx, rc.x, t);
-    sym_stage(sy, rc.y, t);
-    sym_stage(sz, rc.z, t);
-    sym_stage(sr, rc.r, t);
-    sym_stage(sr3, rc.r3, t);
+    for (int sl = t; sl < kSymB; sl += T) {
+      const Rec rc = rec[(size_t)c * kSymB + sl];
+      sym_stage(sx, rc.x, sl);
+      sym_stage(sy, rc.y, sl);
+      sym_stage(sz, rc.z, sl);
+      sym_stage(sr, rc.r * 0x1p22f, sl);
+      sym_stage(sr3, rc.r3, sl);
+    }
     sym_zero(acc[w], lane);
     __syncthreads();
     if (e == 0) {
       for (int kt = 0; kt < kSymB; kt += tile) {
         const int ke = min(kt + tile, kSymB);
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll 4
+        float f[TPL][3] = {};
+#pragma unroll 4 / TPL
         for (int k = kt; k < ke; ++k) {
           const int q = lane + k;
-          dem::dipole_pair<false>(dem::diff32(sx[q], me.x), dem::diff32(sy[q], me.y),
-                                  dem::diff32(sz[q], me.z), ri, sr[q], sr3[q], fx, fy, fz);
+          const uint32_t xj = sx[q], yj = sy[q], zj = sz[q];
+          const float rj = sr[q], rj3 = sr3[q];
+#pragma unroll
+          for (int h = 0; h < TPL; ++h)
+            dem::dipole_pair<false>(dem::diff32(xj, m[h].x), dem::diff32(yj, m[h].y),
+                                    dem::diff32(zj, m[h].z), m[h].r, rj * 0x1p-22f, rj3, f[h][0],
+                                    f[h][1], f[h][2]);
         }
-        ax += (double)fx;
-        ay += (double)fy;
-        az += (double)fz;
+        for (int h = 0; h < TPL; ++h)
+          for (int a = 0; a < 3; ++a) ai[h][a] += (double)f[h][a];
       }
       continue;
     }
     for (int g0 = 0; g0 < 64; g0 += gt) {
       const int ge = min(g0 + gt, 64);
-      float fx = 0.f, fy = 0.f, fz = 0.f;
+      float f[TPL][3] = {};
       for (int kk = g0; kk < ge; ++kk) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int q = lane + kk + 64 * u;
-          const float dx = dem::diff32(sx[q], me.x);
-          const float dy = dem::diff32(sy[q], me.y);
-          const float dz = dem::diff32(sz[q], me.z);
-          const float dz2 = dz * dz;
-          const float dxy2 = dx * dx + dy * dy;
-          const float r2f = dxy2 + dz2;
-          const float r2 = r2f * dem::kSigma2f;
-          const float ir = __builtin_amdgcn_rsqf(r2);
-          const float ir2 = ir * ir;
-          const float ir7 = (ir2 * ir2) * (ir2 * ir);
-          const float sm = dem::separated<false>(r2, ri + sr[q]) ? ir7 : 0.f;  // select
-          const float A = 4.f * dz2 - dxy2;
-          const float Bz = A - 2.f * r2f;  // 2 dz^2 - 3 (dx^2 + dy^2)
-          const float cxy = sm * A, cz = sm * Bz;
-          const float gx = cxy * dx, gy = cxy * dy, gz = cz * dz;
-          const float rj3 = sr3[q];
-          fx += rj3 * gx;
-          fy += rj3 * gy;
-          fz += rj3 * gz;
-          // g is odd in d: the reversed pair adds -r_i^3 g to source q
-          acc[w][0][q] = fmaf(nri3, gx, acc[w][0][q]);
-          acc[w][1][q] = fmaf(nri3, gy, acc[w][1][q]);
-          acc[w][2][q] = fmaf(nri3, gz, acc[w][2][q]);
+          const uint32_t xj = sx[q], yj = sy[q], zj = sz[q];
+          const float rj = sr[q], rj3 = sr3[q];
+          float o[3] = {acc[w][0][q], acc[w][1][q], acc[w][2][q]};
+#pragma unroll
+          for (int h = 0; h < TPL; ++h) {
+            float g[3];
+            force_pair_sym(dem::diff32(xj, m[h].x), dem::diff32(yj, m[h].y),
+                           dem::diff32(zj, m[h].z), rf[h], rj, rj3, f[h], g);
+            // g is odd in d: the reversed pair adds -r_i^3 g to source q
+            for (int a = 0; a < 3; ++a) o[a] = fmaf(nr3[h], g[a], o[a]);
+          }
+          acc[w][0][q] = o[0];
+          acc[w][1][q] = o[1];
+          acc[w][2][q] = o[2];
         }
         wave_sync();
       }
-      ax += (double)fx;
-      ay += (double)fy;
-      az += (double)fz;
+      for (int h = 0; h < TPL; ++h)
+        for (int a = 0; a < 3; ++a) ai[h][a] += (double)f[h][a];
     }
     __syncthreads();
-    sym_fold(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
+    sym_fold<T, W>(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
   }
-  const int i = b * kSymB + t;
-  if (i < n) {
-    part[(size_t)(3 * s + 0) * n + i] = ax;
-    part[(size_t)(3 * s + 1) * n + i] = ay;
-    part[(size_t)(3 * s + 2) * n + i] = az;
+  for (int h = 0; h < TPL; ++h) {
+    const int i = b * kSymB + t + T * h;
+    if (i < n)
+      for (int a = 0; a < 3; ++a) part[(size_t)(3 * s + a) * n + i] = ai[h][a];
   }
 }
 
@@ -485,91 +527,123 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
 // separation lies within one 32-bit unit of half the box along an axis still
 // takes the image the exact positions give (32-bit positions could round it
 // to the other image, which flips that component of the pair's term).
-__global__ __launch_bounds__(kSymB) void k_conv_sym(const RecC* __restrict__ rec, int n, int B,
-                                                    int ec, int tile, double* __restrict__ part,
-                                                    double* __restrict__ part_j) {
+
+// Oseen pair of one target (force fi) with a source (force Fj): adds the
+// source's term to the target's sums and returns the target's term for the
+// source as ir * o.
+__device__ __forceinline__ float oseen_pair_sym(float dx, float dy, float dz, float Fx, float Fy,
+                                                float Fz, const float* fi, float* v, float* o) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  // coincident droplets add nothing: a nonzero r2 is at least 1 >= 1/|d|
+  const float ir = __builtin_amdgcn_fmed3f(r2, 0.f, __builtin_amdgcn_rsqf(r2));
+  const float ir2 = ir * ir;
+  const float tj = (dx * Fx + dy * Fy + dz * Fz) * ir2;
+  v[0] += ir * (Fx + tj * dx);
+  v[1] += ir * (Fy + tj * dy);
+  v[2] += ir * (Fz + tj * dz);
+  const float ti = (dx * fi[0] + dy * fi[1] + dz * fi[2]) * ir2;
+  o[0] = fi[0] + ti * dx;
+  o[1] = fi[1] + ti * dy;
+  o[2] = fi[2] + ti * dz;
+  return ir;
+}
+
+template <int TPL>
+__global__ __launch_bounds__(kSymB / TPL) void k_conv_sym(const RecC* __restrict__ rec, int n,
+                                                          int B, int ec, int tile,
+                                                          double* __restrict__ part,
+                                                          double* __restrict__ part_j) {
+  constexpr int T = kSymB / TPL, W = T / 64;
   __shared__ unsigned long long sx[kSymLds], sy[kSymLds], sz[kSymLds];
   __shared__ float sfx[kSymLds], sfy[kSymLds], sfz[kSymLds];
-  __shared__ float acc[4][3][kSymLds];
+  __shared__ float acc[W][3][kSymLds];
   const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const RecC me = rec[(size_t)b * kSymB + t];
-  const unsigned long long nxi = 0ull - (((unsigned long long)me.xh << 32) | me.xl);
-  const unsigned long long nyi = 0ull - (((unsigned long long)me.yh << 32) | me.yl);
-  const unsigned long long nzi = 0ull - (((unsigned long long)me.zh << 32) | me.zl);
-  const float fxi = me.fx, fyi = me.fy, fzi = me.fz;
+  unsigned long long nq[TPL][3];
+  float fi[TPL][3];
+  double ai[TPL][3];
+  for (int h = 0; h < TPL; ++h) {
+    const RecC m = rec[(size_t)b * kSymB + t + T * h];
+    nq[h][0] = 0ull - (((unsigned long long)m.xh << 32) | m.xl);
+    nq[h][1] = 0ull - (((unsigned long long)m.yh << 32) | m.yl);
+    nq[h][2] = 0ull - (((unsigned long long)m.zh << 32) | m.zl);
+    fi[h][0] = m.fx;
+    fi[h][1] = m.fy;
+    fi[h][2] = m.fz;
+    ai[h][0] = ai[h][1] = ai[h][2] = 0.0;
+  }
   const int E = 1 + sym_partners(b, B);
   const int e0 = s * ec, e1 = min(E, e0 + ec);
   const int gt = max(1, tile / 4);
-  double ax = 0.0, ay = 0.0, az = 0.0;
   for (int e = e0; e < e1; ++e) {
     const int c = e == 0 ? b : (b + e) % B;
     __syncthreads();
-    const RecC rc = rec[(size_t)c * kSymB + t];
-    sym_stage(sx, ((unsigned long long)rc.xh << 32) | rc.xl, t);
-    sym_stage(sy, ((unsigned long long)rc.yh << 32) | rc.yl, t);
-    sym_stage(sz, ((unsigned long long)rc.zh << 32) | rc.zl, t);
-    sym_stage(sfx, rc.fx, t);
-    sym_stage(sfy, rc.fy, t);
-    sym_stage(sfz, rc.fz, t);
+    for (int sl = t; sl < kSymB; sl += T) {
+      const RecC rc = rec[(size_t)c * kSymB + sl];
+      sym_stage(sx, ((unsigned long long)rc.xh << 32) | rc.xl, sl);
+      sym_stage(sy, ((unsigned long long)rc.yh << 32) | rc.yl, sl);
+      sym_stage(sz, ((unsigned long long)rc.zh << 32) | rc.zl, sl);
+      sym_stage(sfx, rc.fx, sl);
+      sym_stage(sfy, rc.fy, sl);
+      sym_stage(sfz, rc.fz, sl);
+    }
     sym_zero(acc[w], lane);
     __syncthreads();
     if (e == 0) {
       for (int kt = 0; kt < kSymB; kt += tile) {
         const int ke = min(kt + tile, kSymB);
-        float vx = 0.f, vy = 0.f, vz = 0.f;
-#pragma unroll 4
+        float v[TPL][3] = {};
+#pragma unroll 4 / TPL
         for (int k = kt; k < ke; ++k) {
           const int q = lane + k;
-          const float dx = dem::diff64n(sx[q], nxi);
-          const float dy = dem::diff64n(sy[q], nyi);
-          const float dz = dem::diff64n(sz[q], nzi);
-          dem::oseen_pair(dx, dy, dz, sfx[q], sfy[q], sfz[q],
-                          dx * dx + dy * dy + dz * dz > 0.f,  // drops j == i
-                          vx, vy, vz);
+          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
+          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
+#pragma unroll
+          for (int h = 0; h < TPL; ++h) {
+            const float dx = dem::diff64n(xj, nq[h][0]), dy = dem::diff64n(yj, nq[h][1]),
+                        dz = dem::diff64n(zj, nq[h][2]);
+            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, dx * dx + dy * dy + dz * dz > 0.f,  // j != i
+                            v[h][0], v[h][1], v[h][2]);
+          }
         }
-        ax += (double)vx;
-        ay += (double)vy;
-        az += (double)vz;
+        for (int h = 0; h < TPL; ++h)
+          for (int a = 0; a < 3; ++a) ai[h][a] += (double)v[h][a];
       }
       continue;
     }
     for (int g0 = 0; g0 < 64; g0 += gt) {
       const int ge = min(g0 + gt, 64);
-      float vx = 0.f, vy = 0.f, vz = 0.f;
+      float v[TPL][3] = {};
       for (int kk = g0; kk < ge; ++kk) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int q = lane + kk + 64 * u;
-          const float dx = dem::diff64n(sx[q], nxi);
-          const float dy = dem::diff64n(sy[q], nyi);
-          const float dz = dem::diff64n(sz[q], nzi);
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          const float ir = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;  // coincident: nothing
-          const float ir2 = ir * ir;
+          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
           const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
-          const float tj = (dx * Fx + dy * Fy + dz * Fz) * ir2;
-          vx += ir * (Fx + tj * dx);
-          vy += ir * (Fy + tj * dy);
-          vz += ir * (Fz + tj * dz);
-          const float ti = (dx * fxi + dy * fyi + dz * fzi) * ir2;
-          acc[w][0][q] = fmaf(ir, fxi + ti * dx, acc[w][0][q]);
-          acc[w][1][q] = fmaf(ir, fyi + ti * dy, acc[w][1][q]);
-          acc[w][2][q] = fmaf(ir, fzi + ti * dz, acc[w][2][q]);
+          float o[3] = {acc[w][0][q], acc[w][1][q], acc[w][2][q]};
+#pragma unroll
+          for (int h = 0; h < TPL; ++h) {
+            float oh[3];
+            const float ir = oseen_pair_sym(dem::diff64n(xj, nq[h][0]), dem::diff64n(yj, nq[h][1]),
+                                            dem::diff64n(zj, nq[h][2]), Fx, Fy, Fz, fi[h], v[h],
+                                            oh);
+            for (int a = 0; a < 3; ++a) o[a] = fmaf(ir, oh[a], o[a]);
+          }
+          acc[w][0][q] = o[0];
+          acc[w][1][q] = o[1];
+          acc[w][2][q] = o[2];
         }
         wave_sync();
       }
-      ax += (double)vx;
-      ay += (double)vy;
-      az += (double)vz;
+      for (int h = 0; h < TPL; ++h)
+        for (int a = 0; a < 3; ++a) ai[h][a] += (double)v[h][a];
     }
     __syncthreads();
-    sym_fold(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
+    sym_fold<T, W>(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
   }
-  const int i = b * kSymB + t;
-  if (i < n) {
-    part[(size_t)(3 * s + 0) * n + i] = ax;
-    part[(size_t)(3 * s + 1) * n + i] = ay;
-    part[(size_t)(3 * s + 2) * n + i] = az;
+  for (int h = 0; h < TPL; ++h) {
+    const int i = b * kSymB + t + T * h;
+    if (i < n)
+      for (int a = 0; a < 3; ++a) part[(size_t)(3 * s + a) * n + i] = ai[h][a];
   }
 }
 
@@ -753,13 +827,36 @@ int prepare(const Layout& l, const double* pos, const double* radii, const doubl
   return DEM_OK;
 }
 
+constexpr int kMaxDevices = 64;
+
+// Workgroups of `kernel` (`threads` each) the current device holds at once:
+// compute units x resident workgroups per unit, cached per device.
+int resident_slots(int* cache, const void* kernel, int threads) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  if (cache[dev] > 0) return cache[dev];
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess ||
+      per <= 0)
+    per = 1;
+  cache[dev] = cus * per;
+  return cache[dev];
+}
+
 int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, double* F,
                hipStream_t st) {
   if (use_sym(n)) {
-    const int B = sym_blocks(n), S = sym_chunks(n);
+    static int slots[kMaxDevices];
+    const int B = sym_blocks(n);
+    const int S = sym_chunks(n, resident_slots(slots, (const void*)k_force_sym<kSymTplForce>,
+                                               kSymB / kSymTplForce));
     const int ec = (1 + B / 2 + S - 1) / S;
     DEM_PROF_LAUNCH("k_force_sym", st,
-                    hipLaunchKernelGGL(k_force_sym, dim3(B, S), dim3(kSymB), 0, st,
+                    hipLaunchKernelGGL(k_force_sym<kSymTplForce>, dim3(B, S),
+                                       dim3(kSymB / kSymTplForce), 0, st,
                                        (const Rec*)l.rec, n, B, ec, choose_tile_force(), l.part,
                                        l.part_j));
     DEM_CHECK_LAUNCH("k_force_sym");
@@ -791,10 +888,14 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
   dem::CorrScalars cs{};
   const bool on = corr && corr->enabled;
   if (!on && use_sym(n)) {
-    const int B = sym_blocks(n), S = sym_chunks(n);
+    static int slots[kMaxDevices];
+    const int B = sym_blocks(n);
+    const int S = sym_chunks(n, resident_slots(slots, (const void*)k_conv_sym<kSymTplConv>,
+                                               kSymB / kSymTplConv));
     const int ec = (1 + B / 2 + S - 1) / S;
     DEM_PROF_LAUNCH("k_conv_sym", st,
-                    hipLaunchKernelGGL(k_conv_sym, dim3(B, S), dim3(kSymB), 0, st,
+                    hipLaunchKernelGGL(k_conv_sym<kSymTplConv>, dim3(B, S),
+                                       dim3(kSymB / kSymTplConv), 0, st,
                                        (const RecC*)l.recc, n, B, ec, choose_tile_conv(),
                                        l.part, l.part_j));
     DEM_CHECK_LAUNCH("k_conv_sym");
