@@ -45,7 +45,9 @@ EXEC_CONV_SYM = 24.5
 # scaled by 2.0, WRITE_SIZE by 1.0: the factors a plain 1 GiB copy reads in
 # the same program); profiles/direct_force_oseen_n65536.md
 TRAFFIC_N65536 = {"k_force_tile": (15384.9 * 2.0 + 12288.0) * 1024,
-                  "k_conv_tile": (23247.7 * 2.0 + 12288.0) * 1024}
+                  "k_conv_tile": (23247.7 * 2.0 + 12288.0) * 1024,
+                  "k_force_sym": (37302.0 * 2.0 + 211200.0) * 1024,
+                  "k_conv_sym": (80172.4 * 2.0 + 208128.0) * 1024}
 
 
 def _errors(x, ref, floor=1e-3) -> dict:
@@ -365,6 +367,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (scale for PMC byte counters)")
+    ap.add_argument("--one-sided", action="store_true",
+                    help="time the one-sided direct kernels instead of the pair-once ones")
     ap.add_argument("--legs", default="hash,tree,ensembles,adjoint",
                     help="comma list of extra measurements: hash, tree, ensembles, adjoint")
     a = ap.parse_args()
@@ -389,6 +393,8 @@ def main() -> int:
     from dem.workloads import make_system
 
     torch.cuda.set_device(0)
+    if a.one_sided:
+        _native.call("dem_direct_set_symmetric", 0)
     s = make_system(a.n, seed=0, phi=0.02)
     fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
                                       eta_water=s.eta_water, E=s.E, L=s.L,
