@@ -124,6 +124,17 @@ int dem_direct_f64_targets(const double* pos, const double* radii, int n,
                            const double* sources_forces, double* F, double* V,
                            dem_stream_t stream);
 
+/* Collision counts between volume classes over all pairs
+ * (DirectCollisionFrequency.compute,
+ * pbm/collision_frequency/direct_collision_freq.py:15-41): every pair i < j with
+ * sqrt(dx^2 + dy^2 + dz^2) <= r_i + r_j, the minimum image taken as
+ * d_a -= L rint(d_a / L) when periodic and L > 0 (float64), adds 1 to
+ * counts[bins[i]][bins[j]] and counts[bins[j]][bins[i]].  counts: n_bins x
+ * n_bins int32, overwritten.  A bin outside [0, n_bins) contributes nothing. */
+int dem_direct_collision_counts(const double* pos, const double* radii, int n, double L,
+                                int periodic, const int32_t* bins, int n_bins, int32_t* counts,
+                                dem_stream_t stream);
+
 /* Tuning knob of the direct kernels (j-split factor and FP32 tile length);
  * 0 keeps the built-in heuristic.  Returns the previous values packed as
  * (split << 16) | tile. */
@@ -234,6 +245,20 @@ int dem_octree_total_velocity(const dem_tree_info* info, double theta_sq, const 
                               double eta_const, double stokes_factor,
                               const dem_correction* corr, double* vel, void* ws,
                               size_t ws_bytes, dem_stream_t stream);
+
+/* Collision counts between volume classes by a pruned walk of the built tree
+ * (TreeCollisionFrequency._count_collisions,
+ * pbm/collision_frequency/tree_collision_freq.py:62-175).  For every particle
+ * i the stackless walk skips empty nodes and internal nodes whose bounding box
+ * is farther from x_i than r_i + max_radius (squared per-axis gaps; when
+ * periodic a gap above L/2 becomes L - gap, :143-175), and at leaves every
+ * j > i with |d|^2 <= (r_i + r_j)^2 (minimum image |d_a| > L/2 ->
+ * d_a -= sign(d_a) L, float64) adds 1 to counts[bins[i]][bins[j]] and
+ * counts[bins[j]][bins[i]].  bins: int32 class of each particle of the last
+ * build (original order, info->n entries); counts: n_bins x n_bins int32,
+ * overwritten.  A bin outside [0, n_bins) contributes nothing. */
+int dem_octree_collision_counts(const dem_tree_info* info, const int32_t* bins, int n_bins,
+                                int32_t* counts, void* ws, size_t ws_bytes, dem_stream_t stream);
 
 /* Byte offset and size inside the octree workspace of a named node / particle
  * array ("count", "first_child", "next", "leaf_start", "min", "max",

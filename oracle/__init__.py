@@ -36,6 +36,13 @@ What each function restates (reference file:line) and what pins it
 * Octree functions (``octree_*``) — ``FlatOctree`` (dem/octree/flat_tree.py).
   Pinned by the build invariants of the reference tests, theta = 0 equalling
   the direct sum, and the reference's tree-vs-direct error bands.
+* ``direct_collision_counts`` / ``OracleTree.collision_counts`` — the
+  population-balance collision counts (pbm/collision_frequency/
+  direct_collision_freq.py:15-41, tree_collision_freq.py:62-175).  Pinned by
+  the reference's own cases (tests/pbm/collision_frequency/*.py), a
+  pure-Python restatement of both loops on small systems, and brute force:
+  for open boxes the pruned walk counts exactly the brute-force pairs under
+  |d|^2 <= (r_i + r_j)^2.
 """
 
 from __future__ import annotations
@@ -99,6 +106,8 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_tree_propagate_forces.argtypes = [p, p]
         _lib.oracle_tree_forces.argtypes = [p, d, d, p, i, i]
         _lib.oracle_tree_stokes.argtypes = [p, i, d, p, d, d, p, p, i, i]
+        _lib.oracle_tree_collision_counts.argtypes = [p, p, i, p]
+        _lib.oracle_direct_collision_counts.argtypes = [p, p, i, d, i, p, i, p]
     return _lib
 
 
@@ -196,6 +205,18 @@ def brute_pairs(pos, radii, L, periodic) -> np.ndarray:
         if k <= cap:
             return out[:k].copy()
         cap = k
+
+
+def direct_collision_counts(pos, radii, L, periodic, bins, n_bins) -> np.ndarray:
+    """Class-pair collision counts over all pairs i < j (DirectCollisionFrequency,
+    pbm/collision_frequency/direct_collision_freq.py:15-41); int32 (n_bins, n_bins)."""
+    pos = _c(pos).reshape(-1, 3)
+    radii = _c(radii)
+    bins = _c(bins, np.int32)
+    out = np.zeros((n_bins, n_bins), dtype=np.int32)
+    lib().oracle_direct_collision_counts(_ptr(pos), _ptr(radii), pos.shape[0], float(L),
+                                         int(bool(periodic)), _ptr(bins), int(n_bins), _ptr(out))
+    return out
 
 
 # ---------------------------------------------------------------- merge / Euler
@@ -334,6 +355,16 @@ class OracleTree:
 
     def total_velocity(self, theta, F, eta, stokes, corr=None, targets=None) -> np.ndarray:
         return self._stokes(1, theta, F, eta, stokes, corr, targets)
+
+    def collision_counts(self, bins, n_bins) -> np.ndarray:
+        """Class-pair collision counts by the pruned walk
+        (TreeCollisionFrequency._count_collisions,
+        pbm/collision_frequency/tree_collision_freq.py:62-175); int32 (n_bins, n_bins)."""
+        bins = _c(bins, np.int32)
+        out = np.zeros((n_bins, n_bins), dtype=np.int32)
+        lib().oracle_tree_collision_counts(ctypes.byref(self.st), _ptr(bins), int(n_bins),
+                                           _ptr(out))
+        return out
 
 
 def stokes_factor(eta_oil, eta_water) -> float:

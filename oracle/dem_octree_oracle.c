@@ -443,3 +443,74 @@ void oracle_tree_stokes(const oracle_tree* t, int mode, double theta, const doub
     V[3 * i + 2] = vz;
   }
 }
+
+/* ------------------------------------------------------------------ PBM */
+
+/* Squared distance from p to a node box with the reference's periodic rule
+ * (TreeCollisionFrequency._min_dist_sq_to_bbox,
+ * pbm/collision_frequency/tree_collision_freq.py:143-175): per axis the gap
+ * outside [bmin, bmax]; when periodic, a gap above L/2 becomes L - gap. */
+static double min_dist_sq_box(const oracle_tree* t, int node, const double p[3]) {
+  double g[3];
+  for (int a = 0; a < 3; ++a) {
+    const double lo = t->bmin[3 * node + a], hi = t->bmax[3 * node + a];
+    double d = 0.0;
+    if (p[a] < lo)
+      d = lo - p[a];
+    else if (p[a] > hi)
+      d = p[a] - hi;
+    if (t->periodic && t->L > 0.0 && d > t->L * 0.5) d = t->L - d;
+    g[a] = d;
+  }
+  return g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+}
+
+/* TreeCollisionFrequency._count_collisions (tree_collision_freq.py:62-140):
+ * for every particle i the stackless walk skips empty nodes, prunes internal
+ * nodes whose box is farther than r_i + max_radius, and at leaves counts every
+ * j > i with |d|^2 <= (r_i + r_j)^2 (minimum image |d| > L/2 -> d -= sign(d) L)
+ * into counts[bin_i][bin_j] and counts[bin_j][bin_i].  counts: n_bins^2,
+ * zeroed here. */
+void oracle_tree_collision_counts(const oracle_tree* t, const int* bins, int n_bins,
+                                  int* counts) {
+  memset(counts, 0, sizeof(int) * (size_t)n_bins * (size_t)n_bins);
+  for (int i = 0; i < t->n; ++i) {
+    const double* pi = t->pos + 3 * i;
+    const double ri = t->r[i];
+    const int bi = bins[i];
+    int node = 0;
+    while (node >= 0) {
+      const int cnt = t->count[node];
+      if (cnt == 0) {
+        node = t->next[node];
+        continue;
+      }
+      if (t->first_child[node] < 0) {
+        const int ls = t->leaf_start[node];
+        for (int k = 0; k < cnt; ++k) {
+          const int j = t->leaf_indices[ls + k];
+          if (j >= 0 && j > i) {
+            const double* pj = t->pos + 3 * j;
+            const double dx = mic(pj[0] - pi[0], t->L, t->periodic);
+            const double dy = mic(pj[1] - pi[1], t->L, t->periodic);
+            const double dz = mic(pj[2] - pi[2], t->L, t->periodic);
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double c = ri + t->r[j];
+            if (d2 <= c * c) {
+              const int bj = bins[j];
+              counts[bi * n_bins + bj] += 1;
+              counts[bj * n_bins + bi] += 1;
+            }
+          }
+        }
+        node = t->next[node];
+        continue;
+      }
+      const double mc = ri + t->maxr[node];
+      if (min_dist_sq_box(t, node, pi) > mc * mc)
+        node = t->next[node];
+      else
+        node = t->first_child[node];
+    }
+  }
+}

@@ -364,3 +364,27 @@ void oracle_corr_pair_term(const oracle_corr* c, double x, double y, double z, d
 }
 
 double oracle_corr_self(const oracle_corr* c) { return (c && c->enabled) ? c->self_coeff : 0.0; }
+
+/* DirectCollisionFrequency.compute (pbm/collision_frequency/
+ * direct_collision_freq.py:15-41): every pair i < j with
+ * |d| = sqrt(dx^2 + dy^2 + dz^2) <= r_i + r_j, the minimum image taken as
+ * d -= L * rint(d / L) when periodic and L > 0, adds 1 to counts[bin_i][bin_j]
+ * and counts[bin_j][bin_i].  counts: n_bins^2, zeroed here. */
+void oracle_direct_collision_counts(const double* pos, const double* r, int n, double L,
+                                    int periodic, const int* bins, int n_bins, int* counts) {
+  memset(counts, 0, sizeof(int) * (size_t)n_bins * (size_t)n_bins);
+  const int wrap = periodic && L > 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double d[3];
+      for (int a = 0; a < 3; ++a) {
+        d[a] = pos[3 * j + a] - pos[3 * i + a];
+        if (wrap) d[a] -= L * rint(d[a] / L);
+      }
+      const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if (dist <= r[i] + r[j]) {
+        counts[bins[i] * n_bins + bins[j]] += 1;
+        counts[bins[j] * n_bins + bins[i]] += 1;
+      }
+    }
+}

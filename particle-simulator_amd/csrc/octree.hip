@@ -827,6 +827,82 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
   for (int a = 0; a < 3; ++a) V[3 * (size_t)o + a] = v[a];
 }
 
+// Collision counts between volume classes (TreeCollisionFrequency.
+// _count_collisions, pbm/collision_frequency/tree_collision_freq.py:62-140).
+// One target per lane in sorted (Morton) order, so a wave's walks are close;
+// float64 throughout, contraction off, as the reference writes the tests.
+// Counts are integers: the atomic adds commute, the result is exact.
+__device__ __forceinline__ double box_gap_sq(const double* lo, const double* hi, const double* p,
+                                             double L, int periodic) {
+#pragma clang fp contract(off)
+  double g[3];
+  for (int a = 0; a < 3; ++a) {
+    double d = 0.0;
+    if (p[a] < lo[a])
+      d = lo[a] - p[a];
+    else if (p[a] > hi[a])
+      d = p[a] - hi[a];
+    if (periodic && L > 0.0 && d > L * 0.5) d = L - d;  // :167-173
+    g[a] = d;
+  }
+  return g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+}
+
+__global__ __launch_bounds__(256) void k_walk_collisions(
+    int n, double L, int periodic, const int* __restrict__ count,
+    const int* __restrict__ first_child, const int* __restrict__ next,
+    const int* __restrict__ leaf_start, const int* __restrict__ mstart,
+    const int* __restrict__ leaf_indices, const int* __restrict__ perm,
+    const double* __restrict__ bmin, const double* __restrict__ bmax,
+    const double* __restrict__ maxr, const double* __restrict__ spos,
+    const double* __restrict__ srad, const int* __restrict__ bins, int n_bins,
+    int* __restrict__ counts) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p];
+  const double pi[3] = {spos[3 * (size_t)p], spos[3 * (size_t)p + 1], spos[3 * (size_t)p + 2]};
+  const double ri = srad[p];
+  const int bi = bins[i];
+  const bool bi_ok = bi >= 0 && bi < n_bins;
+  int node = 0;
+  while (node >= 0) {
+    const int cnt = count[node];
+    if (cnt == 0) {
+      node = next[node];
+      continue;
+    }
+    const int fc = first_child[node];
+    if (fc < 0) {
+      const int ls = leaf_start[node], m0 = mstart[node];
+      for (int k = 0; k < cnt; ++k) {
+        const int j = leaf_indices[ls + k];
+        if (j >= 0 && j > i) {
+          const double* q = spos + 3 * (size_t)(m0 + k);
+          const double dx = mic64(q[0] - pi[0], L, periodic);
+          const double dy = mic64(q[1] - pi[1], L, periodic);
+          const double dz = mic64(q[2] - pi[2], L, periodic);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          const double c = ri + srad[m0 + k];
+          if (d2 <= c * c) {
+            const int bj = bins[j];
+            if (bi_ok && bj >= 0 && bj < n_bins) {
+              atomicAdd(&counts[bi * n_bins + bj], 1);
+              atomicAdd(&counts[bj * n_bins + bi], 1);
+            }
+          }
+        }
+      }
+      node = next[node];
+      continue;
+    }
+    const double mc = ri + maxr[node];
+    node = box_gap_sq(bmin + 3 * (size_t)node, bmax + 3 * (size_t)node, pi, L, periodic) > mc * mc
+               ? next[node]
+               : fc;
+  }
+}
+
 // ---------------------------------------------------------------- host helpers
 
 inline int nb256(long long n) { return (int)((n + 255) / 256); }
@@ -1133,6 +1209,33 @@ int dem_octree_total_velocity(const dem_tree_info* info, double theta_sq, const 
                               double* vel, void* ws, size_t ws_bytes, dem_stream_t stream) {
   return stokes_common(1, info, theta_sq, forces, eta_const, stokes_factor, corr, vel, ws, ws_bytes,
                        stream);
+}
+
+int dem_octree_collision_counts(const dem_tree_info* info, const int32_t* bins, int n_bins,
+                                int32_t* counts, void* ws, size_t ws_bytes, dem_stream_t stream) {
+  TreeLayout l;
+  int rc = check_info(info, ws, ws_bytes, l);
+  if (rc) return rc;
+  if (n_bins < 1 || (long long)n_bins * n_bins > (1ll << 30))
+    return dem::set_error(DEM_ERR_INVALID, "n_bins %d out of range", n_bins);
+  if (!counts) return dem::set_error(DEM_ERR_INVALID, "null counts");
+  if (info->n > 0 && !bins) return dem::set_error(DEM_ERR_INVALID, "null bins");
+  hipStream_t st = dem::as_stream(stream);
+  const int nb2 = n_bins * n_bins;
+  hipLaunchKernelGGL(k_fill_i, dim3(nb256(nb2)), dim3(256), 0, st, counts, nb2, 0);
+  DEM_CHECK_LAUNCH("k_fill_i");
+  if (info->n == 0) return DEM_OK;
+  DEM_PROF_LAUNCH("k_walk_collisions", st,
+                  hipLaunchKernelGGL(k_walk_collisions, dim3(nb256(info->n)), dim3(256), 0, st,
+                                     info->n, info->L, info->periodic, (const int*)l.count,
+                                     (const int*)l.first_child, (const int*)l.next,
+                                     (const int*)l.leaf_start, (const int*)l.mstart,
+                                     (const int*)l.leaf_indices, (const int*)l.perm,
+                                     (const double*)l.bmin, (const double*)l.bmax,
+                                     (const double*)l.maxr, (const double*)l.spos,
+                                     (const double*)l.srad, bins, n_bins, counts));
+  DEM_CHECK_LAUNCH("k_walk_collisions");
+  return DEM_OK;
 }
 
 int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset, size_t* bytes) {

@@ -132,6 +132,8 @@ SIGNATURES: dict[str, tuple] = {
     "dem_octree_stokeslet": (_I, [_P, _D, _P, _D, _P, _P, _P, _S, _P]),
     "dem_octree_total_velocity": (_I, [_P, _D, _P, _D, _D, _P, _P, _P, _S, _P]),
     "dem_octree_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
+    "dem_octree_collision_counts": (_I, [_P, _P, _I, _P, _P, _S, _P]),
+    "dem_direct_collision_counts": (_I, [_P, _P, _I, _D, _I, _P, _I, _P, _P]),
     "dem_batch_split": (_I, [_I, _I]),
     "dem_batch_workspace_bytes": (_S, [_P, _I, _P]),
     "dem_batch_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _S, _P, ctypes.POINTER(_P)]),

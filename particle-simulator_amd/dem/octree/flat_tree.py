@@ -207,6 +207,20 @@ class FlatOctree:
     def total_velocity_device(self, F_d, eta_const: float, stokes_factor: float):
         return self._stokes_device(F_d, eta_const, stokes_factor)
 
+    def collision_counts_device(self, bins, n_bins: int):
+        """Class-pair collision counts by the pruned walk of this build
+        (``dem_octree_collision_counts``): device int32 (n_bins, n_bins)."""
+        t = _device.torch()
+        n = self.num_particles[None]
+        bins_d = bins if t.is_tensor(bins) else t.as_tensor(
+            np.ascontiguousarray(bins, dtype=np.int32), device=_device.device())
+        if int(bins_d.numel()) < n:
+            raise ValueError(f"{int(bins_d.numel())} classes for {n} droplets")
+        out = t.empty((n_bins, n_bins), dtype=t.int32, device=_device.device())
+        self._call("dem_octree_collision_counts", ctypes.byref(self._info), _device.ptr(bins_d),
+                   int(n_bins), _device.ptr(out))
+        return out
+
     # ------------------------------------------------------------------ reference API
 
     def compute_forces(self, m_const: float) -> np.ndarray:
