@@ -116,6 +116,47 @@ def hash_pass(s, reps: int = 20) -> dict:
             "note": "includes the host read-back of the pair count (drop-in semantics)"}
 
 
+def pbm_counts(reps: int = 5) -> dict:
+    """Population-balance collision counts at N = 65536 (phi = 0.02, periodic,
+    30 volume classes): the tree-pruned walk (leaf 16, after a build) and the
+    all-pairs float64 kernel."""
+    import numpy as np
+    import torch
+
+    from dem import _device, _native
+    from dem.octree.flat_tree import FlatOctree
+    from dem.workloads import make_system
+    from pbm import VolumeGrid
+    from pbm.collision_frequency import DirectCollisionFrequency
+    from pbm.volume_grid import sphere_volumes
+
+    s = make_system(65536, seed=0, phi=0.02)
+    g = VolumeGrid.from_radii_range(float(s.radii.min()), float(s.radii.max()), 30)
+    bins = _device.to_device(g.bin_indices(sphere_volumes(s.radii)).astype(np.int32), np.int32)
+    pd, rd = _device.to_device(s.positions), _device.to_device(s.radii)
+    tree = FlatOctree(theta=0.5, mpl=16, num_particles=s.n)
+    tree.build_device(pd, rd, s.L, True)
+    dc = DirectCollisionFrequency(g)
+    for _ in range(2):
+        tree.collision_counts_device(bins, 30)
+        dc.compute_device(pd, rd, bins, s.L, True)
+    torch.cuda.synchronize()
+    lib = _native.load()
+    lib.dem_profile_start()
+    for _ in range(reps):
+        M = tree.collision_counts_device(bins, 30)
+        D = dc.compute_device(pd, rd, bins, s.L, True)
+    torch.cuda.synchronize()
+    lib.dem_profile_stop()
+    k = {name: _native.profile_read(name)[0] / reps
+         for name in ("k_walk_collisions", "k_pair_collisions")}
+    return {"workload": "N=65536, phi=0.02, periodic, 30 volume classes, tree leaf 16",
+            "tree_walk_ms": k["k_walk_collisions"], "all_pairs_ms": k["k_pair_collisions"],
+            "all_pairs_per_s": s.n * (s.n - 1) / 2 / (k["k_pair_collisions"] * 1e-3),
+            "touching_pairs_tree": int(M.sum().item()) // 2,
+            "touching_pairs_all": int(D.sum().item()) // 2}
+
+
 def tree_step(reps: int = 5) -> dict:
     """Barnes-Hut step at N = 2^20, theta = 0.5, leaf size 16, periodic cube:
     build + force walk + total-velocity walk; error vs FP64 direct on 8192
@@ -369,8 +410,8 @@ def main() -> int:
                     help="also run a plain 1 GiB device copy (scale for PMC byte counters)")
     ap.add_argument("--one-sided", action="store_true",
                     help="time the one-sided direct kernels instead of the pair-once ones")
-    ap.add_argument("--legs", default="hash,tree,ensembles,adjoint",
-                    help="comma list of extra measurements: hash, tree, ensembles, adjoint")
+    ap.add_argument("--legs", default="hash,tree,pbm,ensembles,adjoint",
+                    help="comma list of extra measurements: hash, tree, pbm, ensembles, adjoint")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
@@ -473,6 +514,8 @@ def main() -> int:
         extras["hash"] = hash_pass(s)
     if "tree" in legs:
         extras["tree"] = tree_step()
+    if "pbm" in legs:
+        extras["pbm"] = pbm_counts()
     if ens_systems is not None:
         extras["ensembles"] = ensembles_leg(ens_systems, a.ens_steps, a.ens_plain_steps,
                                             tune=not a.no_autotune)
