@@ -639,60 +639,85 @@ struct WalkArgs {
   double theta_sq;
 };
 
+// Wave-coherent walks.  The 64 lanes of a wave (targets consecutive in
+// Morton order, hence close in space) traverse one node sequence together:
+// at an internal node every lane still in the walk applies its own opening
+// test; a lane that takes the node's approximation (or prunes it) records
+// next[node] as the node where it resumes, and the wave descends when any
+// lane opens the node.  A preorder walk reaches next[X] exactly after X's
+// subtree, so each lane meets the same nodes, in the same order, as its own
+// walk would: the interaction sets and summation order are the reference's
+// per-particle ones, while node records and leaf members are loaded at
+// wave-uniform addresses (scalar loads) and control flow does not diverge.
+constexpr int kActive = -2;  // lane state: in the walk; otherwise the resume node
+
+__device__ __forceinline__ int wave_node(int node) { return __builtin_amdgcn_readfirstlane(node); }
+
 // Force walk (:783-906).  Writes forces (original order) and the stored copy.
 __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* __restrict__ scp,
                                                      double m_const, double* __restrict__ F,
                                                      double* __restrict__ pforces) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= wa.n) return;
+  const int t0 = blockIdx.x * 256 + threadIdx.x;
+  const bool live = t0 < wa.n;
+  const int t = live ? t0 : wa.n - 1;
   const Scales sc = *scp;
   const RecF me = wa.srec[t];
   const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
                pz = wa.spos[3 * (size_t)t + 2];
   double ax = 0.0, ay = 0.0, az = 0.0;
+  int resume = live ? kActive : -1;
   int node = 0;
   int steps = wa.guard;  // a preorder walk visits each node at most once
   while (node >= 0 && --steps > 0) {
+    if (resume == node) resume = kActive;
+    const bool act = resume == kActive;
     const NodeW nd = wa.nw[node];
     if (nd.flags & kSkipR3) {
-      node = nd.next;
+      node = wave_node(nd.next);
       continue;
     }
     if (nd.first_child < 0) {
-      float fx = 0.f, fy = 0.f, fz = 0.f;
-      const int e = nd.mstart + nd.count;
-      for (int j = nd.mstart; j < e; ++j) {
-        if (j == t) continue;
-        const RecF sj = wa.srec[j];
-        // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
-        dem::dipole_pair<true>(dem::diff32(sj.x, me.x), dem::diff32(sj.y, me.y),
-                               dem::diff32(sj.z, me.z), me.r, sj.r, sj.r3, fx, fy, fz);
+      if (act) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        const int e = nd.mstart + nd.count;
+        for (int j = nd.mstart; j < e; ++j) {
+          const RecF sj = wa.srec[j];
+          // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
+          if (j != t)
+            dem::dipole_pair<true>(dem::diff32(sj.x, me.x), dem::diff32(sj.y, me.y),
+                                   dem::diff32(sj.z, me.z), me.r, sj.r, sj.r3, fx, fy, fz);
+        }
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
       }
-      ax += (double)fx;
-      ay += (double)fy;
-      az += (double)fz;
-      node = nd.next;
+      node = wave_node(nd.next);
       continue;
     }
-    double dx = mic64(nd.cx - px, wa.L, wa.periodic);
-    double dy = mic64(nd.cy - py, wa.L, wa.periodic);
-    double dz = mic64(nd.cz - pz, wa.L, wa.periodic);
-    double d2 = dx * dx;
-    d2 = d2 + dy * dy;
-    d2 = d2 + dz * dz;
-    if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
-      float fx = 0.f, fy = 0.f, fz = 0.f;
-      dem::dipole_monopole((float)(dx * sc.inv_U), (float)(dy * sc.inv_U),
-                           (float)(dz * sc.inv_U), nd.r3w, fx, fy, fz);
-      ax += (double)fx;
-      ay += (double)fy;
-      az += (double)fz;
-      node = nd.next;
-    } else {
-      node = nd.first_child;
+    bool open = false;
+    if (act) {
+      double dx = mic64(nd.cx - px, wa.L, wa.periodic);
+      double dy = mic64(nd.cy - py, wa.L, wa.periodic);
+      double dz = mic64(nd.cz - pz, wa.L, wa.periodic);
+      double d2 = dx * dx;
+      d2 = d2 + dy * dy;
+      d2 = d2 + dz * dz;
+      if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        dem::dipole_monopole((float)(dx * sc.inv_U), (float)(dy * sc.inv_U),
+                             (float)(dz * sc.inv_U), nd.r3w, fx, fy, fz);
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
+        resume = nd.next;
+      } else {
+        open = true;
+      }
     }
+    node = wave_node(__any(open) ? nd.first_child : nd.next);
   }
+  if (!live) return;
   if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
   const double rl = wa.srad[t] / sc.lambda;
   const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
@@ -716,23 +741,31 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
                                                      const double* __restrict__ Fself,
                                                      double* __restrict__ V) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= wa.n) return;
+  const int t0 = blockIdx.x * 256 + threadIdx.x;
+  const bool live = t0 < wa.n;
+  const int t = live ? t0 : wa.n - 1;
   const Scales sc = *scp;
   const RecF me = wa.srec[t];
   const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
                pz = wa.spos[3 * (size_t)t + 2];
   double ax = 0.0, ay = 0.0, az = 0.0;  // Oseen sum, fixed units
   double cx = 0.0, cy = 0.0, cz = 0.0;  // COMSOL sum
+  int resume = live ? kActive : -1;
   int node = 0;
   int steps = wa.guard;  // a preorder walk visits each node at most once
   while (node >= 0 && --steps > 0) {
+    if (resume == node) resume = kActive;
+    const bool act = resume == kActive;
     const NodeW nd = wa.nw[node];
     if (nd.flags & kSkipF) {
-      node = nd.next;
+      node = wave_node(nd.next);
       continue;
     }
     if (nd.first_child < 0) {
+      if (!act) {
+        node = wave_node(nd.next);
+        continue;
+      }
       float vx = 0.f, vy = 0.f, vz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
       const int e = nd.mstart + nd.count;
       for (int j = nd.mstart; j < e; ++j) {
@@ -758,7 +791,7 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
         cy += (double)uy;
         cz += (double)uz;
       }
-      node = nd.next;
+      node = wave_node(nd.next);
       continue;
     }
     // x_i - centre (:1139-1150)
@@ -768,7 +801,8 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
     double d2 = dx * dx;
     d2 = d2 + dy * dy;
     d2 = d2 + dz * dz;
-    if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
+    const bool approx = act && nd.s2 < wa.theta_sq * d2 && d2 > 0.0;
+    if (approx) {
       const NodeF nf = wa.nf[node];
       // R = centre - x_i in fixed units (sign irrelevant for the monopole)
       const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
@@ -802,11 +836,11 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
         cy += (double)uy2;
         cz += (double)uz2;
       }
-      node = nd.next;
-    } else {
-      node = nd.first_child;
+      resume = nd.next;
     }
+    node = wave_node(__any(act && !approx) ? nd.first_child : nd.next);
   }
+  if (!live) return;
   if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
   const int o = wa.perm[t];
   const double km = eta_const * sc.Fs / sc.U;
@@ -858,24 +892,30 @@ __global__ __launch_bounds__(256) void k_walk_collisions(
     const double* __restrict__ srad, const int* __restrict__ bins, int n_bins,
     int* __restrict__ counts) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= n) return;
+  const int p0 = blockIdx.x * 256 + threadIdx.x;
+  const bool live = p0 < n;
+  const int p = live ? p0 : n - 1;
   const int i = perm[p];
   const double pi[3] = {spos[3 * (size_t)p], spos[3 * (size_t)p + 1], spos[3 * (size_t)p + 2]};
   const double ri = srad[p];
   const int bi = bins[i];
   const bool bi_ok = bi >= 0 && bi < n_bins;
+  // wave-coherent walk (see k_walk_forces): lanes that prune a node resume at
+  // its next; the wave descends when any lane does
+  int resume = live ? kActive : -1;
   int node = 0;
   while (node >= 0) {
+    if (resume == node) resume = kActive;
+    const bool act = resume == kActive;
     const int cnt = count[node];
     if (cnt == 0) {
-      node = next[node];
+      node = wave_node(next[node]);
       continue;
     }
     const int fc = first_child[node];
     if (fc < 0) {
       const int ls = leaf_start[node], m0 = mstart[node];
-      for (int k = 0; k < cnt; ++k) {
+      for (int k = 0; act && k < cnt; ++k) {
         const int j = leaf_indices[ls + k];
         if (j >= 0 && j > i) {
           const double* q = spos + 3 * (size_t)(m0 + k);
@@ -893,13 +933,18 @@ __global__ __launch_bounds__(256) void k_walk_collisions(
           }
         }
       }
-      node = next[node];
+      node = wave_node(next[node]);
       continue;
     }
     const double mc = ri + maxr[node];
-    node = box_gap_sq(bmin + 3 * (size_t)node, bmax + 3 * (size_t)node, pi, L, periodic) > mc * mc
-               ? next[node]
-               : fc;
+    bool descend = false;
+    if (act) {
+      if (box_gap_sq(bmin + 3 * (size_t)node, bmax + 3 * (size_t)node, pi, L, periodic) > mc * mc)
+        resume = next[node];
+      else
+        descend = true;
+    }
+    node = wave_node(__any(descend) ? fc : next[node]);
   }
 }
 
