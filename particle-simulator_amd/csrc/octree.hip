@@ -762,34 +762,32 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       continue;
     }
     if (nd.first_child < 0) {
-      if (!act) {
-        node = wave_node(nd.next);
-        continue;
-      }
-      float vx = 0.f, vy = 0.f, vz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-      const int e = nd.mstart + nd.count;
-      for (int j = nd.mstart; j < e; ++j) {
-        if (j == t) continue;
-        const RecF sj = wa.srec[j];
-        // D = x_j - x_i (fixed units); the Oseen tensor is even in D
-        const float dx = dem::diff32(sj.x, me.x);
-        const float dy = dem::diff32(sj.y, me.y);
-        const float dz = dem::diff32(sj.z, me.z);
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r)) {  // contact mask >= (:1055)
+      if (act) {
+        float vx = 0.f, vy = 0.f, vz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        const int e = nd.mstart + nd.count;
+        for (int j = nd.mstart; j < e; ++j) {
+          // wave-uniform loads first (scalar), then the lane's own tests
+          const RecF sj = wa.srec[j];
           const float Fx = wa.sF[3 * (size_t)j], Fy = wa.sF[3 * (size_t)j + 1],
                       Fz = wa.sF[3 * (size_t)j + 2];
-          dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, true, vx, vy, vz);
-          if (CORR) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+          // D = x_j - x_i (fixed units); the Oseen tensor is even in D
+          const float dx = dem::diff32(sj.x, me.x);
+          const float dy = dem::diff32(sj.y, me.y);
+          const float dz = dem::diff32(sj.z, me.z);
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (j != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r)) {  // >= (:1055)
+            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, true, vx, vy, vz);
+            if (CORR) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+          }
         }
-      }
-      ax += (double)vx;
-      ay += (double)vy;
-      az += (double)vz;
-      if (CORR) {
-        cx += (double)ux;
-        cy += (double)uy;
-        cz += (double)uz;
+        ax += (double)vx;
+        ay += (double)vy;
+        az += (double)vz;
+        if (CORR) {
+          cx += (double)ux;
+          cy += (double)uy;
+          cz += (double)uz;
+        }
       }
       node = wave_node(nd.next);
       continue;
@@ -802,8 +800,8 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
     d2 = d2 + dy * dy;
     d2 = d2 + dz * dz;
     const bool approx = act && nd.s2 < wa.theta_sq * d2 && d2 > 0.0;
+    const NodeF nf = wa.nf[node];  // wave-uniform: loaded for every lane
     if (approx) {
-      const NodeF nf = wa.nf[node];
       // R = centre - x_i in fixed units (sign irrelevant for the monopole)
       const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
                   Rz = (float)(-dz * sc.inv_U);
