@@ -287,9 +287,11 @@ def ensembles_leg(systems, steps: int, plain_steps: int, tune: bool = True) -> d
                     "ms_per_global_step": tb / steps * 1e3, "ensemble_steps": adv_b,
                     "merges": merges_b, "merge_rounds": rounds_b,
                     "phase_ms_per_global_step": phases,
-                    "tuning": {"block": best.block, "jlen": best.jlen},
+                    "tuning": {"block": best.block, "jlen": best.jlen,
+                               "pair_once": bool(best.pair_once)},
                     "pair_interactions_per_s": pairs * adv_b / len(systems) / tb,
-                    "autotune_ms_per_step": ({f"{b}x{j}": v for (b, j), v in times.items()}
+                    "autotune_ms_per_step": ({(f"{b}x{j}" if b != "pair_once" else f"pair_once_{j}"): v
+                                              for (b, j), v in times.items()}
                                              if times else None)},
         "plain": {"ensemble_steps_per_s": rate_p, "global_steps": plain_steps,
                   "ms_per_global_step": tp / plain_steps * 1e3, "ensemble_steps": adv_p,

@@ -141,11 +141,15 @@ int dem_direct_collision_counts(const double* pos, const double* radii, int n, d
 int dem_direct_set_tuning(int split, int tile);
 
 /* Pair-once (Newton's third law) evaluation of the periodic force sum and of
- * the Oseen sum without the COMSOL correction, for 16384 <= n <= 131072 and
- * no explicit split: each unordered pair is evaluated once and credited to
- * both droplets (same accuracy contract, different rounding than the
- * one-sided kernels).  on = 0 keeps the one-sided kernels.  Returns the
- * previous setting.  No counterpart in the reference (a tuning switch). */
+ * the Oseen sum without the COMSOL correction: each unordered pair is
+ * evaluated once and credited to both droplets (same accuracy contract,
+ * different rounding than the one-sided kernels).  mode 0: one-sided kernels
+ * only; 1 (default): pair-once for 16384 <= n <= 131072 when no explicit split
+ * is set; 2: pair-once for every n, the split of dem_direct_set_tuning (when
+ * > 0) giving the number of chunks of each block's partner list (the batched
+ * pair-once sums equal these bit for bit with the split of
+ * dem_batch_pair_once_split).  Workspace sizes depend on the mode.  Returns
+ * the previous mode.  No counterpart in the reference (a tuning switch). */
 int dem_direct_set_symmetric(int on);
 
 /* ------------------------------------------------------------------------ */
@@ -313,10 +317,17 @@ typedef struct dem_batch_tuning {
   int32_t conv_tile;  /* same for the Oseen sum; 0: 16                          */
   int32_t contacts;   /* 1: contact pass; 0: every active system advances (the
                          solver without a collision detector)                   */
-  int32_t reserved;
+  int32_t pair_once;  /* 1: pair-once sums (Newton's third law, blocks of 256;
+                         jlen / 256 partner blocks per work item; block unused);
+                         0: one-sided sums                                      */
 } dem_batch_tuning;
 
 typedef struct dem_batch* dem_batch_t; /* opaque host handle */
+
+/* Pair-once mode: chunks of each 256-droplet block's partner list of a system
+ * of n droplets for a given jlen (the `split` to pass to dem_direct_set_tuning
+ * with dem_direct_set_symmetric(2) for bit-identical single-system sums). */
+int dem_batch_pair_once_split(int n, int jlen);
 
 /* Source split of a system of n droplets for a given jlen (also the `split`
  * to pass to dem_direct_set_tuning for bit-identical single-system sums). */

@@ -51,6 +51,7 @@
 
 #include "dem_common.h"
 #include "direct_core.h"
+#include "direct_sym.h"
 #include "euler_core.h"
 #include "hash_core.h"
 #include "profile.h"
@@ -90,6 +91,19 @@ inline int split_bound(int cap, int jlen) {
   return (int)(s > 64 ? 64 : (s < 1 ? 1 : s));
 }
 
+// Pair-once mode: chunks S of each block's partner list for about jlen / 256
+// partner blocks per chunk (the split to pass to dem_direct_set_tuning in
+// pair-once mode 2 for bit-identical single-system sums); entries per chunk
+// ceil(E / S), E = 1 + B / 2.  Never decreases with n.
+inline int pair_once_split(int n, int jlen) {
+  const int B = (n + dem::kSymB - 1) / dem::kSymB;
+  const int emax = dem::sym_entries_max(B > 0 ? B : 1);
+  const int ect = std::max(1, jlen / dem::kSymB);
+  int S = (emax + ect - 1) / ect;
+  S = std::min(S, std::min(emax, 64));
+  return S < 1 ? 1 : S;
+}
+
 inline int next_pow2(int v) {
   int p = 1;
   while (p < v) p <<= 1;
@@ -103,8 +117,12 @@ struct BEns {
   long long part_off;   // doubles into the partial-sum array
   long long cnt_off;    // ints into the bucket counts (T per system)
   long long start_off;  // ints into the bucket starts (T + 1 per system)
-  int off, n, cap, S;
-  int jchunk, steps, T, pad;
+  long long partj_off;  // doubles into the reversed-pair partials (pair-once mode)
+  int off;              // first droplet in the caller's arrays
+  int roff;             // first record (regions padded to 256 records)
+  int n, cap, S;        // S: source splits, or chunks of a block's partner list
+  int jchunk;           // sources per split, or partner entries per chunk
+  int steps, T;
   double L, dt, sf, m_const, eta_const;
 };
 
@@ -134,6 +152,7 @@ struct BLayout {
   Rec* rec;
   RecC* recc;
   double* part;
+  double* part_j;  // pair-once reversed-pair partials [sum B (B/2) 3 256]
   int* cnt;      // [sum T]
   int* start;    // [sum T + 1]
   int* bucket;   // [sum cap]
@@ -158,23 +177,37 @@ struct BLayout {
 
 struct Plan {
   long long items_cap = 0, rows_cap = 0, part_total = 0, cnt_total = 0, start_total = 0;
-  long long pair_total = 0, n_total = 0, cap_total = 0;
-  std::vector<long long> part_off, cnt_off, start_off;
+  long long pair_total = 0, n_total = 0, cap_total = 0, rec_total = 0, partj_total = 0;
+  std::vector<long long> part_off, cnt_off, start_off, partj_off;
+  std::vector<int> rec_off;
 };
 
-Plan make_plan(const dem_batch_ensemble* ens, int n_ens, int block, int jlen) {
+Plan make_plan(const dem_batch_ensemble* ens, int n_ens, int block, int jlen, int pair_once) {
   Plan p;
   p.part_off.resize(n_ens);
   p.cnt_off.resize(n_ens);
   p.start_off.resize(n_ens);
+  p.partj_off.resize(n_ens);
+  p.rec_off.resize(n_ens);
   for (int e = 0; e < n_ens; ++e) {
     const int cap = ens[e].capacity;
+    const int Bcap = std::max(1, (cap + dem::kSymB - 1) / dem::kSymB);
     // a system only shrinks: bound its items and partials over every n <= cap
-    const int S = split_bound(cap, jlen);
-    p.items_cap += (long long)((cap + block - 1) / block) * S;
-    p.rows_cap += (cap + kRow - 1) / kRow;
     p.part_off[e] = p.part_total;
-    p.part_total += (long long)S * 6 * cap;
+    p.partj_off[e] = p.partj_total;
+    if (pair_once) {
+      const int S = pair_once_split(cap, jlen);
+      p.items_cap += (long long)Bcap * S;
+      p.part_total += (long long)S * 3 * cap;
+      p.partj_total += (long long)std::max(1, Bcap * (Bcap / 2)) * 3 * dem::kSymB;
+    } else {
+      const int S = split_bound(cap, jlen);
+      p.items_cap += (long long)((cap + block - 1) / block) * S;
+      p.part_total += (long long)S * 6 * cap;
+    }
+    p.rows_cap += (cap + kRow - 1) / kRow;
+    p.rec_off[e] = (int)p.rec_total;
+    p.rec_total += (long long)Bcap * dem::kSymB;
     const int T = next_pow2(2 * (cap > 0 ? cap : 1));
     p.cnt_off[e] = p.cnt_total;
     p.cnt_total += T;
@@ -199,9 +232,10 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   l.head = c.take<BHead>(1);
   l.items = c.take<int4>(at_least1(p.items_cap));
   l.rows = c.take<int2>(at_least1(p.rows_cap));
-  l.rec = c.take<Rec>(at_least1(p.n_total));
-  l.recc = c.take<RecC>(at_least1(p.n_total));
+  l.rec = c.take<Rec>(at_least1(p.rec_total));
+  l.recc = c.take<RecC>(at_least1(p.rec_total));
   l.part = c.take<double>(at_least1(p.part_total));
+  l.part_j = c.take<double>(at_least1(p.partj_total));
   l.cnt = c.take<int>(at_least1(p.cnt_total));
   l.start = c.take<int>(at_least1(p.start_total));
   l.bucket = c.take<int>(at_least1(p.n_total));
@@ -281,13 +315,18 @@ __global__ __launch_bounds__(kRow) void k_b_pack(const BEns* __restrict__ ens,
   const int2 row = rows[blockIdx.x];
   const BEns& E = ens[row.x];
   const int i = row.y + threadIdx.x;
-  if (i >= E.n) return;
+  const size_t q = (size_t)E.roff + i;
+  if (i >= E.n) {  // the last block's padding: no radius, no force
+    rec[q] = Rec{};
+    recc[q] = RecC{};
+    return;
+  }
   const size_t g = (size_t)E.off + i;
   Rec o;
   RecC c;
   dem::pack_one(pos + 3 * g, radii[g], nullptr, dev[row.x].sc, o, c);
-  rec[g] = o;
-  recc[g] = c;
+  rec[q] = o;
+  recc[q] = c;
 }
 
 // Merges computed by the caller, one workgroup per listed system.  Stage
@@ -656,7 +695,7 @@ __global__ __launch_bounds__(BW) void k_b_force(const BEns* __restrict__ ens,
   if (!go[it.x]) return;
   const BEns& E = ens[it.x];
   const int n = E.n;
-  const Rec* r = rec + E.off;
+  const Rec* r = rec + E.roff;
   const int i = it.y + threadIdx.x;
   const int ii = i < n ? i : n - 1;
   const int s = it.z;
@@ -672,12 +711,15 @@ __global__ __launch_bounds__(BW) void k_b_force(const BEns* __restrict__ ens,
   }
 }
 
+// SYM: pair-once partials (chunk sums + reversed-pair sums), else source splits.
+template <bool SYM>
 __global__ __launch_bounds__(kRow) void k_b_force_reduce(const BEns* __restrict__ ens,
                                                          const BDev* __restrict__ dev,
                                                          const int2* __restrict__ rows,
                                                          const BHead* __restrict__ head,
                                                          const int* __restrict__ go,
                                                          const double* __restrict__ part,
+                                                         const double* __restrict__ part_j,
                                                          const double* __restrict__ radii,
                                                          double* __restrict__ F,
                                                          RecC* __restrict__ recc) {
@@ -690,16 +732,21 @@ __global__ __launch_bounds__(kRow) void k_b_force_reduce(const BEns* __restrict_
   if (i >= n) return;
   const double* pp = part + E.part_off;
   double sum[3] = {0.0, 0.0, 0.0};
-  for (int s = 0; s < E.S; ++s)
-    for (int a = 0; a < 3; ++a) sum[a] += pp[(size_t)(6 * s + a) * n + i];
+  if (SYM) {
+    dem::sym_sum(pp, E.S, n, part_j + E.partj_off, (n + dem::kSymB - 1) / dem::kSymB, i, sum);
+  } else {
+    for (int s = 0; s < E.S; ++s)
+      for (int a = 0; a < 3; ++a) sum[a] += pp[(size_t)(6 * s + a) * n + i];
+  }
   const size_t g = (size_t)E.off + i;
   double Fi[3];
   float fr[3];
   dem::force_finish(sum, radii[g], dev[row.x].sc, E.m_const, Fi, fr);
   for (int a = 0; a < 3; ++a) F[3 * g + a] = Fi[a];
-  recc[g].fx = fr[0];
-  recc[g].fy = fr[1];
-  recc[g].fz = fr[2];
+  const size_t q = (size_t)E.roff + i;
+  recc[q].fx = fr[0];
+  recc[q].fy = fr[1];
+  recc[q].fz = fr[2];
 }
 
 template <int BW>
@@ -714,7 +761,7 @@ __global__ __launch_bounds__(BW) void k_b_conv(const BEns* __restrict__ ens,
   if (!go[it.x]) return;
   const BEns& E = ens[it.x];
   const int n = E.n;
-  const RecC* r = recc + E.off;
+  const RecC* r = recc + E.roff;
   const int i = it.y + threadIdx.x;
   const int ii = i < n ? i : n - 1;
   const int s = it.z;
@@ -732,10 +779,12 @@ __global__ __launch_bounds__(BW) void k_b_conv(const BEns* __restrict__ ens,
   }
 }
 
+template <bool SYM>
 __global__ __launch_bounds__(kRow) void k_b_conv_euler(
     const BEns* __restrict__ ens, const BDev* __restrict__ dev, const int2* __restrict__ rows,
     const BHead* __restrict__ head, const int* __restrict__ go, const double* __restrict__ part,
-    const double* __restrict__ radii, const double* __restrict__ F, double* __restrict__ V,
+    const double* __restrict__ part_j, const double* __restrict__ radii,
+    const double* __restrict__ F, double* __restrict__ V,
     double* __restrict__ pos, Rec* __restrict__ rec, RecC* __restrict__ recc,
     int* __restrict__ steps_done, double* __restrict__ time) {
   if ((int)blockIdx.x >= head->n_rows) return;
@@ -754,8 +803,12 @@ __global__ __launch_bounds__(kRow) void k_b_conv_euler(
   const double* pp = part + E.part_off;
   double m[3] = {0.0, 0.0, 0.0};
   const double c[3] = {0.0, 0.0, 0.0};
-  for (int s = 0; s < E.S; ++s)
-    for (int a = 0; a < 3; ++a) m[a] += pp[(size_t)(6 * s + a) * n + i];
+  if (SYM) {
+    dem::sym_sum(pp, E.S, n, part_j + E.partj_off, (n + dem::kSymB - 1) / dem::kSymB, i, m);
+  } else {
+    for (int s = 0; s < E.S; ++s)
+      for (int a = 0; a < 3; ++a) m[a] += pp[(size_t)(6 * s + a) * n + i];
+  }
   const size_t g = (size_t)E.off + i;
   const dem::CorrScalars cs{};
   double Vi[3];
@@ -770,8 +823,42 @@ __global__ __launch_bounds__(kRow) void k_b_conv_euler(
   Rec o;
   RecC cc;
   dem::pack_one(x, ri, nullptr, sc, o, cc);
-  rec[g] = o;
-  recc[g] = cc;
+  const size_t q = (size_t)E.roff + i;
+  rec[q] = o;
+  recc[q] = cc;
+}
+
+// Pair-once sums: item (system, target block, chunk) runs one workgroup of
+// the single-system pair-once bodies on the system's padded records, so each
+// system's sums equal dem_direct_* in pair-once mode 2 with the same chunks.
+template <int TPL>
+__global__ __launch_bounds__(dem::kSymB / TPL) void k_b_force_sym(
+    const BEns* __restrict__ ens, const int4* __restrict__ items, const BHead* __restrict__ head,
+    const int* __restrict__ go, const Rec* __restrict__ rec, int tile, double* __restrict__ part,
+    double* __restrict__ part_j) {
+  if ((int)blockIdx.x >= head->n_items) return;
+  const int4 it = items[blockIdx.x];
+  if (!go[it.x]) return;
+  const BEns& E = ens[it.x];
+  const int n = E.n, B = (n + dem::kSymB - 1) / dem::kSymB;
+  const int e0 = it.z * E.jchunk, e1 = min(1 + dem::sym_partners(it.y, B), e0 + E.jchunk);
+  dem::force_sym_block<TPL>(rec + E.roff, n, B, it.y, it.z, e0, e1, tile, part + E.part_off,
+                            part_j + E.partj_off);
+}
+
+template <int TPL>
+__global__ __launch_bounds__(dem::kSymB / TPL) void k_b_conv_sym(
+    const BEns* __restrict__ ens, const int4* __restrict__ items, const BHead* __restrict__ head,
+    const int* __restrict__ go, const RecC* __restrict__ recc, int tile, double* __restrict__ part,
+    double* __restrict__ part_j) {
+  if ((int)blockIdx.x >= head->n_items) return;
+  const int4 it = items[blockIdx.x];
+  if (!go[it.x]) return;
+  const BEns& E = ens[it.x];
+  const int n = E.n, B = (n + dem::kSymB - 1) / dem::kSymB;
+  const int e0 = it.z * E.jchunk, e1 = min(1 + dem::sym_partners(it.y, B), e0 + E.jchunk);
+  dem::conv_sym_block<TPL>(recc + E.roff, n, B, it.y, it.z, e0, e1, tile, part + E.part_off,
+                           part_j + E.partj_off);
 }
 
 }  // namespace
@@ -804,6 +891,8 @@ int check_tuning(const dem_batch_tuning* t) {
     return dem::set_error(DEM_ERR_INVALID, "tuning.block must be 64, 128 or 256 (got %d)", t->block);
   if (t->force_tile < 0 || t->conv_tile < 0)
     return dem::set_error(DEM_ERR_INVALID, "tiles must be >= 0");
+  if (t->pair_once != 0 && t->pair_once != 1)
+    return dem::set_error(DEM_ERR_INVALID, "tuning.pair_once must be 0 or 1");
   return DEM_OK;
 }
 
@@ -845,11 +934,19 @@ int upload_tables(dem_batch* b, hipStream_t st) {
     E.part_off = b->plan.part_off[e];
     E.cnt_off = b->plan.cnt_off[e];
     E.start_off = b->plan.start_off[e];
+    E.partj_off = b->plan.partj_off[e];
     E.off = d.offset;
+    E.roff = b->plan.rec_off[e];
     E.n = d.n;
     E.cap = d.capacity;
-    E.S = batch_split(d.n, b->tune.jlen);
-    E.jchunk = d.n > 0 ? (d.n + E.S - 1) / E.S : 0;
+    if (b->tune.pair_once) {
+      const int B = (d.n + dem::kSymB - 1) / dem::kSymB;
+      E.S = pair_once_split(d.n, b->tune.jlen);
+      E.jchunk = (dem::sym_entries_max(B > 0 ? B : 1) + E.S - 1) / E.S;
+    } else {
+      E.S = batch_split(d.n, b->tune.jlen);
+      E.jchunk = d.n > 0 ? (d.n + E.S - 1) / E.S : 0;
+    }
     E.steps = d.steps;
     E.T = next_pow2(2 * (d.capacity > 0 ? d.capacity : 1));
     E.L = d.L;
@@ -861,9 +958,15 @@ int upload_tables(dem_batch* b, hipStream_t st) {
   // items of the largest systems first (their items take longest)
   for (int e : b->order) {
     const BEns& E = b->ens_h[e];
-    for (int i0 = 0; i0 < E.n; i0 += bw)
-      for (int s = 0; s < E.S; ++s)
-        if (s * E.jchunk < E.n) b->items_h.push_back(make_int4(e, i0, s, 0));
+    if (b->tune.pair_once) {
+      // every chunk of every block: the reductions add all E.S chunk sums
+      for (int blk = 0; blk * dem::kSymB < E.n; ++blk)
+        for (int s = 0; s < E.S; ++s) b->items_h.push_back(make_int4(e, blk, s, 0));
+    } else {
+      for (int i0 = 0; i0 < E.n; i0 += bw)
+        for (int s = 0; s < E.S; ++s)
+          if (s * E.jchunk < E.n) b->items_h.push_back(make_int4(e, i0, s, 0));
+    }
   }
   for (int e = 0; e < b->n_ens; ++e)
     for (int i0 = 0; i0 < b->ens_h[e].n; i0 += kRow) b->rows_h.push_back(make_int2(e, i0));
@@ -977,10 +1080,10 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
     force();
   }
   DEM_CHECK_LAUNCH("k_b_force");
-  hipLaunchKernelGGL(k_b_force_reduce, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+  hipLaunchKernelGGL(k_b_force_reduce<false>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)b->radii, b->F,
-                     l.recc);
+                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const double*)b->radii, b->F, l.recc);
   DEM_CHECK_LAUNCH("k_b_force_reduce");
   if (prof) {
     DEM_PROF_LAUNCH("k_b_conv", st, conv());
@@ -988,15 +1091,62 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
     conv();
   }
   DEM_CHECK_LAUNCH("k_b_conv");
-  hipLaunchKernelGGL(k_b_conv_euler, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+  hipLaunchKernelGGL(k_b_conv_euler<false>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)b->radii,
-                     (const double*)b->F, b->V, b->pos, l.rec, l.recc, l.steps_done, l.time);
+                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const double*)b->radii, (const double*)b->F, b->V, b->pos, l.rec, l.recc,
+                     l.steps_done, l.time);
+  DEM_CHECK_LAUNCH("k_b_conv_euler");
+  return DEM_OK;
+}
+
+int enqueue_advance_sym(dem_batch* b, hipStream_t st, bool prof) {
+  const BLayout& l = b->l;
+  const unsigned gi = (unsigned)std::max<long long>(b->plan.items_cap, 1);
+  const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
+  constexpr int TF = dem::kSymTplForce, TC = dem::kSymTplConv;
+  hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
+                     (const BEns*)l.ens, (const BHead*)l.head, (const int*)l.steps_done,
+                     (const int*)l.pair_count, l.stepped, l.go);
+  DEM_CHECK_LAUNCH("k_b_go");
+  auto force = [&] {
+    hipLaunchKernelGGL(k_b_force_sym<TF>, dim3(gi), dim3(dem::kSymB / TF), 0, st,
+                       (const BEns*)l.ens, (const int4*)l.items, (const BHead*)l.head,
+                       (const int*)l.go, (const Rec*)l.rec, tile_force(b), l.part, l.part_j);
+  };
+  auto conv = [&] {
+    hipLaunchKernelGGL(k_b_conv_sym<TC>, dim3(gi), dim3(dem::kSymB / TC), 0, st,
+                       (const BEns*)l.ens, (const int4*)l.items, (const BHead*)l.head,
+                       (const int*)l.go, (const RecC*)l.recc, tile_conv(b), l.part, l.part_j);
+  };
+  if (prof) {
+    DEM_PROF_LAUNCH("k_b_force", st, force());
+  } else {
+    force();
+  }
+  DEM_CHECK_LAUNCH("k_b_force_sym");
+  hipLaunchKernelGGL(k_b_force_reduce<true>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const double*)b->radii, b->F, l.recc);
+  DEM_CHECK_LAUNCH("k_b_force_reduce");
+  if (prof) {
+    DEM_PROF_LAUNCH("k_b_conv", st, conv());
+  } else {
+    conv();
+  }
+  DEM_CHECK_LAUNCH("k_b_conv_sym");
+  hipLaunchKernelGGL(k_b_conv_euler<true>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const double*)b->radii, (const double*)b->F, b->V, b->pos, l.rec, l.recc,
+                     l.steps_done, l.time);
   DEM_CHECK_LAUNCH("k_b_conv_euler");
   return DEM_OK;
 }
 
 int enqueue_advance(dem_batch* b, hipStream_t st, bool prof) {
+  if (b->tune.pair_once) return enqueue_advance_sym(b, st, prof);
   switch (b->tune.block) {
     case 64: return enqueue_advance_bw<64>(b, st, prof);
     case 128: return enqueue_advance_bw<128>(b, st, prof);
@@ -1047,10 +1197,12 @@ extern "C" {
 
 int dem_batch_split(int n, int jlen) { return batch_split(n, jlen); }
 
+int dem_batch_pair_once_split(int n, int jlen) { return pair_once_split(n, jlen); }
+
 size_t dem_batch_workspace_bytes(const dem_batch_ensemble* ens, int n_ens,
                                  const dem_batch_tuning* tuning) {
   if (check_ensembles(ens, n_ens) || check_tuning(tuning)) return 0;
-  Plan p = make_plan(ens, n_ens, tuning->block, tuning->jlen);
+  Plan p = make_plan(ens, n_ens, tuning->block, tuning->jlen, tuning->pair_once);
   return carve(nullptr, n_ens, p).bytes;
 }
 
@@ -1064,7 +1216,7 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   if ((rc = check_tuning(tuning))) return rc;
   if (!pos || !radii || !forces || !velocities)
     return dem::set_error(DEM_ERR_INVALID, "null state array");
-  Plan p = make_plan(ens, n_ens, tuning->block, tuning->jlen);
+  Plan p = make_plan(ens, n_ens, tuning->block, tuning->jlen, tuning->pair_once);
   if (p.items_cap > 0x7fffffffLL || p.rows_cap > 0x7fffffffLL)
     return dem::set_error(DEM_ERR_CAPACITY, "too many work items");
   BLayout l = carve(ws, n_ens, p);

@@ -31,8 +31,11 @@
 //     with the tile length, not with N).
 #include <math.h>
 
+#include <algorithm>
+
 #include "dem_common.h"
 #include "direct_core.h"
+#include "direct_sym.h"
 #include "profile.h"
 
 namespace {
@@ -45,27 +48,29 @@ constexpr int WG = 256;
 
 int g_split = 0;  // 0: heuristic
 int g_tile = 0;   // 0: heuristic
-int g_sym = 1;    // 1: symmetric kernels where they apply, 0: one-sided only
+int g_sym = 1;    // 0: one-sided only; 1: pair-once for kSymMin <= n <= kSymMax without an
+                  // explicit split; 2: pair-once for every n, g_split giving the chunk count
 
 // Symmetric (pair-once) evaluation: blocks of kSymB droplets; block b pairs
 // with the next sym_partners(b, B) blocks cyclically, which covers every
 // unordered block pair exactly once.  Its float64 partials of the reversed
 // pairs take B * B/2 * 256 * 3 doubles, so it runs for mid-size N only.
-constexpr int kSymB = 256;
-constexpr int kSymLds = kSymB + 64;  // sources lane + k, k < 256, without a wrap
-constexpr int kSymTplForce = 2;  // targets per thread, force / Oseen kernels
-constexpr int kSymTplConv = 2;
+using dem::kSymB;
+using dem::kSymTplConv;
+using dem::kSymTplForce;
+using dem::sym_partners;
 constexpr int kSymMin = 16384;
 constexpr int kSymMax = 131072;
 
 inline int sym_blocks(int n) { return (n + kSymB - 1) / kSymB; }
-__host__ __device__ inline int sym_partners(int b, int B) {
-  if (B & 1) return (B - 1) / 2;
-  return b < B / 2 ? B / 2 : B / 2 - 1;
-}
 inline bool sym_fits(int n) { return n >= kSymMin && n <= kSymMax; }
 // An explicit split (dem_direct_set_tuning) selects the one-sided kernels.
-inline bool use_sym(int n) { return g_sym != 0 && g_split == 0 && sym_fits(n); }
+inline bool use_sym(int n) {
+  if (g_sym == 2) return n >= 1;
+  return g_sym == 1 && g_split == 0 && sym_fits(n);
+}
+// Workspaces hold the pair-once partials whenever the kernels may run.
+inline bool sym_alloc(int n) { return (g_sym == 2 && n >= 1) || sym_fits(n); }
 constexpr int kSymMaxChunks = 64;
 // grid.y for `slots` co-resident workgroups: one round of B x S workgroups,
 // at most one chunk per partner entry.
@@ -119,20 +124,20 @@ Layout carve(void* ws, int n) {
   dem::Carver c(ws);
   Layout l;
   const int nn = n > 0 ? n : 1;
-  l.npad = sym_fits(n) ? sym_blocks(n) * kSymB : nn;
+  l.npad = sym_alloc(n) ? sym_blocks(n) * kSymB : nn;
   l.sc = c.take<DevScales>(1);
   l.bbox = c.take<double>(kBBoxBlocks * 6);
   l.rec = c.take<Rec>((size_t)l.npad);
   l.recc = c.take<RecC>((size_t)l.npad);
   size_t parts = (size_t)split_max(n) * 6 * (size_t)nn;
-  if (sym_fits(n)) {
+  if (sym_alloc(n)) {
     const size_t ps = (size_t)kSymMaxChunks * 3 * (size_t)nn;
     if (ps > parts) parts = ps;
   }
   l.part = c.take<double>(parts);
-  if (sym_fits(n)) {
+  if (sym_alloc(n)) {
     const size_t B = (size_t)sym_blocks(n);
-    l.part_j = c.take<double>(B * (B / 2) * 3 * kSymB);
+    l.part_j = c.take<double>(std::max<size_t>(B * (B / 2), 1) * 3 * kSymB);
   } else {
     l.part_j = nullptr;
   }
@@ -254,195 +259,14 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
 
 // ---------------------------------------------------------------- phase 1, pair-once
 
-// Block b (256 targets; 256 / TPL threads, thread t holding targets
-// t + (256 / TPL) h, h < TPL) works through entries e of its partner list in
-// [e0, e1): e = 0 is b itself (one-sided, every ordered pair of the block),
-// e >= 1 is block c = (b + e) mod B, whose pairs with b are evaluated once
-// and credited to both sides.  Sources sit in LDS as arrays; lane l at step
-// k takes source slot l + k (the first 64 sources are repeated after 256, so
-// no wrap) for each of its targets.  The reversed pairs' sums are added into
-// the wave's own LDS accumulator of that slot by a plain read-add-write: one
-// instruction's lanes touch distinct slots, and steps run in groups
-// k = kk + 64u (u = 0..3) whose four slot windows [k, k + 63] are disjoint, so
-// only consecutive groups share slots and wave_sync() orders them.  Per
-// (b, c) the waves' accumulators are summed in fixed order into float64
-// part_j[b][e-1][3][256]; the target side keeps float64 accumulators as the
-// one-sided kernel does.
-
-// Orders one wave's LDS accesses across its lanes: the compiler keeps program
-// order across it, and the LDS executes a wave's instructions in order.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Source slot s (0..255) and, for s < 64, its repeat at s + 256.
-template <typename T>
-__device__ __forceinline__ void sym_stage(T* a, T v, int s) {
-  a[s] = v;
-  if (s < 64) a[s + kSymB] = v;
-}
-
-__device__ __forceinline__ void sym_zero(float (*acc)[kSymLds], int lane) {
-  for (int q = lane; q < kSymLds; q += 64) {
-    acc[0][q] = 0.f;
-    acc[1][q] = 0.f;
-    acc[2][q] = 0.f;
-  }
-}
-
-// Reversed-pair sums of the partner block's sources t + T h, all W waves.
-template <int T, int W>
-__device__ __forceinline__ void sym_fold(const float (*acc)[3][kSymLds], int t,
-                                         double* __restrict__ dst) {
-  for (int slot = t; slot < kSymB; slot += T) {
-    for (int a = 0; a < 3; ++a) {
-      double v = 0.0;
-      for (int w = 0; w < W; ++w) {
-        v += (double)acc[w][a][slot];
-        if (slot < 64) v += (double)acc[w][a][slot + kSymB];
-      }
-      dst[a * kSymB + slot] = v;
-    }
-  }
-}
-
-// Dipole pair of one target with a source (radii rif, rjf in fixed units, rj3
-// in work units): adds to the target's FP32 sums and returns g (the pair's
-// force per r_i^3 r_j^3, odd in d) for the reversed side.  The contact mask
-// |d| > r_i + r_j is h = |d|^2 - (r_i + r_j)^2 > 0 in fixed units, where a
-// positive h is at least 2^20 (an ulp of |d|^2 there) and so exceeds 1/|d|
-// in work units: med3(h, 0, 1/|d|) is 1/|d| for separated pairs and 0 for
-// touching ones and for j == i.
-__device__ __forceinline__ void force_pair_sym(float dx, float dy, float dz, float rif, float rjf,
-                                               float rj3, float* f, float* g) {
-  const float dz2 = dz * dz;
-  const float dxy2 = dx * dx + dy * dy;
-  const float r2f = dxy2 + dz2;
-  const float rsf = rif + rjf;
-  const float h = fmaf(-rsf, rsf, r2f);
-  const float ir = __builtin_amdgcn_fmed3f(h, 0.f, __builtin_amdgcn_rsqf(r2f * dem::kSigma2f));
-  const float ir2 = ir * ir;
-  const float sm = (ir2 * ir2) * (ir2 * ir);
-  const float A = 4.f * dz2 - dxy2;
-  const float Bz = A - 2.f * r2f;  // 2 dz^2 - 3 (dx^2 + dy^2)
-  const float cxy = sm * A, cz = sm * Bz;
-  g[0] = cxy * dx;
-  g[1] = cxy * dy;
-  g[2] = cz * dz;
-  f[0] += rj3 * g[0];
-  f[1] += rj3 * g[1];
-  f[2] += rj3 * g[2];
-}
-
 template <int TPL>
 __global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec, int n,
                                                            int B, int ec, int tile,
                                                            double* __restrict__ part,
                                                            double* __restrict__ part_j) {
-  constexpr int T = kSymB / TPL, W = T / 64;
-  __shared__ uint32_t sx[kSymLds], sy[kSymLds], sz[kSymLds];
-  __shared__ float sr[kSymLds], sr3[kSymLds];  // radius in fixed units, r^3 in work units
-  __shared__ float acc[W][3][kSymLds];
-  const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
-  Rec m[TPL];
-  float nr3[TPL], rf[TPL];
-  double ai[TPL][3];
-  for (int h = 0; h < TPL; ++h) {
-    m[h] = rec[(size_t)b * kSymB + t + T * h];
-    nr3[h] = -m[h].r3;
-    rf[h] = m[h].r * 0x1p22f;
-    ai[h][0] = ai[h][1] = ai[h][2] = 0.0;
-  }
-  const int E = 1 + sym_partners(b, B);
-  const int e0 = s * ec, e1 = min(E, e0 + ec);
-  const int gt = max(1, tile / 4);  // step groups per FP32 partial
-  for (int e = e0; e < e1; ++e) {
-    const int c = e == 0 ? b : (b + e) % B;
-    __syncthreads();  // the previous entry's LDS reads are done
-    for (int sl = t; sl < kSymB; sl += T) {
-      const Rec rc = rec[(size_t)c * kSymB + sl];
-      sym_stage(sx, rc.x, sl);
-      sym_stage(sy, rc.y, sl);
-      sym_stage(sz, rc.z, sl);
-      sym_stage(sr, rc.r * 0x1p22f, sl);
-      sym_stage(sr3, rc.r3, sl);
-    }
-    sym_zero(acc[w], lane);
-    __syncthreads();
-    if (e == 0) {
-      for (int kt = 0; kt < kSymB; kt += tile) {
-        const int ke = min(kt + tile, kSymB);
-        float f[TPL][3] = {};
-#pragma unroll 4 / TPL
-        for (int k = kt; k < ke; ++k) {
-          const int q = lane + k;
-          const uint32_t xj = sx[q], yj = sy[q], zj = sz[q];
-          const float rj = sr[q], rj3 = sr3[q];
-#pragma unroll
-          for (int h = 0; h < TPL; ++h)
-            dem::dipole_pair<false>(dem::diff32(xj, m[h].x), dem::diff32(yj, m[h].y),
-                                    dem::diff32(zj, m[h].z), m[h].r, rj * 0x1p-22f, rj3, f[h][0],
-                                    f[h][1], f[h][2]);
-        }
-        for (int h = 0; h < TPL; ++h)
-          for (int a = 0; a < 3; ++a) ai[h][a] += (double)f[h][a];
-      }
-      continue;
-    }
-    for (int g0 = 0; g0 < 64; g0 += gt) {
-      const int ge = min(g0 + gt, 64);
-      float f[TPL][3] = {};
-      for (int kk = g0; kk < ge; ++kk) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = lane + kk + 64 * u;
-          const uint32_t xj = sx[q], yj = sy[q], zj = sz[q];
-          const float rj = sr[q], rj3 = sr3[q];
-          float o[3] = {acc[w][0][q], acc[w][1][q], acc[w][2][q]};
-#pragma unroll
-          for (int h = 0; h < TPL; ++h) {
-            float g[3];
-            force_pair_sym(dem::diff32(xj, m[h].x), dem::diff32(yj, m[h].y),
-                           dem::diff32(zj, m[h].z), rf[h], rj, rj3, f[h], g);
-            // g is odd in d: the reversed pair adds -r_i^3 g to source q
-            for (int a = 0; a < 3; ++a) o[a] = fmaf(nr3[h], g[a], o[a]);
-          }
-          acc[w][0][q] = o[0];
-          acc[w][1][q] = o[1];
-          acc[w][2][q] = o[2];
-        }
-        wave_sync();
-      }
-      for (int h = 0; h < TPL; ++h)
-        for (int a = 0; a < 3; ++a) ai[h][a] += (double)f[h][a];
-    }
-    __syncthreads();
-    sym_fold<T, W>(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
-  }
-  for (int h = 0; h < TPL; ++h) {
-    const int i = b * kSymB + t + T * h;
-    if (i < n)
-      for (int a = 0; a < 3; ++a) part[(size_t)(3 * s + a) * n + i] = ai[h][a];
-  }
-}
-
-// Sum of a target's pair-once partials: its own chunks, then the reversed
-// pairs from every block that has its block as a partner (ascending b).
-__device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, int n,
-                                        const double* __restrict__ part_j, int B, int i,
-                                        double* sum) {
-  for (int s = 0; s < S; ++s)
-    for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
-  const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
-  for (int b = 0; b < B; ++b) {
-    if (b == bi) continue;
-    const int p = (bi - b - 1 + B) % B;
-    if (p >= sym_partners(b, B)) continue;
-    for (int a = 0; a < 3; ++a)
-      sum[a] += part_j[(((size_t)b * Pmax + p) * 3 + a) * kSymB + il];
-  }
+  const int b = blockIdx.x, s = blockIdx.y;
+  const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
+  dem::force_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
 __global__ __launch_bounds__(256) void k_force_sym_reduce(
@@ -452,7 +276,7 @@ __global__ __launch_bounds__(256) void k_force_sym_reduce(
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   double sum[3] = {0.0, 0.0, 0.0};
-  sym_sum(part, S, n, part_j, B, i, sum);
+  dem::sym_sum(part, S, n, part_j, B, i, sum);
   const DevScales sc = *scp;
   double Fi[3];
   float fr[3];
@@ -520,131 +344,14 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
   for (int a = 0; a < 3; ++a) V[3 * (size_t)i + a] = Vi[a];
 }
 
-// Oseen sum, pair-once (no periodic correction): the Oseen tensor is even in
-// the separation, so the pair's tensor serves both sides; same schedule, slot
-// order and partial layout as k_force_sym.  Separations are the high words of
-// the 64-bit ring differences, as in the one-sided kernel: a pair whose
-// separation lies within one 32-bit unit of half the box along an axis still
-// takes the image the exact positions give (32-bit positions could round it
-// to the other image, which flips that component of the pair's term).
-
-// Oseen pair of one target (force fi) with a source (force Fj): adds the
-// source's term to the target's sums and returns the target's term for the
-// source as ir * o.
-__device__ __forceinline__ float oseen_pair_sym(float dx, float dy, float dz, float Fx, float Fy,
-                                                float Fz, const float* fi, float* v, float* o) {
-  const float r2 = dx * dx + dy * dy + dz * dz;
-  // coincident droplets add nothing: a nonzero r2 is at least 1 >= 1/|d|
-  const float ir = __builtin_amdgcn_fmed3f(r2, 0.f, __builtin_amdgcn_rsqf(r2));
-  const float ir2 = ir * ir;
-  const float tj = (dx * Fx + dy * Fy + dz * Fz) * ir2;
-  v[0] += ir * (Fx + tj * dx);
-  v[1] += ir * (Fy + tj * dy);
-  v[2] += ir * (Fz + tj * dz);
-  const float ti = (dx * fi[0] + dy * fi[1] + dz * fi[2]) * ir2;
-  o[0] = fi[0] + ti * dx;
-  o[1] = fi[1] + ti * dy;
-  o[2] = fi[2] + ti * dz;
-  return ir;
-}
-
 template <int TPL>
 __global__ __launch_bounds__(kSymB / TPL) void k_conv_sym(const RecC* __restrict__ rec, int n,
                                                           int B, int ec, int tile,
                                                           double* __restrict__ part,
                                                           double* __restrict__ part_j) {
-  constexpr int T = kSymB / TPL, W = T / 64;
-  __shared__ unsigned long long sx[kSymLds], sy[kSymLds], sz[kSymLds];
-  __shared__ float sfx[kSymLds], sfy[kSymLds], sfz[kSymLds];
-  __shared__ float acc[W][3][kSymLds];
-  const int b = blockIdx.x, s = blockIdx.y, t = threadIdx.x, w = t >> 6, lane = t & 63;
-  unsigned long long nq[TPL][3];
-  float fi[TPL][3];
-  double ai[TPL][3];
-  for (int h = 0; h < TPL; ++h) {
-    const RecC m = rec[(size_t)b * kSymB + t + T * h];
-    nq[h][0] = 0ull - (((unsigned long long)m.xh << 32) | m.xl);
-    nq[h][1] = 0ull - (((unsigned long long)m.yh << 32) | m.yl);
-    nq[h][2] = 0ull - (((unsigned long long)m.zh << 32) | m.zl);
-    fi[h][0] = m.fx;
-    fi[h][1] = m.fy;
-    fi[h][2] = m.fz;
-    ai[h][0] = ai[h][1] = ai[h][2] = 0.0;
-  }
-  const int E = 1 + sym_partners(b, B);
-  const int e0 = s * ec, e1 = min(E, e0 + ec);
-  const int gt = max(1, tile / 4);
-  for (int e = e0; e < e1; ++e) {
-    const int c = e == 0 ? b : (b + e) % B;
-    __syncthreads();
-    for (int sl = t; sl < kSymB; sl += T) {
-      const RecC rc = rec[(size_t)c * kSymB + sl];
-      sym_stage(sx, ((unsigned long long)rc.xh << 32) | rc.xl, sl);
-      sym_stage(sy, ((unsigned long long)rc.yh << 32) | rc.yl, sl);
-      sym_stage(sz, ((unsigned long long)rc.zh << 32) | rc.zl, sl);
-      sym_stage(sfx, rc.fx, sl);
-      sym_stage(sfy, rc.fy, sl);
-      sym_stage(sfz, rc.fz, sl);
-    }
-    sym_zero(acc[w], lane);
-    __syncthreads();
-    if (e == 0) {
-      for (int kt = 0; kt < kSymB; kt += tile) {
-        const int ke = min(kt + tile, kSymB);
-        float v[TPL][3] = {};
-#pragma unroll 4 / TPL
-        for (int k = kt; k < ke; ++k) {
-          const int q = lane + k;
-          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
-          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
-#pragma unroll
-          for (int h = 0; h < TPL; ++h) {
-            const float dx = dem::diff64n(xj, nq[h][0]), dy = dem::diff64n(yj, nq[h][1]),
-                        dz = dem::diff64n(zj, nq[h][2]);
-            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, dx * dx + dy * dy + dz * dz > 0.f,  // j != i
-                            v[h][0], v[h][1], v[h][2]);
-          }
-        }
-        for (int h = 0; h < TPL; ++h)
-          for (int a = 0; a < 3; ++a) ai[h][a] += (double)v[h][a];
-      }
-      continue;
-    }
-    for (int g0 = 0; g0 < 64; g0 += gt) {
-      const int ge = min(g0 + gt, 64);
-      float v[TPL][3] = {};
-      for (int kk = g0; kk < ge; ++kk) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = lane + kk + 64 * u;
-          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
-          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
-          float o[3] = {acc[w][0][q], acc[w][1][q], acc[w][2][q]};
-#pragma unroll
-          for (int h = 0; h < TPL; ++h) {
-            float oh[3];
-            const float ir = oseen_pair_sym(dem::diff64n(xj, nq[h][0]), dem::diff64n(yj, nq[h][1]),
-                                            dem::diff64n(zj, nq[h][2]), Fx, Fy, Fz, fi[h], v[h],
-                                            oh);
-            for (int a = 0; a < 3; ++a) o[a] = fmaf(ir, oh[a], o[a]);
-          }
-          acc[w][0][q] = o[0];
-          acc[w][1][q] = o[1];
-          acc[w][2][q] = o[2];
-        }
-        wave_sync();
-      }
-      for (int h = 0; h < TPL; ++h)
-        for (int a = 0; a < 3; ++a) ai[h][a] += (double)v[h][a];
-    }
-    __syncthreads();
-    sym_fold<T, W>(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
-  }
-  for (int h = 0; h < TPL; ++h) {
-    const int i = b * kSymB + t + T * h;
-    if (i < n)
-      for (int a = 0; a < 3; ++a) part[(size_t)(3 * s + a) * n + i] = ai[h][a];
-  }
+  const int b = blockIdx.x, s = blockIdx.y;
+  const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
+  dem::conv_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
 __global__ __launch_bounds__(256) void k_conv_sym_reduce(
@@ -655,7 +362,7 @@ __global__ __launch_bounds__(256) void k_conv_sym_reduce(
   if (i >= n) return;
   double m[3] = {0.0, 0.0, 0.0};
   const double c[3] = {0.0, 0.0, 0.0};
-  sym_sum(part, S, n, part_j, B, i, m);
+  dem::sym_sum(part, S, n, part_j, B, i, m);
   const DevScales sc = *scp;
   const dem::CorrScalars cs{};
   double Vi[3];
@@ -829,6 +536,19 @@ int prepare(const Layout& l, const double* pos, const double* radii, const doubl
 
 constexpr int kMaxDevices = 64;
 
+// Chunks of the pair-once entry lists: the explicit split in mode 2, else
+// one round of co-resident workgroups.
+int sym_split(int n, int slots) {
+  if (g_sym == 2 && g_split > 0) {
+    int S = g_split;
+    const int emax = dem::sym_entries_max(sym_blocks(n));
+    if (S > emax) S = emax;
+    if (S > kSymMaxChunks) S = kSymMaxChunks;
+    return S < 1 ? 1 : S;
+  }
+  return sym_chunks(n, slots);
+}
+
 // Workgroups of `kernel` (`threads` each) the current device holds at once:
 // compute units x resident workgroups per unit, cached per device.
 int resident_slots(int* cache, const void* kernel, int threads) {
@@ -851,8 +571,8 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
   if (use_sym(n)) {
     static int slots[kMaxDevices];
     const int B = sym_blocks(n);
-    const int S = sym_chunks(n, resident_slots(slots, (const void*)k_force_sym<kSymTplForce>,
-                                               kSymB / kSymTplForce));
+    const int S = sym_split(n, resident_slots(slots, (const void*)k_force_sym<kSymTplForce>,
+                                              kSymB / kSymTplForce));
     const int ec = (1 + B / 2 + S - 1) / S;
     DEM_PROF_LAUNCH("k_force_sym", st,
                     hipLaunchKernelGGL(k_force_sym<kSymTplForce>, dim3(B, S),
@@ -890,8 +610,8 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
   if (!on && use_sym(n)) {
     static int slots[kMaxDevices];
     const int B = sym_blocks(n);
-    const int S = sym_chunks(n, resident_slots(slots, (const void*)k_conv_sym<kSymTplConv>,
-                                               kSymB / kSymTplConv));
+    const int S = sym_split(n, resident_slots(slots, (const void*)k_conv_sym<kSymTplConv>,
+                                              kSymB / kSymTplConv));
     const int ec = (1 + B / 2 + S - 1) / S;
     DEM_PROF_LAUNCH("k_conv_sym", st,
                     hipLaunchKernelGGL(k_conv_sym<kSymTplConv>, dim3(B, S),
@@ -940,7 +660,7 @@ size_t dem_direct_workspace_bytes(int n) { return carve(nullptr, n < 0 ? 0 : n).
 
 int dem_direct_set_symmetric(int on) {
   const int prev = g_sym;
-  g_sym = on ? 1 : 0;
+  g_sym = on < 0 ? 0 : (on > 2 ? 2 : on);
   return prev;
 }
 

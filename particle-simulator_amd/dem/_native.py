@@ -100,7 +100,7 @@ class DemBatchTuning(ctypes.Structure):
         ("force_tile", ctypes.c_int32),
         ("conv_tile", ctypes.c_int32),
         ("contacts", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("pair_once", ctypes.c_int32),
     ]
 
 
@@ -134,6 +134,7 @@ SIGNATURES: dict[str, tuple] = {
     "dem_octree_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_S), ctypes.POINTER(_S)]),
     "dem_octree_collision_counts": (_I, [_P, _P, _I, _P, _P, _S, _P]),
     "dem_direct_collision_counts": (_I, [_P, _P, _I, _D, _I, _P, _I, _P, _P]),
+    "dem_batch_pair_once_split": (_I, [_I, _I]),
     "dem_batch_split": (_I, [_I, _I]),
     "dem_batch_workspace_bytes": (_S, [_P, _I, _P]),
     "dem_batch_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _S, _P, ctypes.POINTER(_P)]),

@@ -39,6 +39,7 @@ from .solution.droplet_solution import merge_collided, merge_plan_rows
 
 BLOCKS = (64, 128, 256)
 JLENS = (1024, 2048, 4096, 8192)
+PAIR_ONCE_JLENS = (1024, 2048, 4096, 8192)  # 4 ... 32 partner blocks per work item
 
 
 @dataclass
@@ -48,15 +49,20 @@ class Tuning:
     force_tile: int = 0
     conv_tile: int = 0
     contacts: bool = True  # False: no contact pass (solver without a detector)
+    pair_once: bool = False  # True: pair-once sums, jlen / 256 partner blocks per work item
 
     def native(self) -> _native.DemBatchTuning:
         return _native.DemBatchTuning(self.block, self.jlen, self.force_tile, self.conv_tile,
-                                      1 if self.contacts else 0, 0)
+                                      1 if self.contacts else 0, 1 if self.pair_once else 0)
 
 
-def split_for(n: int, jlen: int) -> int:
-    """Source split the batch uses for a system of n droplets."""
-    return int(_native.load().dem_batch_split(int(n), int(jlen)))
+def split_for(n: int, jlen: int, pair_once: bool = False) -> int:
+    """Source split (one-sided) or partner-list chunks (pair-once) the batch
+    uses for a system of n droplets."""
+    lib = _native.load()
+    if pair_once:
+        return int(lib.dem_batch_pair_once_split(int(n), int(jlen)))
+    return int(lib.dem_batch_split(int(n), int(jlen)))
 
 
 def _consts(s):
@@ -325,14 +331,17 @@ class SequentialEnsembles:
     Per step of system e: contact pairs (``dem_hash_pairs``; the pair count
     crosses to the host), merge and re-check while there are contacts, then
     ``dem_direct_forces_and_convection`` and ``dem_euler_step``.  With
-    ``jlen`` set, each force evaluation uses the batched source split
-    (``split_for(n, jlen)``), which makes its sums bit-identical to
-    ``BatchedEnsembles`` with the same ``jlen``.
+    ``jlen`` set, each force evaluation uses the batched split
+    (``split_for(n, jlen, pair_once)``; pair-once mode 2 when ``pair_once``),
+    which makes its sums bit-identical to ``BatchedEnsembles`` with the same
+    ``jlen`` and ``pair_once``.
     """
 
-    def __init__(self, systems, dts, steps, jlen: int | None = None, contacts: bool = True):
+    def __init__(self, systems, dts, steps, jlen: int | None = None, contacts: bool = True,
+                 pair_once: bool = False):
         self.n_ens = len(systems)
         self.jlen = jlen
+        self.pair_once = pair_once
         self.contacts = contacts
         self.dt = np.asarray(dts, dtype=np.float64).reshape(self.n_ens)
         self.steps = np.asarray(steps, dtype=np.int64).reshape(self.n_ens)
@@ -369,14 +378,18 @@ class SequentialEnsembles:
             self.radii[e] = _device.to_device(new_r)
             self.merges += 1
         n = int(self.radii[e].shape[0])
-        prev = None
+        prev = prev_sym = None
         if self.jlen is not None:
-            prev = _native.call("dem_direct_set_tuning", split_for(n, self.jlen), 0)
+            if self.pair_once:
+                prev_sym = _native.call("dem_direct_set_symmetric", 2)
+            prev = _native.call("dem_direct_set_tuning", split_for(n, self.jlen, self.pair_once), 0)
         try:
             F, V = self.fc[e].forces_and_convection_device(self.pos[e], self.radii[e])
         finally:
             if prev is not None:
                 _native.call("dem_direct_set_tuning", prev >> 16, prev & 0xFFFF)
+            if prev_sym is not None:
+                _native.call("dem_direct_set_symmetric", prev_sym)
         _native.call("dem_euler_step", _device.ptr(self.pos[e]), _device.ptr(F), _device.ptr(V),
                      _device.ptr(self.radii[e]), n, float(self.sf[e]), float(self.dt[e]), L, 1,
                      _device.stream())
@@ -398,32 +411,37 @@ class SequentialEnsembles:
         return self.radii[e].cpu().numpy()
 
 
-def autotune(systems, dts, blocks=BLOCKS, jlens=JLENS, warmup: int = 1, reps: int = 3,
-             verbose: bool = False):
-    """Fastest (block, jlen) for these systems: each candidate runs ``warmup``
-    + ``reps`` captured force/Oseen/Euler steps of every system (contact pass
-    off, so every system advances) on a throw-away copy of the state.
-    Returns (Tuning with contacts on, {(block, jlen): ms per step})."""
+def autotune(systems, dts, blocks=BLOCKS, jlens=JLENS, pair_once_jlens=PAIR_ONCE_JLENS,
+             warmup: int = 1, reps: int = 3, verbose: bool = False):
+    """Fastest work shape for these systems: one-sided (block, jlen) and
+    pair-once (jlen) candidates each run ``warmup`` + ``reps`` captured
+    force/Oseen/Euler steps of every system (contact pass off, so every system
+    advances) on a throw-away copy of the state.  Returns (Tuning with
+    contacts on, {candidate: ms per step}); candidates are (block, jlen) and
+    ("pair_once", jlen)."""
     t = _device.torch()
     times = {}
     big = 1 << 30
-    for b in blocks:
-        for j in jlens:
-            eng = BatchedEnsembles(systems, dts, [big] * len(systems),
-                                   Tuning(b, j, contacts=False), capture=True)
-            st = _device.stream()
-            _native.call("dem_batch_run", eng._h, warmup, st)
-            t.cuda.synchronize()
-            t0 = time.perf_counter()
-            _native.call("dem_batch_run", eng._h, reps, st)
-            t.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / reps
-            times[(b, j)] = ms
-            if verbose:
-                print(f"autotune block={b} jlen={j}: {ms:.3f} ms/step", flush=True)
-            eng.close()
-            del eng
+    cands = [((b, j), Tuning(b, j, contacts=False)) for b in blocks for j in jlens]
+    cands += [(("pair_once", j), Tuning(256, j, contacts=False, pair_once=True))
+              for j in pair_once_jlens]
+    for key, tune in cands:
+        eng = BatchedEnsembles(systems, dts, [big] * len(systems), tune, capture=True)
+        st = _device.stream()
+        _native.call("dem_batch_run", eng._h, warmup, st)
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        _native.call("dem_batch_run", eng._h, reps, st)
+        t.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        times[key] = ms
+        if verbose:
+            print(f"autotune {key}: {ms:.3f} ms/step", flush=True)
+        eng.close()
+        del eng
     best = min(times, key=times.get)
+    if best[0] == "pair_once":
+        return Tuning(256, best[1], pair_once=True), times
     return Tuning(best[0], best[1]), times
 
 

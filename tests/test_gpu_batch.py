@@ -86,11 +86,32 @@ def test_batched_equals_sequential_bit_for_bit(gpu, setup, block, jlen):
     b.close()
 
 
-def test_first_step_against_fp64_oracle(gpu, setup):
+@pytest.mark.parametrize("jlen", [256, 1024, 100000])
+def test_pair_once_batched_equals_sequential_bit_for_bit(gpu, setup, jlen):
+    """Pair-once sums (blocks of 256, 1 / 4 / all partner blocks per work
+    item) against the single-system path in pair-once mode 2 with the same
+    chunks, through contacts, merges and uneven step budgets."""
+    systems, dts = setup
+    b = BatchedEnsembles(systems, dts, STEPS, Tuning(256, jlen, pair_once=True), capture=True)
+    b.run_until_done(max_global_steps=200)
+    s = SequentialEnsembles(systems, dts, STEPS, jlen=jlen, pair_once=True)
+    s.run_until_done()
+    assert b.done() and b.merges == s.merges and b.merges > 0
+    for e in range(len(systems)):
+        assert np.array_equal(b.radii_of(e), s.radii_of(e)), f"system {e} radii"
+        assert np.array_equal(b.positions(e), s.positions(e)), f"system {e} positions"
+        n = len(b.radii_of(e))
+        np.testing.assert_array_equal(b.forces(e), s.F[e].cpu().numpy()[:n])
+    b.close()
+
+
+@pytest.mark.parametrize("pair_once", [False, True])
+def test_first_step_against_fp64_oracle(gpu, setup, pair_once):
     systems, dts = setup
     sub = [systems[0], systems[2], systems[5]]
     sdt = [dts[0], dts[2], dts[5]]
-    b = BatchedEnsembles(sub, sdt, [1, 1, 1], Tuning(256, 512), capture=False)
+    b = BatchedEnsembles(sub, sdt, [1, 1, 1], Tuning(256, 512, pair_once=pair_once),
+                         capture=False)
     b.run(1, merge=False)
     assert np.array_equal(b.steps_done.cpu().numpy(), np.array([1, 1, 1]))
     for e, sy in enumerate(sub):
@@ -214,6 +235,9 @@ def test_without_contact_pass_equals_sequential_without_detector(gpu, setup):
 
 def test_autotune_returns_a_timed_candidate(gpu, setup):
     systems, dts = setup
-    best, times = autotune(systems[:3], dts[:3], blocks=(64, 256), jlens=(512, 4096), reps=1)
-    assert (best.block, best.jlen) in times and len(times) == 4
+    best, times = autotune(systems[:3], dts[:3], blocks=(64, 256), jlens=(512, 4096),
+                           pair_once_jlens=(1024,), reps=1)
+    key = ("pair_once", best.jlen) if best.pair_once else (best.block, best.jlen)
+    assert key in times and len(times) == 5 and ("pair_once", 1024) in times
+    assert times[key] == min(times.values())
     assert all(t > 0 for t in times.values())

@@ -231,3 +231,27 @@ def test_pair_once_kernels_against_fp64_kernel(gpu, n, boundary):
     # repeated evaluation is bit-identical (fixed summation order)
     F2, V2 = fc.forces_and_convection_device(pd, rd)
     assert np.array_equal(F2.cpu().numpy(), F) and np.array_equal(V2.cpu().numpy(), V)
+
+
+@pytest.mark.parametrize("n,split", [(300, 0), (2000, 1), (2000, 3), (5000, 7)])
+def test_forced_pair_once_small_n(gpu, n, split):
+    """Pair-once mode 2 (every N, explicit chunk count) under the accuracy
+    contract: one block of 256 plus padding (N = 300), several blocks with
+    1 / 3 / 7 chunks of the partner lists."""
+    from dem import _device, _native
+
+    s = make_system(n, seed=5, phi=0.04)
+    fc = _calc(s)
+    pd, rd = _device.to_device(s.positions), _device.to_device(s.radii)
+    prev_sym = _native.call("dem_direct_set_symmetric", 2)
+    prev = _native.call("dem_direct_set_tuning", split, 0)
+    try:
+        F, V = fc.forces_and_convection_device(pd, rd)
+    finally:
+        _native.call("dem_direct_set_tuning", prev >> 16, prev & 0xFFFF)
+        _native.call("dem_direct_set_symmetric", prev_sym)
+    F64, V64 = fc.forces_and_convection_f64_device(pd, rd)
+    ef = summary(F.cpu().numpy(), F64.cpu().numpy(), FLOOR)
+    ev = summary(V.cpu().numpy(), V64.cpu().numpy(), FLOOR)
+    print(f"\n[pair-once mode 2 N={n} split={split}] force {ef}\n  conv {ev}")
+    assert _ok_forces(ef) and _ok_velocities(ev)
