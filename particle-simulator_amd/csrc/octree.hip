@@ -654,52 +654,73 @@ constexpr int kActive = -2;  // lane state: in the walk; otherwise the resume no
 __device__ __forceinline__ int wave_node(int node) { return __builtin_amdgcn_readfirstlane(node); }
 
 // Force walk (:783-906).  Writes forces (original order) and the stored copy.
+// TPL targets per lane: a wave walks for 64 TPL consecutive (Morton-ordered)
+// targets, lane l holding targets base + 64 h + l; node loads, leaf-member
+// loads and the walk's control are shared by all of them.
+template <int TPL>
 __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* __restrict__ scp,
                                                      double m_const, double* __restrict__ F,
                                                      double* __restrict__ pforces) {
 #pragma clang fp contract(off)
-  const int t0 = blockIdx.x * 256 + threadIdx.x;
-  const bool live = t0 < wa.n;
-  const int t = live ? t0 : wa.n - 1;
+  const int lane = threadIdx.x & 63;
+  const int base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 * TPL;
   const Scales sc = *scp;
-  const RecF me = wa.srec[t];
-  const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
-               pz = wa.spos[3 * (size_t)t + 2];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  int resume = live ? kActive : -1;
+  int t[TPL], resume[TPL];
+  bool live[TPL];
+  RecF me[TPL];
+  double p[TPL][3], acc[TPL][3];
+  for (int h = 0; h < TPL; ++h) {
+    const int t0 = base + 64 * h + lane;
+    live[h] = t0 < wa.n;
+    t[h] = live[h] ? t0 : wa.n - 1;
+    me[h] = wa.srec[t[h]];
+    for (int a = 0; a < 3; ++a) {
+      p[h][a] = wa.spos[3 * (size_t)t[h] + a];
+      acc[h][a] = 0.0;
+    }
+    resume[h] = live[h] ? kActive : -1;
+  }
   int node = 0;
   int steps = wa.guard;  // a preorder walk visits each node at most once
   while (node >= 0 && --steps > 0) {
-    if (resume == node) resume = kActive;
-    const bool act = resume == kActive;
+    bool act[TPL];
+    bool any = false;
+    for (int h = 0; h < TPL; ++h) {
+      if (resume[h] == node) resume[h] = kActive;
+      act[h] = resume[h] == kActive;
+      any = any || act[h];
+    }
     const NodeW nd = wa.nw[node];
     if (nd.flags & kSkipR3) {
       node = wave_node(nd.next);
       continue;
     }
     if (nd.first_child < 0) {
-      if (act) {
-        float fx = 0.f, fy = 0.f, fz = 0.f;
+      if (any) {
+        float f[TPL][3] = {};
         const int e = nd.mstart + nd.count;
         for (int j = nd.mstart; j < e; ++j) {
           const RecF sj = wa.srec[j];
-          // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
-          if (j != t)
-            dem::dipole_pair<true>(dem::diff32(sj.x, me.x), dem::diff32(sj.y, me.y),
-                                   dem::diff32(sj.z, me.z), me.r, sj.r, sj.r3, fx, fy, fz);
+          for (int h = 0; h < TPL; ++h)
+            // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
+            if (act[h] && j != t[h])
+              dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
+                                     dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3, f[h][0],
+                                     f[h][1], f[h][2]);
         }
-        ax += (double)fx;
-        ay += (double)fy;
-        az += (double)fz;
+        for (int h = 0; h < TPL; ++h)
+          if (act[h])
+            for (int a = 0; a < 3; ++a) acc[h][a] += (double)f[h][a];
       }
       node = wave_node(nd.next);
       continue;
     }
     bool open = false;
-    if (act) {
-      double dx = mic64(nd.cx - px, wa.L, wa.periodic);
-      double dy = mic64(nd.cy - py, wa.L, wa.periodic);
-      double dz = mic64(nd.cz - pz, wa.L, wa.periodic);
+    for (int h = 0; h < TPL; ++h) {
+      if (!act[h]) continue;
+      double dx = mic64(nd.cx - p[h][0], wa.L, wa.periodic);
+      double dy = mic64(nd.cy - p[h][1], wa.L, wa.periodic);
+      double dz = mic64(nd.cz - p[h][2], wa.L, wa.periodic);
       double d2 = dx * dx;
       d2 = d2 + dy * dy;
       d2 = d2 + dz * dz;
@@ -707,28 +728,31 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
         float fx = 0.f, fy = 0.f, fz = 0.f;
         dem::dipole_monopole((float)(dx * sc.inv_U), (float)(dy * sc.inv_U),
                              (float)(dz * sc.inv_U), nd.r3w, fx, fy, fz);
-        ax += (double)fx;
-        ay += (double)fy;
-        az += (double)fz;
-        resume = nd.next;
+        acc[h][0] += (double)fx;
+        acc[h][1] += (double)fy;
+        acc[h][2] += (double)fz;
+        resume[h] = nd.next;
       } else {
         open = true;
       }
     }
     node = wave_node(__any(open) ? nd.first_child : nd.next);
   }
-  if (!live) return;
-  if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
-  const double rl = wa.srad[t] / sc.lambda;
-  const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
-  const int o = wa.perm[t];
-  const double Fx = c * ax, Fy = c * ay, Fz = c * az;
-  F[3 * (size_t)o] = Fx;
-  F[3 * (size_t)o + 1] = Fy;
-  F[3 * (size_t)o + 2] = Fz;
-  pforces[3 * (size_t)o] = Fx;
-  pforces[3 * (size_t)o + 1] = Fy;
-  pforces[3 * (size_t)o + 2] = Fz;
+  for (int h = 0; h < TPL; ++h) {
+    if (!live[h]) continue;
+    double ax = acc[h][0], ay = acc[h][1], az = acc[h][2];
+    if (steps <= 0) ax = ay = az = NAN;  // malformed next chain: fail loudly
+    const double rl = wa.srad[t[h]] / sc.lambda;
+    const double c = m_const * sc.lambda * sc.lambda * kSigma3 * (rl * rl * rl);
+    const int o = wa.perm[t[h]];
+    const double Fx = c * ax, Fy = c * ay, Fz = c * az;
+    F[3 * (size_t)o] = Fx;
+    F[3 * (size_t)o + 1] = Fy;
+    F[3 * (size_t)o + 2] = Fz;
+    pforces[3 * (size_t)o] = Fx;
+    pforces[3 * (size_t)o + 1] = Fy;
+    pforces[3 * (size_t)o + 2] = Fz;
+  }
 }
 
 // Stokeslet / total-velocity walk (:1000-1280, :1326-1588).
@@ -949,6 +973,9 @@ __global__ __launch_bounds__(256) void k_walk_collisions(
 // ---------------------------------------------------------------- host helpers
 
 inline int nb256(long long n) { return (int)((n + 255) / 256); }
+// workgroups of 256 lanes walking `tpl` targets each
+inline int nb256w(long long n, int tpl) { return (int)((n + 256LL * tpl - 1) / (256LL * tpl)); }
+constexpr int kWalkTpl = 1;  // 2 measured slower: the wider union of the walks costs more than it hides
 
 int check_info(const dem_tree_info* info, const void* ws, size_t ws_bytes, TreeLayout& l) {
   if (!info) return dem::set_error(DEM_ERR_INVALID, "tree info is NULL");
@@ -1186,7 +1213,8 @@ int dem_octree_forces(dem_tree_info* info, double theta_sq, double m_const, doub
   hipStream_t st = dem::as_stream(stream);
   WalkArgs wa = walk_args(info, l, theta_sq);
   DEM_PROF_LAUNCH("k_walk_forces", st,
-                  hipLaunchKernelGGL(k_walk_forces, dim3(nb256(info->n)), dim3(256), 0, st, wa,
+                  hipLaunchKernelGGL(k_walk_forces<kWalkTpl>, dim3(nb256w(info->n, kWalkTpl)),
+                                     dim3(256), 0, st, wa,
                                      (const Scales*)l.sc, m_const, forces, l.pforces));
   DEM_CHECK_LAUNCH("k_walk_forces");
   info->forces_valid = 1;
