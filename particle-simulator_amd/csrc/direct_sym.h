@@ -330,19 +330,22 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
 }
 
 // Sum of a target's pair-once partials: its own chunks, then the reversed
-// pairs from every block that has its block as a partner (ascending b).
+// pairs from the blocks b = bi - e (mod B), e = 1, 2, ..., that have block bi
+// as their e-th partner (e <= P_b).
 __device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, int n,
                                         const double* __restrict__ part_j, int B, int i,
                                         double* sum) {
   for (int s = 0; s < S; ++s)
     for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
   const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
-  for (int b = 0; b < B; ++b) {
-    if (b == bi) continue;
-    const int p = (bi - b - 1 + B) % B;
-    if (p >= sym_partners(b, B)) continue;
-    for (int a = 0; a < 3; ++a)
-      sum[a] += part_j[(((size_t)b * Pmax + p) * 3 + a) * kSymB + il];
+#pragma unroll 4
+  for (int e = 1; e <= Pmax; ++e) {
+    const int b = bi >= e ? bi - e : bi - e + B;
+    if (e > sym_partners(b, B)) continue;
+    const double* q = part_j + ((size_t)b * Pmax + (e - 1)) * 3 * kSymB + il;
+    sum[0] += q[0];
+    sum[1] += q[kSymB];
+    sum[2] += q[2 * kSymB];
   }
 }
 
