@@ -16,8 +16,8 @@
 // This file reproduces exactly that set.
 //
 // MI355X design: counting sort of particles by bucket (atomic counts, device
-// scan, atomic scatter), then two passes over targets: count pairs per i,
-// scan, emit.  Each target visits each distinct bucket once, so pairs are
+// scan, atomic scatter with a bucket-ordered copy of (x, y, z, r)), then two
+// passes over targets in bucket order: count pairs per i, scan, emit.  Each target visits each distinct bucket once, so pairs are
 // unique without a global sort; each target's pairs land in its own segment,
 // sorted by j in-segment, which makes the output lexicographic and
 // deterministic.  The contact predicate is evaluated in float64 with FMA
@@ -49,7 +49,8 @@ struct Layout {
   int* counts;        // [T]
   int* offsets;       // [T]
   int* cursor;        // [T]
-  int* sorted;        // [n]
+  int* sorted;        // [n] droplet of each slot, bucket order
+  double4* spos;      // [n] (x, y, z, r) of each slot
   int* pair_cnt;      // [n]
   int* pair_off;      // [n]
   int* scan_ws;
@@ -70,6 +71,7 @@ Layout carve(void* ws, int n, int T) {
   l.offsets = c.take<int>((size_t)T);
   l.cursor = c.take<int>((size_t)T);
   l.sorted = c.take<int>(nn);
+  l.spos = c.take<double4>(nn);
   l.pair_cnt = c.take<int>(nn);
   l.pair_off = c.take<int>(nn);
   l.scan_ws = c.take<int>(dem::scan_workspace_ints(big));
@@ -123,78 +125,95 @@ __global__ __launch_bounds__(256) void k_assign(const double* __restrict__ pos, 
 
 __global__ __launch_bounds__(256) void k_scatter(int n, const HashParams* __restrict__ hpp,
                                                  const int* __restrict__ bucket,
-                                                 int* __restrict__ cursor, int* __restrict__ sorted) {
+                                                 const double* __restrict__ pos,
+                                                 const double* __restrict__ r,
+                                                 int* __restrict__ cursor, int* __restrict__ sorted,
+                                                 double4* __restrict__ spos) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n || hpp->skip) return;
   int slot = atomicAdd(&cursor[bucket[i]], 1);
   sorted[slot] = i;
+  spos[slot] = make_double4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2],
+                            r[i]);
 }
 
-// EMIT = false: count pairs of target i; EMIT = true: write them.
+// EMIT = false: count pairs of each target; EMIT = true: write them.  Thread
+// t takes the droplet in bucket slot t (neighbouring threads then search the
+// same buckets).  The distinct neighbour buckets' ranges are loaded up front
+// and candidates come from the bucket-ordered (x, y, z, r) copy; the droplet
+// index of a slot is read only for a contact.
 template <bool EMIT>
-__global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ pos,
-                                               const double* __restrict__ r, int n, double L,
-                                               int periodic, int mask,
+__global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, int mask,
                                                const HashParams* __restrict__ hpp,
                                                const int* __restrict__ counts,
                                                const int* __restrict__ offsets,
                                                const int* __restrict__ sorted,
+                                               const double4* __restrict__ spos,
                                                int* __restrict__ pair_cnt,
                                                const int* __restrict__ pair_off,
                                                int* __restrict__ pairs, int cap) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
   const HashParams hp = *hpp;
-  if (hp.skip) {
-    if (!EMIT) pair_cnt[i] = 0;
+  if (hp.skip) {  // nothing was bucketed: slot t stands for droplet t
+    if (!EMIT) pair_cnt[t] = 0;
     return;
   }
   const int g = hp.g;
-  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
-  const double ri = r[i];
-  const int ix = cell_of(xi, hp.cell, g, periodic);
-  const int iy = cell_of(yi, hp.cell, g, periodic);
-  const int iz = cell_of(zi, hp.cell, g, periodic);
-  int seen[27];
+  const int i = sorted[t];
+  const double4 me = spos[t];
+  const int ix = cell_of(me.x, hp.cell, g, periodic);
+  const int iy = cell_of(me.y, hp.cell, g, periodic);
+  const int iz = cell_of(me.z, hp.cell, g, periodic);
+  int nb[27];
   int nseen = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    int nx = ix + k / 9 - 1, ny = iy + (k / 3) % 3 - 1, nz = iz + k % 3 - 1;
+    bool valid = true;
+    if (periodic) {
+      nx = pymod(nx, g);
+      ny = pymod(ny, g);
+      nz = pymod(nz, g);
+    } else {
+      valid = nx >= 0 && ny >= 0 && nz >= 0 && nx < g && ny < g && nz < g;
+    }
+    if (valid) {
+      const int nh = hash_cell(nx, ny, nz, mask);
+      bool dup = false;
+#pragma unroll
+      for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
+      if (!dup) nb[nseen++] = nh;
+    }
+  }
+  int s0[27], s1[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int h = k < nseen ? nb[k] : 0;
+    s0[k] = offsets[h];
+    s1[k] = k < nseen ? s0[k] + counts[h] : s0[k];
+  }
   int cnt = 0;
   const int base = EMIT ? pair_off[i] : 0;
-#pragma unroll 1
-  for (int di = -1; di <= 1; ++di)
-#pragma unroll 1
-    for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll 1
-      for (int dk = -1; dk <= 1; ++dk) {
-        int nx = ix + di, ny = iy + dj, nz = iz + dk;
-        if (periodic) {
-          nx = pymod(nx, g);
-          ny = pymod(ny, g);
-          nz = pymod(nz, g);
-        } else if (nx < 0 || ny < 0 || nz < 0 || nx >= g || ny >= g || nz >= g) {
-          continue;
-        }
-        const int nh = hash_cell(nx, ny, nz, mask);
-        bool dup = false;
-        for (int k = 0; k < nseen; ++k) dup |= (seen[k] == nh);
-        if (dup) continue;
-        seen[nseen++] = nh;
-        const int start = offsets[nh];
-        const int end = start + counts[nh];
-        for (int slot = start; slot < end; ++slot) {
-          const int j = sorted[slot];
-          if (i < j && contact_pair(xi, yi, zi, ri, pos[3 * (size_t)j], pos[3 * (size_t)j + 1],
-                                    pos[3 * (size_t)j + 2], r[j], L, periodic)) {
-            if (EMIT) {
-              const int at = base + cnt;
-              if (at < cap) {
-                pairs[2 * (size_t)at] = i;
-                pairs[2 * (size_t)at + 1] = j;
-              }
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    for (int slot = s0[k]; slot < s1[k]; ++slot) {
+      const double4 q = spos[slot];
+      if (contact_pair(me.x, me.y, me.z, me.w, q.x, q.y, q.z, q.w, L, periodic)) {
+        const int j = sorted[slot];
+        if (i < j) {
+          if (EMIT) {
+            const int at = base + cnt;
+            if (at < cap) {
+              pairs[2 * (size_t)at] = i;
+              pairs[2 * (size_t)at + 1] = j;
             }
-            ++cnt;
           }
+          ++cnt;
         }
       }
+    }
+  }
   if (!EMIT) {
     pair_cnt[i] = cnt;
   } else if (cnt > 1) {
@@ -272,18 +291,18 @@ int dem_hash_pairs(const double* pos, const double* radii, int n, double L, int 
                      l.cursor, T);
   DEM_CHECK_LAUNCH("k_copy");
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, (const HashParams*)l.hp,
-                     (const int*)l.bucket, l.cursor, l.sorted);
+                     (const int*)l.bucket, pos, radii, l.cursor, l.sorted, l.spos);
   DEM_CHECK_LAUNCH("k_scatter");
-  hipLaunchKernelGGL(k_pairs<false>, dim3(nb), dim3(256), 0, st, pos, radii, n, L,
-                     periodic ? 1 : 0, mask, (const HashParams*)l.hp, (const int*)l.counts,
-                     (const int*)l.offsets, (const int*)l.sorted, l.pair_cnt,
+  hipLaunchKernelGGL(k_pairs<false>, dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
+                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
+                     (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
                      (const int*)l.pair_off, pairs, pair_capacity);
   DEM_CHECK_LAUNCH("k_pairs<count>");
   rc = dem::exclusive_scan(l.pair_cnt, l.pair_off, n, l.scan_ws, count_out, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pairs<true>, dim3(nb), dim3(256), 0, st, pos, radii, n, L,
-                     periodic ? 1 : 0, mask, (const HashParams*)l.hp, (const int*)l.counts,
-                     (const int*)l.offsets, (const int*)l.sorted, l.pair_cnt,
+  hipLaunchKernelGGL(k_pairs<true>, dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
+                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
+                     (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
                      (const int*)l.pair_off, pairs, pair_capacity);
   DEM_CHECK_LAUNCH("k_pairs<emit>");
   return DEM_OK;
