@@ -206,11 +206,30 @@ def tree_step(reps: int = 5) -> dict:
         return {"mean_pct": float(e.mean() * 100), "max_pct": float(e.max() * 100)}
 
     info = tc.octree._info
+    # algorithmic bytes (HBM roofline, 8 TB/s): build N (24 key/index + 24 p radix
+    # passes, p = ceil(3 D / 8) + 32 permute + 8 D level scans + 16 leaf
+    # aggregates) + nodes x 96 B; walks compulsory N (16 in + 24 out) + nodes x
+    # 64 B (force) or + nodes x 128 B and 24 B more per target (total velocity)
+    D = int(info.max_depth)
+    M = int(info.node_count)
+    p = -(-3 * D // 8)
+    b_build = n * (24 + 24 * p + 32 + 8 * D + 16) + M * 96
+    b_force = n * (16 + 24) + M * 64
+    b_total = n * (16 + 24 + 24) + M * 128
+    hbm = {"build_bytes": b_build, "build_GBps": b_build / (build_ms * 1e-3) / 1e9,
+           "force_walk_bytes": b_force,
+           "force_walk_GBps": b_force / (walks["k_walk_forces"] * 1e-3) / 1e9,
+           "total_walk_bytes": b_total,
+           "total_walk_GBps": b_total / (walks["k_walk_total"] * 1e-3) / 1e9,
+           "peak_GBps": 8000.0,
+           "note": "compulsory bytes; node arrays and leaf records are L2/MALL-resident, so "
+                   "the walks are bound by their dependent node loads and FP32 interactions, "
+                   "the build by its ~100 dependent launches"}
     return {
         "workload": "N=1048576, phi=0.02, periodic, theta=0.5, leaf 16, seed 0",
         "ms_per_step": ms, "build_ms": build_ms,
-        "walk_ms": walks, "nodes": int(info.node_count), "max_nodes": int(tc.octree.max_nodes),
-        "depth": int(info.max_depth),
+        "walk_ms": walks, "hbm": hbm, "nodes": M, "max_nodes": int(tc.octree.max_nodes),
+        "depth": D,
         "force_err_vs_fp64_direct": err(Ft, F.cpu().numpy()[tg]),
         "total_velocity_err_vs_fp64_direct": err(vt_ref, V.cpu().numpy()[tg]),
         "note": "errors are the reference algorithm's Barnes-Hut errors (the GPU walk matches "

@@ -149,7 +149,7 @@ __device__ __forceinline__ void force_sym_block(const Rec* __restrict__ rec, int
       for (int kt = 0; kt < kSymB; kt += tile) {
         const int ke = min(kt + tile, kSymB);
         float f[TPL][3] = {};
-#pragma unroll 4 / TPL
+#pragma unroll 2
         for (int k = kt; k < ke; ++k) {
           const int q = lane + k;
           const uint32_t xj = sx[q], yj = sy[q], zj = sz[q];
@@ -274,7 +274,7 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
       for (int kt = 0; kt < kSymB; kt += tile) {
         const int ke = min(kt + tile, kSymB);
         float v[TPL][3] = {};
-#pragma unroll 4 / TPL
+#pragma unroll 2
         for (int k = kt; k < ke; ++k) {
           const int q = lane + k;
           const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
