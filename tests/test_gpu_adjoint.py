@@ -65,9 +65,9 @@ def test_step_vjp_matches_fp64_autograd(gpu):
     sums = torch.empty(3, dtype=torch.float64, device=dev)
     import ctypes
 
-    ph = _native.DemPhys(float(m), float(eta), s.L, 1, 0)
+    ph = _native.DemPhys(m.item(), eta.item(), s.L, 1, 0)
     p = lambda t: ctypes.c_void_p(t.data_ptr())
-    _native.call("dem_direct_step_vjp", p(xd), p(r), p(F), p(a), s.n, ctypes.byref(ph), float(sf),
+    _native.call("dem_direct_step_vjp", p(xd), p(r), p(F), p(a), s.n, ctypes.byref(ph), sf.item(),
                  p(gx), p(sums), p(ws), ws.numel(), None)
     torch.cuda.synchronize()
     S = sums.cpu().numpy()
@@ -81,13 +81,13 @@ def test_step_vjp_matches_fp64_autograd(gpu):
                          + (a * V64).sum(1).abs().sum()) / m)
         scale_eta = float((a * V64).sum(1).abs().sum() / eta)
         scale_sf = float((aF / r).sum() / sf**2)
-    assert abs(S[0] / float(m) - m.grad.item()) <= 1e-5 * scale_m
-    assert abs(S[1] / float(eta) - eta.grad.item()) <= 1e-5 * scale_eta
-    assert abs(-S[2] / float(sf) - sf.grad.item()) <= 1e-5 * scale_sf
+    assert abs(S[0] / m.item() - m.grad.item()) <= 1e-5 * scale_m
+    assert abs(S[1] / eta.item() - eta.grad.item()) <= 1e-5 * scale_eta
+    assert abs(-S[2] / sf.item() - sf.grad.item()) <= 1e-5 * scale_sf
     # and relative to the value itself, at the trajectory tolerance
-    assert _rel(S[0] / float(m), m.grad.item()) <= 1e-4
-    assert _rel(S[1] / float(eta), eta.grad.item()) <= 1e-4
-    assert _rel(-S[2] / float(sf), sf.grad.item()) <= 1e-4
+    assert _rel(S[0] / m.item(), m.grad.item()) <= 1e-4
+    assert _rel(S[1] / eta.item(), eta.grad.item()) <= 1e-4
+    assert _rel(-S[2] / sf.item(), sf.grad.item()) <= 1e-4
 
 
 def test_contact_loss_matches_fp64(gpu):
