@@ -116,6 +116,59 @@ def hash_pass(s, reps: int = 20) -> dict:
             "note": "includes the host read-back of the pair count (drop-in semantics)"}
 
 
+def step_leg(steps: int = 20) -> dict:
+    """End-to-end DEM step at N = 65536 (phi = 0.02, periodic): the reference
+    loop (contact detection, merges, forces + Oseen velocities, Euler step and
+    wrap) on the device-resident solver, dt = 0.1 r_min / max|v(0)|."""
+    import numpy as np
+    import torch
+
+    from dem.collision_detector import SpatialHashCollisionDetector
+    from dem.force_calculator import DirectDropletForceCalculator
+    from dem.particle_state import DropletState
+    from dem.post_processor import DropletPostProcessor
+    from dem.solution import DropletSolution
+    from dem.solver.device_solver import DeviceEulerDropletSolver
+    from dem.workloads import make_system
+
+    s = make_system(65536, seed=0, phi=0.02)
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                      eta_water=s.eta_water, E=s.E, L=s.L,
+                                      boundary_mode="periodic")
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    vmax = np.max(np.linalg.norm(F / (s.stokes_factor * s.radii[:, None]) + V, axis=1))
+    dt = 0.1 * s.radii.min() / vmax
+    det = SpatialHashCollisionDetector(num_particles=s.n, L=s.L, boundary_mode="periodic")
+
+    def run(nsteps):
+        # a fresh solution per run: like the reference's, a solution compacted at the
+        # end of solve() does not take further steps
+        sol = DropletSolution(DropletState(s.positions.copy(), s.radii.copy(), 0.0), length=8)
+        pp = DropletPostProcessor(sol, box_size=s.L)
+        # save_interval 1 (the reference's default): every step's positions reach the
+        # host trajectory, and merges start from the current positions
+        solver = DeviceEulerDropletSolver(force_calculator=fc, solution=sol, post_processor=pp,
+                                          collision_detector=det, save_interval=1,
+                                          verbose=False)
+        t0 = time.perf_counter()
+        solver.solve(dt, (nsteps - 0.5) * dt)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, solver
+
+    run(2)
+    el, solver = run(steps)
+    k = solver.steps_done
+    ms = el / max(k, 1) * 1e3
+    n = s.n
+    return {"workload": "N=65536, phi=0.02, periodic, seed 0, direct operator, hash every step",
+            "dt_s": dt, "steps": k, "merges": solver.merges, "ms_per_step": ms,
+            "pair_interactions_per_s": n * (n - 1) / (ms * 1e-3),
+            "note": "device-resident loop (DeviceEulerDropletSolver), save_interval 1: contact "
+                    "pass with the pair count read back every step, positions copied to the "
+                    "host trajectory every step, merges on the host in float64 (the "
+                    "reference's merge)"}
+
+
 def pbm_counts(reps: int = 5) -> dict:
     """Population-balance collision counts at N = 65536 (phi = 0.02, periodic,
     30 volume classes): the tree-pruned walk (leaf 16, after a build) and the
@@ -431,8 +484,9 @@ def main() -> int:
                     help="also run a plain 1 GiB device copy (scale for PMC byte counters)")
     ap.add_argument("--one-sided", action="store_true",
                     help="time the one-sided direct kernels instead of the pair-once ones")
-    ap.add_argument("--legs", default="hash,tree,pbm,ensembles,adjoint",
-                    help="comma list of extra measurements: hash, tree, pbm, ensembles, adjoint")
+    ap.add_argument("--legs", default="hash,step,tree,pbm,ensembles,adjoint",
+                    help="comma list of extra measurements: hash, step, tree, pbm, ensembles, "
+                         "adjoint")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
@@ -537,6 +591,8 @@ def main() -> int:
         extras["tree"] = tree_step()
     if "pbm" in legs:
         extras["pbm"] = pbm_counts()
+    if "step" in legs:
+        extras["step"] = step_leg()
     if ens_systems is not None:
         extras["ensembles"] = ensembles_leg(ens_systems, a.ens_steps, a.ens_plain_steps,
                                             tune=not a.no_autotune)
