@@ -699,14 +699,14 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
       if (any) {
         float f[TPL][3] = {};
         const int e = nd.mstart + nd.count;
+        // lanes outside the walk compute too and drop their sums below
         for (int j = nd.mstart; j < e; ++j) {
           const RecF sj = wa.srec[j];
           for (int h = 0; h < TPL; ++h)
-            // tree leaves keep r^2 >= (r_i + r_j)^2 (:835)
-            if (act[h] && j != t[h])
-              dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
-                                     dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3, f[h][0],
-                                     f[h][1], f[h][2]);
+            // tree leaves keep r^2 >= (r_i + r_j)^2 (:835); j == i adds nothing
+            dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
+                                   dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3, j != t[h],
+                                   f[h][0], f[h][1], f[h][2]);
         }
         for (int h = 0; h < TPL; ++h)
           if (act[h])
@@ -799,10 +799,10 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
           const float dy = dem::diff32(sj.y, me.y);
           const float dz = dem::diff32(sj.z, me.z);
           const float r2 = dx * dx + dy * dy + dz * dz;
-          if (j != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r)) {  // >= (:1055)
-            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, true, vx, vy, vz);
-            if (CORR) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
-          }
+          // >= (:1055); j == i and contacts add nothing
+          const bool keep = j != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r);
+          dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, keep, vx, vy, vz);
+          if (CORR && keep) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
         }
         ax += (double)vx;
         ay += (double)vy;

@@ -32,21 +32,30 @@ __device__ __forceinline__ float r2_work(float dx, float dy, float dz) {
   return (dxy2 + dz2) * kSigma2f;
 }
 
-// Dipole force of one source; excluded pairs (contact, j == i) add nothing.
+// Dipole force of one source; excluded pairs (contact, keep == false for the
+// caller's own exclusion such as j == i) add nothing.
 template <bool INCLUSIVE>
 __device__ __forceinline__ void dipole_pair(float dx, float dy, float dz, float ri, float rj,
-                                            float rj3, float& fx, float& fy, float& fz) {
+                                            float rj3, bool keep, float& fx, float& fy,
+                                            float& fz) {
   const float dz2 = dz * dz;
   const float dxy2 = dx * dx + dy * dy;
   const float r2 = (dxy2 + dz2) * kSigma2f;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float ir2 = ir * ir;
   const float ir7 = (ir2 * ir2) * (ir2 * ir);
-  const float s7 = separated<INCLUSIVE>(r2, ri + rj) ? rj3 * ir7 : 0.f;  // select, never 0*inf
+  // select, never 0*inf
+  const float s7 = (keep && separated<INCLUSIVE>(r2, ri + rj)) ? rj3 * ir7 : 0.f;
   const float txy = s7 * (4.f * dz2 - dxy2);
   fx += txy * dx;
   fy += txy * dy;
   fz += (s7 * (2.f * dz2 - 3.f * dxy2)) * dz;
+}
+
+template <bool INCLUSIVE>
+__device__ __forceinline__ void dipole_pair(float dx, float dy, float dz, float ri, float rj,
+                                            float rj3, float& fx, float& fy, float& fz) {
+  dipole_pair<INCLUSIVE>(dx, dy, dz, ri, rj, rj3, true, fx, fy, fz);
 }
 
 // Dipole force of an aggregate (R^3 sum w3 at distance D); no contact rule.
