@@ -70,11 +70,41 @@ struct alignas(16) NodeF {
 };
 static_assert(sizeof(NodeF) == 64, "node force record");
 
+// Sorted source record: fixed-point position, radius and r^3 in work units,
+// and the source force / Fs (set when forces are propagated to the tree).
 struct RecF {
   uint32_t x, y, z;
   float r, r3;
-  uint32_t pad[3];
+  float f[3];
 };
+
+// Two consecutive records, read with one 64-byte load.
+typedef uint32_t RecWords __attribute__((ext_vector_type(16), aligned(4)));
+struct RecF2 {
+  RecF a, b;
+};
+
+__device__ __forceinline__ RecF2 load_two(const RecF* __restrict__ srec, int j) {
+  const RecWords w = *reinterpret_cast<const RecWords*>(srec + j);
+  RecF2 o;
+  o.a.x = w[0];
+  o.a.y = w[1];
+  o.a.z = w[2];
+  o.a.r = __uint_as_float(w[3]);
+  o.a.r3 = __uint_as_float(w[4]);
+  o.a.f[0] = __uint_as_float(w[5]);
+  o.a.f[1] = __uint_as_float(w[6]);
+  o.a.f[2] = __uint_as_float(w[7]);
+  o.b.x = w[8];
+  o.b.y = w[9];
+  o.b.z = w[10];
+  o.b.r = __uint_as_float(w[11]);
+  o.b.r3 = __uint_as_float(w[12]);
+  o.b.f[0] = __uint_as_float(w[13]);
+  o.b.f[1] = __uint_as_float(w[14]);
+  o.b.f[2] = __uint_as_float(w[15]);
+  return o;
+}
 
 struct Scales {
   double origin[3];
@@ -115,8 +145,7 @@ struct TreeLayout {
   int* leaf_indices;             // [N]
   double* spos;                  // [3N] sorted float64 positions
   double* srad;                  // [N]
-  RecF* srec;                    // [N]
-  float* sF;                     // [3N] sorted source forces / Fs
+  RecF* srec;                    // [N + 1]
   double* pforces;               // [3N] forces of the last force walk (original order)
   int* scan_ws;
   int* sort_ws;
@@ -157,8 +186,7 @@ TreeLayout carve(void* ws, int ncap, int M) {
   l.leaf_indices = c.take<int>(N);
   l.spos = c.take<double>(3 * N);
   l.srad = c.take<double>(N);
-  l.srec = c.take<RecF>(N);
-  l.sF = c.take<float>(3 * N);
+  l.srec = c.take<RecF>(N + 1);  // + one spare record (leaf loops read members in twos)
   l.pforces = c.take<double>(3 * N);
   const size_t big = N > m ? N : m;
   l.scan_ws = c.take<int>(dem::scan_workspace_ints((int)big));
@@ -510,6 +538,7 @@ __global__ __launch_bounds__(256) void k_sorted_particles(
     const int* __restrict__ perm, const Scales* __restrict__ scp, double* __restrict__ spos,
     double* __restrict__ srad, RecF* __restrict__ srec) {
   int p = blockIdx.x * 256 + threadIdx.x;
+  if (p == 0) srec[n] = RecF{};  // the spare record after the last
   if (p >= n) return;
   const Scales sc = *scp;
   const int o = perm[p];
@@ -534,7 +563,7 @@ __global__ __launch_bounds__(256) void k_sorted_particles(
   rr.z = q[2];
   rr.r = (float)rl;
   rr.r3 = (float)(rl * rl * rl);
-  rr.pad[0] = rr.pad[1] = rr.pad[2] = 0u;
+  rr.f[0] = rr.f[1] = rr.f[2] = 0.f;
   srec[p] = rr;
 }
 
@@ -607,14 +636,14 @@ __global__ void k_apply_flags(int nn, const int* __restrict__ fl, NodeW* nw) {
   if (i < nn) nw[i].flags = fl[i];
 }
 
-// Sorted source forces / Fs.
+// Source forces / Fs into the sorted records.
 __global__ void k_sorted_forces(int n, const int* __restrict__ perm, const double* __restrict__ F,
-                                const Scales* __restrict__ scp, float* __restrict__ sF) {
+                                const Scales* __restrict__ scp, RecF* __restrict__ srec) {
   int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= n) return;
   const double fs = scp->Fs;
   const int o = perm[p];
-  for (int a = 0; a < 3; ++a) sF[3 * (size_t)p + a] = (float)(F[3 * (size_t)o + a] / fs);
+  for (int a = 0; a < 3; ++a) srec[p].f[a] = (float)(F[3 * (size_t)o + a] / fs);
 }
 
 // ---------------------------------------------------------------- walks
@@ -630,7 +659,6 @@ struct WalkArgs {
   const RecF* srec;
   const double* spos;
   const double* srad;
-  const float* sF;
   const int* perm;
   int n;
   int guard;  // iteration bound of a walk (2 * node_count + 8)
@@ -699,14 +727,20 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
       if (any) {
         float f[TPL][3] = {};
         const int e = nd.mstart + nd.count;
-        // lanes outside the walk compute too and drop their sums below
-        for (int j = nd.mstart; j < e; ++j) {
-          const RecF sj = wa.srec[j];
-          for (int h = 0; h < TPL; ++h)
+        // members in twos, both records in one scalar load (the record
+        // array ends with a spare record); lanes outside the walk compute
+        // too and drop their sums below
+        for (int j = nd.mstart; j < e; j += 2) {
+          const RecF2 two = load_two(wa.srec, j);
+          for (int u = 0; u < 2; ++u) {
+            const RecF& sj = u == 0 ? two.a : two.b;
+            const int jj = j + u;
             // tree leaves keep r^2 >= (r_i + r_j)^2 (:835); j == i adds nothing
-            dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
-                                   dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3, j != t[h],
-                                   f[h][0], f[h][1], f[h][2]);
+            for (int h = 0; h < TPL; ++h)
+              dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
+                                     dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3,
+                                     jj < e && jj != t[h], f[h][0], f[h][1], f[h][2]);
+          }
         }
         for (int h = 0; h < TPL; ++h)
           if (act[h])
@@ -789,20 +823,25 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       if (act) {
         float vx = 0.f, vy = 0.f, vz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
         const int e = nd.mstart + nd.count;
-        for (int j = nd.mstart; j < e; ++j) {
-          // wave-uniform loads first (scalar), then the lane's own tests
-          const RecF sj = wa.srec[j];
-          const float Fx = wa.sF[3 * (size_t)j], Fy = wa.sF[3 * (size_t)j + 1],
-                      Fz = wa.sF[3 * (size_t)j + 2];
-          // D = x_j - x_i (fixed units); the Oseen tensor is even in D
-          const float dx = dem::diff32(sj.x, me.x);
-          const float dy = dem::diff32(sj.y, me.y);
-          const float dz = dem::diff32(sj.z, me.z);
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          // >= (:1055); j == i and contacts add nothing
-          const bool keep = j != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r);
-          dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, keep, vx, vy, vz);
-          if (CORR && keep) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+        for (int j = nd.mstart; j < e; j += 2) {
+          // members in twos: one wave-uniform (scalar) load of both records
+          // (the array ends with a spare record), then the lane's tests
+          const RecF2 two = load_two(wa.srec, j);
+          for (int u = 0; u < 2; ++u) {
+            const RecF& sj = u == 0 ? two.a : two.b;
+            const int jj = j + u;
+            const float Fx = sj.f[0], Fy = sj.f[1], Fz = sj.f[2];
+            // D = x_j - x_i (fixed units); the Oseen tensor is even in D
+            const float dx = dem::diff32(sj.x, me.x);
+            const float dy = dem::diff32(sj.y, me.y);
+            const float dz = dem::diff32(sj.z, me.z);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            // >= (:1055); j == i, contacts and the spare past the leaf add nothing
+            const bool keep =
+                jj < e && jj != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r);
+            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, keep, vx, vy, vz);
+            if (CORR && keep) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
+          }
         }
         ax += (double)vx;
         ay += (double)vy;
@@ -1010,7 +1049,7 @@ int propagate_forces(const dem_tree_info* info, const TreeLayout& l, const doubl
                      l.nw);
   DEM_CHECK_LAUNCH("k_apply_flags");
   hipLaunchKernelGGL(k_sorted_forces, dim3(nb256(info->n)), dim3(256), 0, st, info->n,
-                     (const int*)l.perm, F, (const Scales*)l.sc, l.sF);
+                     (const int*)l.perm, F, (const Scales*)l.sc, l.srec);
   DEM_CHECK_LAUNCH("k_sorted_forces");
   return DEM_OK;
 }
@@ -1022,7 +1061,6 @@ WalkArgs walk_args(const dem_tree_info* info, const TreeLayout& l, double theta_
   w.srec = l.srec;
   w.spos = l.spos;
   w.srad = l.srad;
-  w.sF = l.sF;
   w.perm = l.perm;
   w.n = info->n;
   w.guard = 2 * info->node_count + 8;
