@@ -58,6 +58,7 @@ struct Layout {
 };
 
 constexpr int kMaxBlocks = 256;
+constexpr int kCountSplit = 4;  // lanes per droplet in the count pass
 
 Layout carve(void* ws, int n, int T) {
   dem::Carver c(ws);
@@ -93,11 +94,14 @@ __global__ __launch_bounds__(256) void k_rmax_partial(const double* __restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// One wave: the largest radius over the partial maxima (fmax is exact and
+// order-free), then the cell size and grid.
 __global__ void k_hash_setup(const double* __restrict__ part, int nblk, double L,
                              HashParams* __restrict__ hp) {
-  if (threadIdx.x != 0) return;
   double m = -INFINITY;
-  for (int b = 0; b < nblk; ++b) m = fmax(m, part[b]);
+  for (int b = threadIdx.x; b < nblk; b += 64) m = fmax(m, part[b]);
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (threadIdx.x != 0) return;
   HashParams p;
   p.cell = 2.0 * m;
   p.skip = !(p.cell > 0.0);
@@ -137,12 +141,15 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const HashParams* __rest
                             r[i]);
 }
 
-// EMIT = false: count pairs of each target; EMIT = true: write them.  Thread
-// t takes the droplet in bucket slot t (neighbouring threads then search the
-// same buckets).  The distinct neighbour buckets' ranges are loaded up front
-// and candidates come from the bucket-ordered (x, y, z, r) copy; the droplet
-// index of a slot is read only for a contact.
-template <bool EMIT>
+// EMIT = false: count pairs of each target; EMIT = true: write them.  The
+// droplet in bucket slot t is searched by SPLIT consecutive lanes (neighbouring
+// slots then search the same buckets), lane `sub` taking the distinct
+// neighbour buckets k = sub, sub + SPLIT, ...; their counts are summed across
+// the lanes.  The buckets' ranges are loaded up front and candidates come from
+// the bucket-ordered (x, y, z, r) copy; the droplet index of a slot is read
+// only for a contact.  The emit pass (SPLIT = 1) skips droplets the count
+// pass found without pairs.
+template <bool EMIT, int SPLIT>
 __global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, int mask,
                                                const HashParams* __restrict__ hpp,
                                                const int* __restrict__ counts,
@@ -152,15 +159,19 @@ __global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, in
                                                int* __restrict__ pair_cnt,
                                                const int* __restrict__ pair_off,
                                                int* __restrict__ pairs, int cap) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= n) return;
+  static_assert(!EMIT || SPLIT == 1, "the emit pass writes a target's pairs from one lane");
+  static_assert(SPLIT == 1 || SPLIT == 2 || SPLIT == 4, "lanes per target");
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int t = gid / SPLIT, sub = gid % SPLIT;
+  if (t >= n) return;  // the SPLIT lanes of a target leave together
   const HashParams hp = *hpp;
   if (hp.skip) {  // nothing was bucketed: slot t stands for droplet t
-    if (!EMIT) pair_cnt[t] = 0;
+    if (!EMIT && sub == 0) pair_cnt[t] = 0;
     return;
   }
   const int g = hp.g;
   const int i = sorted[t];
+  if (EMIT && pair_cnt[i] == 0) return;
   const double4 me = spos[t];
   const int ix = cell_of(me.x, hp.cell, g, periodic);
   const int iy = cell_of(me.y, hp.cell, g, periodic);
@@ -186,18 +197,21 @@ __global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, in
       if (!dup) nb[nseen++] = nh;
     }
   }
-  int s0[27], s1[27];
+  constexpr int KS = (27 + SPLIT - 1) / SPLIT;
+  int s0[KS], s1[KS];
 #pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    const int h = k < nseen ? nb[k] : 0;
-    s0[k] = offsets[h];
-    s1[k] = k < nseen ? s0[k] + counts[h] : s0[k];
+  for (int u = 0; u < KS; ++u) {
+    const int k = sub + SPLIT * u;
+    const bool ok = k < nseen;
+    const int h = ok ? nb[k < 27 ? k : 0] : 0;
+    s0[u] = offsets[h];
+    s1[u] = ok ? s0[u] + counts[h] : s0[u];
   }
   int cnt = 0;
   const int base = EMIT ? pair_off[i] : 0;
 #pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    for (int slot = s0[k]; slot < s1[k]; ++slot) {
+  for (int u = 0; u < KS; ++u) {
+    for (int slot = s0[u]; slot < s1[u]; ++slot) {
       const double4 q = spos[slot];
       if (contact_pair(me.x, me.y, me.z, me.w, q.x, q.y, q.z, q.w, L, periodic)) {
         const int j = sorted[slot];
@@ -215,7 +229,8 @@ __global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, in
     }
   }
   if (!EMIT) {
-    pair_cnt[i] = cnt;
+    for (int o = 1; o < SPLIT; o <<= 1) cnt += __shfl_xor(cnt, o);
+    if (sub == 0) pair_cnt[i] = cnt;
   } else if (cnt > 1) {
     // In-segment insertion sort by j (segments hold a handful of pairs).
     const int lo = base, hi = min(base + cnt, cap);
@@ -293,14 +308,15 @@ int dem_hash_pairs(const double* pos, const double* radii, int n, double L, int 
   hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, (const HashParams*)l.hp,
                      (const int*)l.bucket, pos, radii, l.cursor, l.sorted, l.spos);
   DEM_CHECK_LAUNCH("k_scatter");
-  hipLaunchKernelGGL(k_pairs<false>, dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
+  hipLaunchKernelGGL((k_pairs<false, kCountSplit>), dim3((int)(((long long)n * kCountSplit + 255) / 256)),
+                     dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
                      (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
                      (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
                      (const int*)l.pair_off, pairs, pair_capacity);
   DEM_CHECK_LAUNCH("k_pairs<count>");
   rc = dem::exclusive_scan(l.pair_cnt, l.pair_off, n, l.scan_ws, count_out, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pairs<true>, dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
+  hipLaunchKernelGGL((k_pairs<true, 1>), dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
                      (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
                      (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
                      (const int*)l.pair_off, pairs, pair_capacity);
