@@ -63,6 +63,7 @@ using dem::Rec;
 using dem::RecC;
 
 constexpr int kRow = 256;       // droplets per row
+constexpr int kSearchSplit = 4;  // lanes per droplet in the contact count pass
 constexpr int kScanWG = 1024;   // threads of a per-system scan workgroup
 constexpr int kZeroBlocks = 1024;
 
@@ -584,30 +585,85 @@ __device__ int search_target(const BEns& E, const BDev& D, const double* __restr
   return cnt;
 }
 
+// Count pass of the droplet in bucket slot `slot` (neighbouring slots share
+// neighbour buckets): lane `sub` of SPLIT takes the distinct neighbour buckets
+// k = sub, sub + SPLIT, ...; the candidate's droplet index is read only for a
+// contact.  Returns this lane's share of the droplet's pairs (i, j), i < j.
+template <int SPLIT>
+__device__ int count_slot(const BEns& E, const BDev& D, const int* __restrict__ start,
+                          const int* __restrict__ sorted, const double4* __restrict__ spos,
+                          int slot, int sub, int* i_out) {
+  const int mask = E.T - 1, g = D.g;
+  const double4 me = spos[slot];
+  const int i = sorted[slot];
+  *i_out = i;
+  const int ix = dem::cell_of(me.x, D.cell, g, 1);
+  const int iy = dem::cell_of(me.y, D.cell, g, 1);
+  const int iz = dem::cell_of(me.z, D.cell, g, 1);
+  int nb[27];
+  int nseen = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    const int di = k / 9 - 1, dj = (k / 3) % 3 - 1, dk = k % 3 - 1;
+    const int nh = dem::hash_cell(dem::pymod(ix + di, g), dem::pymod(iy + dj, g),
+                                  dem::pymod(iz + dk, g), mask);
+    bool dup = false;
+#pragma unroll
+    for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
+    if (!dup) nb[nseen++] = nh;
+  }
+  constexpr int KS = (27 + SPLIT - 1) / SPLIT;
+  int s0[KS], s1[KS];
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int k = sub + SPLIT * u;
+    const bool ok = k < nseen;
+    const int h = ok ? nb[k < 27 ? k : 0] : 0;
+    s0[u] = start[h];
+    s1[u] = ok ? start[h + 1] : s0[u];
+  }
+  int cnt = 0;
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    for (int q = s0[u]; q < s1[u]; ++q) {
+      const double4 c = spos[q];
+      if (dem::contact_pair(me.x, me.y, me.z, me.w, c.x, c.y, c.z, c.w, E.L, 1) &&
+          i < sorted[q])
+        ++cnt;
+    }
+  }
+  return cnt;
+}
+
+// Per-droplet pair counts pc (original order) and per-system totals.  A row
+// of kRow droplets is searched by kSearchSplit workgroups, kSearchSplit lanes
+// per droplet, in bucket-slot order.
 __global__ __launch_bounds__(kRow) void k_b_search(const BEns* __restrict__ ens,
                                                    const BDev* __restrict__ dev,
                                                    const int2* __restrict__ rows,
                                                    const BHead* __restrict__ head,
                                                    const int* __restrict__ active,
-                                                   const double* __restrict__ pos,
-                                                   const double* __restrict__ radii,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ sorted,
                                                    const double4* __restrict__ spos,
                                                    int* __restrict__ pc,
                                                    int* __restrict__ pair_count) {
   __shared__ int s_sum[kRow / 64];
-  if ((int)blockIdx.x >= head->n_rows) return;
-  const int2 row = rows[blockIdx.x];
+  const int rb = (int)blockIdx.x / kSearchSplit, part = (int)blockIdx.x % kSearchSplit;
+  if (rb >= head->n_rows) return;
+  const int2 row = rows[rb];
   if (!hashed(head, active, dev, row.x)) return;  // uniform over the block
   const BEns& E = ens[row.x];
-  const int i = row.y + threadIdx.x;
+  const int sub = threadIdx.x % kSearchSplit;
+  const int slot = row.y + part * (kRow / kSearchSplit) + (int)threadIdx.x / kSearchSplit;
   int c = 0;
-  if (i < E.n) {
-    c = search_target<false>(E, dev[row.x], pos + 3 * (size_t)E.off, radii + E.off,
-                             start + E.start_off, sorted + E.off, spos + E.off, i, nullptr,
-                             nullptr, 0);
-    pc[(size_t)E.off + i] = c;
+  if (slot < E.n) {  // the kSearchSplit lanes of a droplet are all here or all not
+    int i = 0;
+    c = count_slot<kSearchSplit>(E, dev[row.x], start + E.start_off, sorted + E.off,
+                                 spos + E.off, slot, sub, &i);
+    int tot = c;
+    for (int o = 1; o < kSearchSplit; o <<= 1) tot += __shfl_xor(tot, o);
+    if (sub == 0) pc[(size_t)E.off + i] = tot;
   }
   for (int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
   if ((threadIdx.x & 63) == 0) s_sum[threadIdx.x >> 6] = c;
@@ -1033,11 +1089,10 @@ int enqueue_check(dem_batch* b, hipStream_t st, int new_step) {
                      (const int*)l.rank, (const double*)b->pos, (const double*)b->radii, l.sorted,
                      l.spos);
   DEM_CHECK_LAUNCH("k_b_scatter");
-  hipLaunchKernelGGL(k_b_search, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
-                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.active, (const double*)b->pos, (const double*)b->radii,
-                     (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos, l.pc,
-                     l.pair_count);
+  hipLaunchKernelGGL(k_b_search, dim3(gr * kSearchSplit), dim3(kRow), 0, st,
+                     (const BEns*)l.ens, (const BDev*)l.dev, (const int2*)l.rows,
+                     (const BHead*)l.head, (const int*)l.active, (const int*)l.start,
+                     (const int*)l.sorted, (const double4*)l.spos, l.pc, l.pair_count);
   DEM_CHECK_LAUNCH("k_b_search");
   hipLaunchKernelGGL(k_b_pair_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
                      (const BHead*)l.head, (const int*)l.pair_count, (const int*)l.pc, l.poff);
