@@ -165,8 +165,9 @@ def step_leg(steps: int = 20) -> dict:
             "pair_interactions_per_s": n * (n - 1) / (ms * 1e-3),
             "note": "device-resident loop (DeviceEulerDropletSolver), save_interval 1: contact "
                     "pass with the pair count read back every step, positions copied to the "
-                    "host trajectory every step, merges on the host in float64 (the "
-                    "reference's merge)"}
+                    "host trajectory every step (pinned staging); merge plans on the host "
+                    "in float64 (the reference's greedy arithmetic), compaction on the "
+                    "device (dem_merge_compact)"}
 
 
 def pbm_counts(reps: int = 5) -> dict:

@@ -190,6 +190,18 @@ int dem_euler_step(double* pos, const double* forces, const double* conv,
 int dem_euler_step_total(double* pos, const double* vel, int n, double dt,
                          double L, int periodic, dem_stream_t stream);
 
+/* Volume-conserving merge, compaction step (droplet_solution.py:263-268,
+ * merge_collided's np.delete / np.append after the greedy plan): survivors of
+ * (pos [n][3], radii [n]) keep their order, then the n_merged merged droplets
+ * (merged [n_merged][4] = x, y, z, r) are appended, into out_pos
+ * [n - n_removed + n_merged][3] and out_radii (distinct from the inputs).
+ * removed: the n_removed distinct removed indices, ascending (device).
+ * All pointers are device pointers. */
+int dem_merge_compact(const double* pos, const double* radii, int n,
+                      const int32_t* removed, int n_removed, const double* merged,
+                      int n_merged, double* out_pos, double* out_radii,
+                      dem_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Flat Barnes-Hut octree                                                     */
 /* ------------------------------------------------------------------------ */

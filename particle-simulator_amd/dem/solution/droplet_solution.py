@@ -216,9 +216,15 @@ class DropletSolution(Solution):
 
     def generate_next_solution(self):
         """Merge the collided pairs and start the next trajectory segment."""
-        new_positions, new_radii, self.collided_droplets = merge_collided(
+        new_positions, new_radii, collided = merge_collided(
             self.trajectories[self.current_step], self.radii, self.collided_droplets)
+        return self.next_solution(new_positions, new_radii, collided)
 
+    def next_solution(self, new_positions, new_radii, collided):
+        """The second half of ``generate_next_solution``: start the next
+        trajectory segment from already merged arrays (``merge_collided``'s
+        outputs, or the same values compacted on the device)."""
+        self.collided_droplets = collided
         new_initial_state = DropletState(new_positions, new_radii, self.times[self.current_step])
         next_solution = DropletSolution(
             initial_droplet_state=new_initial_state,

@@ -8,13 +8,21 @@ wrap — but positions and radii live in device memory between steps:
 
 * contact detection: ``dem_hash_pairs`` on the device arrays; only the pair
   count crosses to the host each step;
-* merge (rare): the pairs and the state are copied to the host and merged by
-  ``DropletSolution.generate_next_solution`` — the reference's float64 numpy
-  merge, bit-exact — then copied back;
+* merge: the reference merges from the last *saved* positions
+  (``trajectories[current_step]``).  When those equal the device state (always
+  with save_interval 1), the greedy plan runs on the host over the pairs only
+  (``merge_plan``, the reference's float64 arithmetic) and
+  ``dem_merge_compact`` compacts the state on the device (survivors in order,
+  merged droplets appended), which stays there; otherwise the state is merged
+  on the host by ``DropletSolution.generate_next_solution`` and uploaded.
+  Either way the result is ``merge_collided``'s, bit for bit, and the host
+  copy of the merged state starts the next solution segment;
 * forces + velocities: the operator's ``*_device`` entry point;
 * Euler update and wrap: ``dem_euler_step`` / ``dem_euler_step_total``,
   bit-exact with the reference's numpy update;
-* ``solution.save_step`` receives a host copy only on saved steps.
+* ``solution.save_step`` receives a host copy only on saved steps; the
+  solution's ``initial_droplet_state.positions``, which the reference's loop
+  advances in place, receives the state at the end of each segment.
 
 Given identical operator outputs, trajectories are bit-identical to
 ``EulerDropletSolver``'s.
@@ -27,6 +35,7 @@ import time
 import numpy as np
 
 from .. import _device, _native
+from ..solution.droplet_solution import merge_plan
 
 
 class DeviceEulerDropletSolver:
@@ -44,11 +53,56 @@ class DeviceEulerDropletSolver:
         self.stokes_factor = 2 * np.pi * eo * (2 * eo + 3 * ew) / (eo + ew)
         self.steps_done = 0
         self.merges = 0
+        self._pin = None  # pinned staging for device -> host copies
 
     def _upload(self):
         st = self.solution.initial_droplet_state
         return (_device.to_device(np.asarray(st.positions, dtype=np.float64).reshape(-1, 3)),
                 _device.to_device(np.asarray(st.radii, dtype=np.float64)))
+
+    def _merge_on_device(self, sol, pairs, saved, pos_d, r_d):
+        """``sol.generate_next_solution()`` with the compaction on the device;
+        ``saved`` (the saved positions the reference merges from) equals the
+        device state.  Returns the next solution and its device state."""
+        radii = sol.radii
+        removed, new_r, new_p, collided = merge_plan(pairs, lambda i: radii[i],
+                                                     lambda i: saved[i])
+        rem = np.asarray(sorted(removed), dtype=np.int32)
+        m = len(new_r)
+        merged = np.empty((m, 4), dtype=np.float64)
+        if m:
+            merged[:, :3] = np.asarray(new_p, dtype=np.float64).reshape(m, 3)
+            merged[:, 3] = np.asarray(new_r, dtype=np.float64)
+        n = int(pos_d.shape[0])
+        n_out = n - len(rem) + m
+        rem_d = _device.to_device(rem, np.int32) if len(rem) else None
+        merged_d = _device.to_device(merged) if m else None
+        out_p = _device.empty((n_out, 3))
+        out_r = _device.empty((n_out,))
+        _native.call("dem_merge_compact", _device.ptr(pos_d), _device.ptr(r_d), n,
+                     _device.ptr(rem_d), len(rem), _device.ptr(merged_d), m, _device.ptr(out_p),
+                     _device.ptr(out_r), _device.stream())
+        new_p_h = np.array(self._staged(out_p))
+        new_r_h = np.array(self._staged(out_r))
+        return sol.next_solution(new_p_h, new_r_h, collided), (out_p, out_r)
+
+    def _staged(self, d):
+        """Device float64 tensor -> numpy view of a pinned staging buffer (one
+        copy and one stream sync); valid until the next call."""
+        t = _device.torch()
+        k = d.numel()
+        if self._pin is None or self._pin.numel() < k:
+            self._pin = t.empty(max(k, 1 << 16), dtype=t.float64, pin_memory=True)
+        v = self._pin[:k].view(d.shape)
+        v.copy_(d, non_blocking=True)
+        t.cuda.current_stream().synchronize()
+        return v.numpy()
+
+    def _write_back(self, cur):
+        """The reference advances ``initial_droplet_state.positions`` in place."""
+        st = self.solution.initial_droplet_state
+        if np.shape(st.positions) == cur.shape:
+            st.positions[...] = cur
 
     def _forces(self, pos_d, r_d):
         fc = self.force_calculator
@@ -65,10 +119,20 @@ class DeviceEulerDropletSolver:
         is_aborted = False
         periodic = fc.boundary_mode == "periodic"
         L = float(fc.L)
+        carried = None  # device state of the new segment after a device merge
+        pos_d = None
         while t < t_end and not is_aborted:
             if self.post_processor.stop_simulation:
                 break
-            pos_d, r_d = self._upload()
+            if carried is not None:
+                pos_d, r_d = carried
+                carried = None
+                saved_is_current = True  # the new segment saved its initial state
+            else:
+                pos_d, r_d = self._upload()
+                sol = self.solution
+                saved_is_current = np.array_equal(sol.trajectories[sol.current_step],
+                                                  sol.initial_droplet_state.positions)
             radii_host = self.solution.initial_droplet_state.radii
             n = int(pos_d.shape[0])
             remaining = int((t_end - t) / dt) + 1
@@ -86,8 +150,7 @@ class DeviceEulerDropletSolver:
                     k, pairs_d = det.detect_device(pos_d, r_d, det._L, det._boundary_mode == "periodic")
                     if k > 0:
                         # as in the reference, the merge starts from the last *saved*
-                        # positions (trajectories[current_step]); with save_interval = 1
-                        # those are the current ones
+                        # positions (trajectories[current_step])
                         pairs = pairs_d.cpu().numpy()
                         if self.pbm_coupling is not None:
                             self.pbm_coupling.on_collision(pairs, radii_host)
@@ -95,8 +158,18 @@ class DeviceEulerDropletSolver:
                         sol.is_collision = True
                         sol.collided_droplets = pairs
                         sol.compact()
-                        self.solution = sol.generate_next_solution()
+                        saved = sol.trajectories[sol.current_step]
+                        cur = saved if saved_is_current else self._staged(pos_d)
+                        self._write_back(cur)
+                        if saved_is_current or (saved.shape == cur.shape
+                                                and np.array_equal(saved, cur)):
+                            self.solution, carried = self._merge_on_device(sol, pairs, saved,
+                                                                           pos_d, r_d)
+                        else:
+                            self.solution = sol.generate_next_solution()
                         self.post_processor.update_solution(self.solution)
+                        if self.solution.real_time_visualization:
+                            self.post_processor.update_live_plot()
                         self.merges += 1
                         break
                 kind, (F, V) = self._forces(pos_d, r_d)
@@ -111,11 +184,14 @@ class DeviceEulerDropletSolver:
                     self.pbm_coupling.step(t + dt, radii_host, dt)
                 t += dt
                 self.steps_done += 1
-                if (self.save_interval <= 1 or step % self.save_interval == 0
-                        or step == remaining - 1):
-                    self.solution.save_step(t, pos_d.cpu().numpy())
+                saved_is_current = (self.save_interval <= 1 or step % self.save_interval == 0
+                                    or step == remaining - 1)
+                if saved_is_current:
+                    self.solution.save_step(t, self._staged(pos_d))
                 if self.solution.should_update_visualization():
                     self.post_processor.live_plot()
+        if pos_d is not None and carried is None:
+            self._write_back(self._staged(pos_d))
         self.solution.compact_all()
         self.simulation_time = time.time() - time_start
         if self.verbose:

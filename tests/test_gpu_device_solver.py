@@ -1,5 +1,8 @@
 """The device-resident Euler loop reproduces the host loop bit for bit
-(same operators, same merge, bit-exact Euler kernel), merges included."""
+(same operators, same merge, bit-exact Euler kernel), merges included: with
+save_interval 1 the merges are compacted on the device
+(``dem_merge_compact``), with a longer interval the reference's merge from
+stale saved positions runs on the host."""
 
 from __future__ import annotations
 
@@ -19,7 +22,7 @@ from dem.workloads import make_system
 pytestmark = pytest.mark.gpu
 
 
-def _run(solver_cls, s, tree=False, steps=12, dt=None):
+def _run(solver_cls, s, tree=False, steps=12, dt=None, save_interval=1):
     if tree:
         fc = TreeDropletForceCalculator(num_particles=s.n, theta=0.5, mpl=4, eps_oil=s.eps_oil,
                                         eta_oil=s.eta_oil, eta_water=s.eta_water, E=s.E, L=s.L,
@@ -33,7 +36,7 @@ def _run(solver_cls, s, tree=False, steps=12, dt=None):
     pp = DropletPostProcessor(sol, box_size=s.L)
     kw = {"verbose": False} if solver_cls is DeviceEulerDropletSolver else {}
     solver = solver_cls(force_calculator=fc, solution=sol, post_processor=pp,
-                        collision_detector=det, **kw)
+                        collision_detector=det, save_interval=save_interval, **kw)
     solver.solve(dt, dt * steps)
     return solver
 
@@ -42,25 +45,68 @@ def _chain(sol):
     out = []
     cur = sol.get_first_solution()
     while cur is not None:
-        out.append((cur.radii.copy(), cur.get_trajectories().copy(), cur.get_times().copy()))
+        out.append((cur.radii.copy(), cur.get_trajectories().copy(), cur.get_times().copy(),
+                    np.array(cur.initial_droplet_state.positions, copy=True)))
         cur = cur._next
     return out
 
 
-@pytest.mark.parametrize("tree", [False, True])
-def test_device_loop_is_bit_identical_to_host_loop(gpu, tree):
-    s = make_system(1500, seed=4, phi=0.04)
+def _dt(s):
     # a velocity scale from the first evaluation: dt moves droplets ~0.5 r_min
     fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
                                       eta_water=s.eta_water, E=s.E, L=s.L)
     F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
     vmax = np.max(np.linalg.norm(F / (s.stokes_factor * s.radii[:, None]) + V, axis=1))
-    dt = 0.5 * s.radii.min() / vmax
-    a = _run(EulerDropletSolver, s, tree, dt=dt)
-    b = _run(DeviceEulerDropletSolver, s, tree, dt=dt)
+    return 0.5 * s.radii.min() / vmax
+
+
+@pytest.mark.parametrize("tree,save_interval", [(False, 1), (True, 1), (False, 3)])
+def test_device_loop_is_bit_identical_to_host_loop(gpu, tree, save_interval):
+    s = make_system(1500, seed=4, phi=0.04)
+    dt = _dt(s)
+    a = _run(EulerDropletSolver, s, tree, dt=dt, save_interval=save_interval)
+    b = _run(DeviceEulerDropletSolver, s, tree, dt=dt, save_interval=save_interval)
     ca, cb = _chain(a.solution), _chain(b.solution)
     assert len(ca) == len(cb) and len(ca) >= 2, "the run should contain merges"
-    for (ra, ta, ha), (rb, tb, hb) in zip(ca, cb):
+    for (ra, ta, ha, pa), (rb, tb, hb, pb) in zip(ca, cb):
         assert np.array_equal(ra, rb)
         assert np.array_equal(ta, tb)
         assert np.array_equal(ha, hb)
+        # the reference advances each segment's initial positions in place
+        assert np.array_equal(pa, pb)
+
+
+def test_merge_compact_matches_merge_collided(gpu):
+    import torch
+
+    from dem import _device, _native
+    from dem.solution.droplet_solution import merge_collided, merge_plan
+
+    rng = np.random.default_rng(7)
+    for n, k in ((2, 1), (50, 6), (4000, 300)):
+        pos = rng.random((n, 3))
+        r = rng.uniform(1e-3, 2e-3, n)
+        # random pairs with repeats and droplets in several pairs (greedy skips)
+        a = rng.integers(0, n, k)
+        b = (a + 1 + rng.integers(0, max(n - 1, 1), k)) % n
+        pairs = np.unique(np.sort(np.stack([a, b], 1), axis=1), axis=0).astype(np.int32)
+        pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+        ref_p, ref_r, ref_c = merge_collided(pos, r, pairs)
+        removed, new_r, new_p, coll = merge_plan(pairs, lambda i: r[i], lambda i: pos[i])
+        np.testing.assert_array_equal(coll, ref_c)
+        rem = np.asarray(sorted(removed), dtype=np.int32)
+        m = len(new_r)
+        merged = np.column_stack([np.asarray(new_p).reshape(m, 3), new_r])
+        n_out = n - len(rem) + m
+        out_p = torch.full((n_out, 3), np.nan, dtype=torch.float64, device="cuda")
+        out_r = torch.full((n_out,), np.nan, dtype=torch.float64, device="cuda")
+        # every device buffer stays referenced until the results are read: a
+        # tensor freed right after its pointer is taken can be handed to the
+        # next allocation before the kernel runs
+        pd, rd = _device.to_device(pos), _device.to_device(r)
+        rem_d, merged_d = _device.to_device(rem, np.int32), _device.to_device(merged)
+        _native.call("dem_merge_compact", _device.ptr(pd), _device.ptr(rd), n,
+                     _device.ptr(rem_d), len(rem), _device.ptr(merged_d), m, _device.ptr(out_p),
+                     _device.ptr(out_r), _device.stream())
+        assert np.array_equal(out_p.cpu().numpy(), ref_p)
+        assert np.array_equal(out_r.cpu().numpy(), ref_r)
