@@ -62,13 +62,41 @@ struct alignas(16) NodeW {
 static_assert(sizeof(NodeW) == 64, "node record");
 
 // Per-node Stokeslet data: force sum / Fs and the centred dipole tensor
-// D_ab = P_ab - cm_a F_b divided by Fs*U (fixed units).
+// D_ab = P_ab - cm_a F_b divided by Fs*U (fixed units), kept as the parts the
+// far-field velocity uses (formed in float64): trace, antisymmetric part
+// a_xy = D_yx - D_xy, a_xz = D_zx - D_xz, a_yz = D_zy - D_yz, and the
+// symmetric quadratic form q(n) = sum_ab D_ab n_a n_b as diagonal and doubled
+// off-diagonal coefficients.
 struct alignas(16) NodeF {
-  float fx, fy, fz, pad;
-  float d[9];
-  float pad2[3];
+  float fx, fy, fz, tr;
+  float axy, axz, ayz;
+  float sxx, syy, szz, sxy2, sxz2, syz2;
+  float pad[3];
 };
 static_assert(sizeof(NodeF) == 64, "node force record");
+
+// Far-field velocity of a node at a target (FP32, fixed units): the Oseen
+// monopole of the force sum and the dipole correction (flat_tree.py:
+// 1242-1265) for r = x_i - centre = -R, written with the unit vector
+// n = r / |r| to stay inside the FP32 range:
+//   v += |r|^-2 ((D^T - D) n - tr(D) n + 3 n (n.D n)).
+// Outside the walk kernels so that FP contraction applies.
+__device__ __forceinline__ void node_velocity(float Rx, float Ry, float Rz, const NodeF& nf,
+                                              float& vx, float& vy, float& vz) {
+  dem::oseen_pair(Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, true, vx, vy, vz);
+  const float ir = __builtin_amdgcn_rsqf(Rx * Rx + Ry * Ry + Rz * Rz);
+  const float ir2 = ir * ir;
+  const float nx = -Rx * ir, ny = -Ry * ir, nz = -Rz * ir;
+  const float ex = nf.axy * ny + nf.axz * nz;
+  const float ey = nf.ayz * nz - nf.axy * nx;
+  const float ez = -(nf.axz * nx + nf.ayz * ny);
+  const float q = nf.sxx * nx * nx + nf.syy * ny * ny + nf.szz * nz * nz +
+                  nf.sxy2 * nx * ny + nf.sxz2 * nx * nz + nf.syz2 * ny * nz;
+  const float q3 = 3.f * q;
+  vx += ir2 * ((ex - nf.tr * nx) + q3 * nx);
+  vy += ir2 * ((ey - nf.tr * ny) + q3 * ny);
+  vz += ir2 * ((ez - nf.tr * nz) + q3 * nz);
+}
 
 // Sorted source record: fixed-point position, radius and r^3 in work units,
 // and the source force / Fs (set when forces are propagated to the tree).
@@ -618,13 +646,23 @@ __global__ void k_node_force(int nn, const double* __restrict__ fsum, const doub
   o.fx = (float)(Fx / sc.Fs);
   o.fy = (float)(Fy / sc.Fs);
   o.fz = (float)(Fz / sc.Fs);
-  o.pad = 0.f;
   const double cm[3] = {w.cx, w.cy, w.cz};
   const double F[3] = {Fx, Fy, Fz};
   const double k = 1.0 / (sc.Fs * sc.U);
+  double D[9];
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) o.d[3 * a + b] = (float)((fmom[9 * (size_t)i + 3 * a + b] - cm[a] * F[b]) * k);
-  o.pad2[0] = o.pad2[1] = o.pad2[2] = 0.f;
+    for (int b = 0; b < 3; ++b) D[3 * a + b] = (fmom[9 * (size_t)i + 3 * a + b] - cm[a] * F[b]) * k;
+  o.tr = (float)(D[0] + D[4] + D[8]);
+  o.axy = (float)(D[3] - D[1]);
+  o.axz = (float)(D[6] - D[2]);
+  o.ayz = (float)(D[7] - D[5]);
+  o.sxx = (float)D[0];
+  o.syy = (float)D[4];
+  o.szz = (float)D[8];
+  o.sxy2 = (float)(D[1] + D[3]);
+  o.sxz2 = (float)(D[2] + D[6]);
+  o.syz2 = (float)(D[5] + D[7]);
+  o.pad[0] = o.pad[1] = o.pad[2] = 0.f;
   nf[i] = o;
   int fl = w.flags & ~kSkipF;
   if (w.count == 0 || fs2 < kFsqSkip) fl |= kSkipF;
@@ -869,24 +907,7 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
                   Rz = (float)(-dz * sc.inv_U);
       float vx = 0.f, vy = 0.f, vz = 0.f;
-      dem::oseen_pair(Rx, Ry, Rz, nf.fx, nf.fy, nf.fz, true, vx, vy, vz);
-      const float ir = __builtin_amdgcn_rsqf(Rx * Rx + Ry * Ry + Rz * Rz);
-      const float ir2 = ir * ir;
-      // dipole correction with r = x_i - centre = -R (:1242-1265), written with
-      // the unit vector n = r / |r| to stay inside the FP32 range
-      const float nx = -Rx * ir, ny = -Ry * ir, nz = -Rz * ir;
-      const float* D = nf.d;
-      const float ux = D[0] * nx + D[3] * ny + D[6] * nz;
-      const float uy = D[1] * nx + D[4] * ny + D[7] * nz;
-      const float uz = D[2] * nx + D[5] * ny + D[8] * nz;
-      const float wx = D[0] * nx + D[1] * ny + D[2] * nz;
-      const float wy = D[3] * nx + D[4] * ny + D[5] * nz;
-      const float wz = D[6] * nx + D[7] * ny + D[8] * nz;
-      const float str = D[0] + D[4] + D[8];
-      const float q = nx * wx + ny * wy + nz * wz;
-      vx += ir2 * ((ux - wx - str * nx) + 3.f * nx * q);
-      vy += ir2 * ((uy - wy - str * ny) + 3.f * ny * q);
-      vz += ir2 * ((uz - wz - str * nz) + 3.f * nz * q);
+      node_velocity(Rx, Ry, Rz, nf, vx, vy, vz);
       ax += (double)vx;
       ay += (double)vy;
       az += (double)vz;
