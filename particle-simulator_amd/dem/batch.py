@@ -267,12 +267,25 @@ class BatchedEnsembles:
             pairs_h = self.pairs[:total].cpu().numpy()
             vals_h = self.pair_values[:total].cpu().numpy()
         roff, moff, removed, merged = [0], [0], [], []
-        for e in hit:
-            pe, ve = self._system_pairs(int(e), int(base[e]), int(counts[e]), pairs_h, vals_h)
-            rem, mg = merge_plan_rows(pe, ve)
-            removed.extend(rem)
+        cap = self.pair_capacity
+        for e in hit.tolist():
+            a, c = int(base[e]), int(counts[e])
+            if c == 1 and a + 1 <= cap:
+                # one pair: merge_plan_rows' arithmetic without its bookkeeping
+                (i, j), = pairs_h[a: a + 1].tolist()
+                x1, y1, z1, r1, x2, y2, z2, r2 = vals_h[a].tolist()
+                c1 = r1**3
+                c2 = r2**3
+                sm = c1 + c2
+                removed.extend((i, j) if i < j else (j, i))
+                merged.append(((x1 * c1 + x2 * c2) / sm, (y1 * c1 + y2 * c2) / sm,
+                               (z1 * c1 + z2 * c2) / sm, sm ** (1 / 3)))
+            else:
+                pe, ve = self._system_pairs(e, a, c, pairs_h, vals_h)
+                rem, mg = merge_plan_rows(pe, ve)
+                removed.extend(rem)
+                merged.extend(mg)
             roff.append(len(removed))
-            merged.extend(mg)
             moff.append(len(merged))
         es = np.ascontiguousarray(hit, dtype=np.int32)
         ro = np.asarray(roff, dtype=np.int32)
