@@ -532,13 +532,18 @@ __device__ int search_target(const BEns& E, const BDev& D, const double* __restr
   const int ix = dem::cell_of(xi, D.cell, g, 1);
   const int iy = dem::cell_of(yi, D.cell, g, 1);
   const int iz = dem::cell_of(zi, D.cell, g, 1);
+  int cx[3], cy[3], cz[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cx[d] = dem::nb_coord(ix, d - 1, g, 1);
+    cy[d] = dem::nb_coord(iy, d - 1, g, 1);
+    cz[d] = dem::nb_coord(iz, d - 1, g, 1);
+  }
   int nb[27];
   int nseen = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const int di = k / 9 - 1, dj = (k / 3) % 3 - 1, dk = k % 3 - 1;
-    const int nh = dem::hash_cell(dem::pymod(ix + di, g), dem::pymod(iy + dj, g),
-                                  dem::pymod(iz + dk, g), mask);
+    const int nh = dem::hash_cell(cx[k / 9], cy[(k / 3) % 3], cz[k % 3], mask);
     bool dup = false;
 #pragma unroll
     for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
@@ -600,13 +605,18 @@ __device__ int count_slot(const BEns& E, const BDev& D, const int* __restrict__ 
   const int ix = dem::cell_of(me.x, D.cell, g, 1);
   const int iy = dem::cell_of(me.y, D.cell, g, 1);
   const int iz = dem::cell_of(me.z, D.cell, g, 1);
+  int cx[3], cy[3], cz[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cx[d] = dem::nb_coord(ix, d - 1, g, 1);
+    cy[d] = dem::nb_coord(iy, d - 1, g, 1);
+    cz[d] = dem::nb_coord(iz, d - 1, g, 1);
+  }
   int nb[27];
   int nseen = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const int di = k / 9 - 1, dj = (k / 3) % 3 - 1, dk = k % 3 - 1;
-    const int nh = dem::hash_cell(dem::pymod(ix + di, g), dem::pymod(iy + dj, g),
-                                  dem::pymod(iz + dk, g), mask);
+    const int nh = dem::hash_cell(cx[k / 9], cy[(k / 3) % 3], cz[k % 3], mask);
     bool dup = false;
 #pragma unroll
     for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);

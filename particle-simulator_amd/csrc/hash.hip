@@ -34,7 +34,6 @@ namespace {
 using dem::cell_of;
 using dem::contact_pair;
 using dem::hash_cell;
-using dem::pymod;
 
 struct HashParams {
   double cell;  // 2 * max radius
@@ -178,17 +177,17 @@ __global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, in
   const int iz = cell_of(me.z, hp.cell, g, periodic);
   int nb[27];
   int nseen = 0;
+  int cx[3], cy[3], cz[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cx[d] = dem::nb_coord(ix, d - 1, g, periodic);
+    cy[d] = dem::nb_coord(iy, d - 1, g, periodic);
+    cz[d] = dem::nb_coord(iz, d - 1, g, periodic);
+  }
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    int nx = ix + k / 9 - 1, ny = iy + (k / 3) % 3 - 1, nz = iz + k % 3 - 1;
-    bool valid = true;
-    if (periodic) {
-      nx = pymod(nx, g);
-      ny = pymod(ny, g);
-      nz = pymod(nz, g);
-    } else {
-      valid = nx >= 0 && ny >= 0 && nz >= 0 && nx < g && ny < g && nz < g;
-    }
+    const int nx = cx[k / 9], ny = cy[(k / 3) % 3], nz = cz[k % 3];
+    const bool valid = nx >= 0 && ny >= 0 && nz >= 0;
     if (valid) {
       const int nh = hash_cell(nx, ny, nz, mask);
       bool dup = false;

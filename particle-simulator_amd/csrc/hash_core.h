@@ -22,6 +22,15 @@ __device__ __forceinline__ int cell_of(double x, double cell, int g, int periodi
   return c < 0 ? 0 : (c >= g ? g - 1 : c);
 }
 
+// Neighbour coordinate c + d, d in {-1, 0, 1}, of a cell c in [0, g): the
+// periodic wrap (equal to pymod(c + d, g) for these arguments, without an
+// integer division), or -1 outside an open box.
+__device__ __forceinline__ int nb_coord(int c, int d, int g, int periodic) {
+  const int a = c + d;
+  if (periodic) return a < 0 ? a + g : (a >= g ? a - g : a);
+  return (a < 0 || a >= g) ? -1 : a;
+}
+
 __device__ __forceinline__ int hash_cell(int ix, int iy, int iz, int mask) {
   uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
   return (int)(h & (uint32_t)mask);
