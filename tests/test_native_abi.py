@@ -51,6 +51,15 @@ def test_invalid_arguments_are_rejected_before_any_launch():
     assert rc == _native.DEM_ERR_WORKSPACE
     with pytest.raises(_native.NativeError):
         _native.call("dem_euler_step", None, None, None, None, 5, 1.0, 1.0, 1.0, 0, None)
+    # merge compaction: more removed than droplets, negative counts, null outputs
+    for args in ((4, 5, 0), (-1, 0, 0), (4, 0, -1)):
+        rc = lib.dem_merge_compact(ctypes.c_void_p(8), ctypes.c_void_p(8), args[0],
+                                   ctypes.c_void_p(8), args[1], ctypes.c_void_p(8), args[2],
+                                   ctypes.c_void_p(8), ctypes.c_void_p(8), None)
+        assert rc == _native.DEM_ERR_INVALID and "bad counts" in _native.last_error()
+    rc = lib.dem_merge_compact(ctypes.c_void_p(8), ctypes.c_void_p(8), 4, ctypes.c_void_p(8), 2,
+                               ctypes.c_void_p(8), 1, None, None, None)
+    assert rc == _native.DEM_ERR_INVALID and "null" in _native.last_error()
 
 
 def test_zero_particles_is_a_no_op():
@@ -58,6 +67,7 @@ def test_zero_particles_is_a_no_op():
     ph = _native.DemPhys(1.0, 1.0, 1.0, 1, 0)
     assert lib.dem_direct_forces(None, None, 0, ctypes.byref(ph), None, None, 0, None) == 0
     assert lib.dem_euler_step(None, None, None, None, 0, 1.0, 1.0, 1.0, 1, None) == 0
+    assert lib.dem_merge_compact(None, None, 0, None, 0, None, 0, None, None, None) == 0
 
 
 def _split_max(n):
