@@ -144,7 +144,7 @@ int dem_direct_set_tuning(int split, int tile);
  * the Oseen sum without the COMSOL correction: each unordered pair is
  * evaluated once and credited to both droplets (same accuracy contract,
  * different rounding than the one-sided kernels).  mode 0: one-sided kernels
- * only; 1 (default): pair-once for 16384 <= n <= 131072 when no explicit split
+ * only; 1 (default): pair-once for 10240 <= n <= 131072 when no explicit split
  * is set; 2: pair-once for every n, the split of dem_direct_set_tuning (when
  * > 0) giving the number of chunks of each block's partner list (the batched
  * pair-once sums equal these bit for bit with the split of

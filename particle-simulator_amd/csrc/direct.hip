@@ -59,7 +59,7 @@ using dem::kSymB;
 using dem::kSymTplConv;
 using dem::kSymTplForce;
 using dem::sym_partners;
-constexpr int kSymMin = 16384;
+constexpr int kSymMin = 10240;  // below, the one-sided kernels are faster
 constexpr int kSymMax = 131072;
 
 inline int sym_blocks(int n) { return (n + kSymB - 1) / kSymB; }

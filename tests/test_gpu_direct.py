@@ -190,10 +190,10 @@ def test_n65536_against_fp64_kernel(gpu):
                                atol=1e-12 * np.abs(Fo[:256]).max())
 
 
-@pytest.mark.parametrize("n,boundary", [(16384, "periodic"), (20000, "periodic"),
+@pytest.mark.parametrize("n,boundary", [(10240, "periodic"), (20000, "periodic"),
                                         (20000, "open"), (131072, "periodic")])
 def test_pair_once_kernels_against_fp64_kernel(gpu, n, boundary):
-    """Pair-once (Newton's third law) kernels, used for 16384 <= N <= 131072:
+    """Pair-once (Newton's third law) kernels, used for 10240 <= N <= 131072:
     smallest and largest N, a ragged last block of 256 (N = 20000: 79 blocks,
     an odd count) and both boundary modes.  Accuracy contract against the FP64
     kernel, for the pair-once and the one-sided evaluation alike; the two
