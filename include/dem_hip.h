@@ -302,6 +302,9 @@ int dem_octree_field(const dem_tree_info* info, const char* name, size_t* offset
  *   while some system has pairs: dem_batch_merge (the caller's merge plan),
  *                                dem_batch_check(new_step=0) (only the
  *                                systems the previous check found in contact);
+ *                                or, on a second stream beside the advance:
+ *                                dem_batch_wait_go, then dem_batch_merge_async
+ *                                and the re-checks;
  *   dem_batch_advance            the remaining systems advance.
  * Each system advances at most once per global step ("stepped" flag).  Launch
  * sizes are fixed at the capacities, so the three captured HIP graphs stay
@@ -378,6 +381,38 @@ int dem_batch_advance(dem_batch_t b, dem_stream_t stream);
 int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int32_t* removed_off,
                     const int32_t* removed, const int32_t* merged_off, const double* merged,
                     dem_stream_t stream);
+
+/* dem_batch_merge without the rebuild of the work tables: only the merged
+ * systems' descriptors, hash cells and records are rewritten (on `stream`),
+ * so it may run on a second stream while an advance of the other systems
+ * runs (after dem_batch_wait_go on that stream).  The tables are rebuilt by
+ * the next first check or advance; a re-check (new_step=0) does not need
+ * them.  Synchronises `stream`.  No reference counterpart (scheduling). */
+int dem_batch_merge_async(dem_batch_t b, int count, const int32_t* systems,
+                          const int32_t* removed_off, const int32_t* removed,
+                          const int32_t* merged_off, const double* merged,
+                          dem_stream_t stream);
+
+/* Makes `stream` wait until the last dem_batch_advance has decided which
+ * systems it moves; from then on the systems it leaves alone (those found in
+ * contact) may be merged and re-checked on `stream` while it runs.  No
+ * reference counterpart (scheduling). */
+int dem_batch_wait_go(dem_batch_t b, dem_stream_t stream);
+
+/* Greedy merge plans of `count` systems, host only (no GPU): system k's
+ * pairs are pairs[pair_off[k] .. pair_off[k+1]) (local (i, j)) with
+ * values[8 p ..] = (x, y, z, r) of i then of j, visited in the order
+ * order[pair_off[k] + q] (positions inside the system's segment: the
+ * reference's np.argsort of the first column).  A pair whose droplet already
+ * merged is skipped; otherwise the merged droplet r = (r_i^3 + r_j^3)^(1/3) at
+ * the r^3-weighted centre is appended to merged[4 m ..] and i, j are removed.
+ * Outputs: removed[removed_off[k] .. removed_off[k+1]) ascending and
+ * merged_off (count + 1 each; at most 2 removed and 1 merged per pair): the
+ * lists dem_batch_merge takes.  merge_plan_rows / droplet_solution.py:208-262
+ * with the same IEEE operations. */
+int dem_merge_plan(int count, const int32_t* pair_off, const int32_t* pairs,
+                   const double* values, const int32_t* order, int32_t* removed_off,
+                   int32_t* removed, int32_t* merged_off, double* merged);
 
 /* Systems systems[k] now hold n[k] droplets (the caller rewrote their slices
  * of pos/radii itself): rebuilds the work tables, hash cells and records.

@@ -167,6 +167,7 @@ struct BLayout {
   int* pair_base;// [n_ens + 1] exclusive scan of pair_count
   int* stage_i;  // merge lists uploaded by dem_batch_merge
   double* stage_d;
+  BEns* ens_stage;  // [n_ens] updated descriptors of the systems just merged
   int* active;
   int* stepped;
   int* go;
@@ -250,6 +251,7 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   l.pair_base = c.take<int>(ne + 1);
   l.stage_i = c.take<int>(at_least1(p.cap_total) + 3 * ne + 3);
   l.stage_d = c.take<double>(4 * at_least1(p.cap_total));
+  l.ens_stage = c.take<BEns>(ne);
   l.active = c.take<int>(ne);
   l.stepped = c.take<int>(ne);
   l.go = c.take<int>(ne);
@@ -279,13 +281,9 @@ __global__ void k_b_init(const BEns* __restrict__ ens, BDev* __restrict__ dev, i
 
 // Hash cell of every system: cell = 2 max r (spatial_hash...py:257-258; cell
 // <= 0 means no contacts, :259-260), g = max(1, ceil(L / cell)) (:262).
-__global__ __launch_bounds__(256) void k_b_cells(const BEns* __restrict__ ens,
-                                                 const BHead* __restrict__ head,
-                                                 const double* __restrict__ radii,
-                                                 BDev* __restrict__ dev) {
+__device__ void cells_of(const BEns* __restrict__ ens, const double* __restrict__ radii,
+                         BDev* __restrict__ dev, int e) {
   __shared__ double s_max[4];
-  const int e = blockIdx.x;
-  if (e >= head->n_ens) return;
   const BEns& E = ens[e];
   double m = -INFINITY;
   for (int i = threadIdx.x; i < E.n; i += 256) m = fmax(m, radii[(size_t)E.off + i]);
@@ -304,6 +302,45 @@ __global__ __launch_bounds__(256) void k_b_cells(const BEns* __restrict__ ens,
   dev[e].skip = skip;
 }
 
+__global__ __launch_bounds__(256) void k_b_cells(const BEns* __restrict__ ens,
+                                                 const BHead* __restrict__ head,
+                                                 const double* __restrict__ radii,
+                                                 BDev* __restrict__ dev) {
+  if ((int)blockIdx.x >= head->n_ens) return;
+  cells_of(ens, radii, dev, blockIdx.x);
+}
+
+// The same for the `count` listed systems only (grid = count).
+__global__ __launch_bounds__(256) void k_b_cells_list(const BEns* __restrict__ ens, int count,
+                                                      const int* __restrict__ systems,
+                                                      const double* __restrict__ radii,
+                                                      BDev* __restrict__ dev) {
+  if ((int)blockIdx.x >= count) return;
+  cells_of(ens, radii, dev, systems[blockIdx.x]);
+}
+
+// Quantise droplet i of system e into its record (zero padding at i >= n).
+__device__ __forceinline__ void pack_droplet(const BEns* __restrict__ ens,
+                                             const BDev* __restrict__ dev,
+                                             const double* __restrict__ pos,
+                                             const double* __restrict__ radii,
+                                             Rec* __restrict__ rec, RecC* __restrict__ recc, int e,
+                                             int i) {
+  const BEns& E = ens[e];
+  const size_t q = (size_t)E.roff + i;
+  if (i >= E.n) {  // the last block's padding: no radius, no force
+    rec[q] = Rec{};
+    recc[q] = RecC{};
+    return;
+  }
+  const size_t g = (size_t)E.off + i;
+  Rec o;
+  RecC c;
+  dem::pack_one(pos + 3 * g, radii[g], nullptr, dev[e].sc, o, c);
+  rec[q] = o;
+  recc[q] = c;
+}
+
 // Quantise the droplets of every row.
 __global__ __launch_bounds__(kRow) void k_b_pack(const BEns* __restrict__ ens,
                                                  const BDev* __restrict__ dev,
@@ -314,20 +351,34 @@ __global__ __launch_bounds__(kRow) void k_b_pack(const BEns* __restrict__ ens,
                                                  Rec* __restrict__ rec, RecC* __restrict__ recc) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  const BEns& E = ens[row.x];
-  const int i = row.y + threadIdx.x;
-  const size_t q = (size_t)E.roff + i;
-  if (i >= E.n) {  // the last block's padding: no radius, no force
-    rec[q] = Rec{};
-    recc[q] = RecC{};
-    return;
-  }
-  const size_t g = (size_t)E.off + i;
-  Rec o;
-  RecC c;
-  dem::pack_one(pos + 3 * g, radii[g], nullptr, dev[row.x].sc, o, c);
-  rec[q] = o;
-  recc[q] = c;
+  pack_droplet(ens, dev, pos, radii, rec, recc, row.x, row.y + threadIdx.x);
+}
+
+// The same for the rows of the `count` listed systems (grid = count x
+// rows_max, rows_max = rows of the largest capacity): droplets below n and
+// the padding of the last 256-record block.
+__global__ __launch_bounds__(kRow) void k_b_pack_list(const BEns* __restrict__ ens,
+                                                      const BDev* __restrict__ dev, int count,
+                                                      int rows_max,
+                                                      const int* __restrict__ systems,
+                                                      const double* __restrict__ pos,
+                                                      const double* __restrict__ radii,
+                                                      Rec* __restrict__ rec,
+                                                      RecC* __restrict__ recc) {
+  const int k = (int)blockIdx.x / rows_max, r = (int)blockIdx.x % rows_max;
+  if (k >= count) return;
+  const int e = systems[k];
+  const int i = r * kRow + threadIdx.x;
+  const int n = ens[e].n;
+  if (i >= ((n + dem::kSymB - 1) / dem::kSymB) * dem::kSymB) return;
+  pack_droplet(ens, dev, pos, radii, rec, recc, e, i);
+}
+
+// Descriptors of the systems just merged (dem_batch_merge_async).
+__global__ void k_b_ens_scatter(int count, const int* __restrict__ systems,
+                                const BEns* __restrict__ src, BEns* __restrict__ dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) dst[systems[k]] = src[k];
 }
 
 // Merges computed by the caller, one workgroup per listed system.  Stage
@@ -943,6 +994,10 @@ struct dem_batch {
   std::vector<int4> items_h;
   std::vector<int2> rows_h;
   std::vector<int> stage_h;
+  std::vector<BEns> ens_stage_h;
+  int cap_max = 0;            // largest capacity (rows of k_b_pack_list)
+  bool tables_dirty = false;  // merged systems' items / rows not rebuilt yet
+  hipEvent_t go_done = nullptr;  // recorded after the "go" kernel of an advance
   hipStream_t cap_stream = nullptr;
   // 0: first check of a global step, 1: check again, 2: advance
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
@@ -988,39 +1043,42 @@ int check_ensembles(const dem_batch_ensemble* ens, int n_ens) {
 
 // Rebuilds the system table, items and rows from the host descriptions and
 // uploads them (synchronising `st` so the host tables can be reused).
+// Device descriptor of system e from its host description and the plan.
+BEns ens_entry(const dem_batch* b, int e) {
+  const dem_batch_ensemble& d = b->desc[e];
+  BEns E{};
+  E.part_off = b->plan.part_off[e];
+  E.cnt_off = b->plan.cnt_off[e];
+  E.start_off = b->plan.start_off[e];
+  E.partj_off = b->plan.partj_off[e];
+  E.off = d.offset;
+  E.roff = b->plan.rec_off[e];
+  E.n = d.n;
+  E.cap = d.capacity;
+  if (b->tune.pair_once) {
+    const int B = (d.n + dem::kSymB - 1) / dem::kSymB;
+    E.S = pair_once_split(d.n, b->tune.jlen);
+    E.jchunk = (dem::sym_entries_max(B > 0 ? B : 1) + E.S - 1) / E.S;
+  } else {
+    E.S = batch_split(d.n, b->tune.jlen);
+    E.jchunk = d.n > 0 ? (d.n + E.S - 1) / E.S : 0;
+  }
+  E.steps = d.steps;
+  E.T = next_pow2(2 * (d.capacity > 0 ? d.capacity : 1));
+  E.L = d.L;
+  E.dt = d.dt;
+  E.sf = d.stokes_factor;
+  E.m_const = d.m_const;
+  E.eta_const = d.eta_const;
+  return E;
+}
+
 int upload_tables(dem_batch* b, hipStream_t st) {
   const int bw = b->tune.block;
   b->ens_h.resize(b->n_ens);
   b->items_h.clear();
   b->rows_h.clear();
-  for (int e = 0; e < b->n_ens; ++e) {
-    const dem_batch_ensemble& d = b->desc[e];
-    BEns& E = b->ens_h[e];
-    E = BEns{};
-    E.part_off = b->plan.part_off[e];
-    E.cnt_off = b->plan.cnt_off[e];
-    E.start_off = b->plan.start_off[e];
-    E.partj_off = b->plan.partj_off[e];
-    E.off = d.offset;
-    E.roff = b->plan.rec_off[e];
-    E.n = d.n;
-    E.cap = d.capacity;
-    if (b->tune.pair_once) {
-      const int B = (d.n + dem::kSymB - 1) / dem::kSymB;
-      E.S = pair_once_split(d.n, b->tune.jlen);
-      E.jchunk = (dem::sym_entries_max(B > 0 ? B : 1) + E.S - 1) / E.S;
-    } else {
-      E.S = batch_split(d.n, b->tune.jlen);
-      E.jchunk = d.n > 0 ? (d.n + E.S - 1) / E.S : 0;
-    }
-    E.steps = d.steps;
-    E.T = next_pow2(2 * (d.capacity > 0 ? d.capacity : 1));
-    E.L = d.L;
-    E.dt = d.dt;
-    E.sf = d.stokes_factor;
-    E.m_const = d.m_const;
-    E.eta_const = d.eta_const;
-  }
+  for (int e = 0; e < b->n_ens; ++e) b->ens_h[e] = ens_entry(b, e);
   // items of the largest systems first (their items take longest)
   for (int e : b->order) {
     const BEns& E = b->ens_h[e];
@@ -1125,10 +1183,6 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
   const BLayout& l = b->l;
   const unsigned gi = (unsigned)std::max<long long>(b->plan.items_cap, 1);
   const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
-  hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
-                     (const BEns*)l.ens, (const BHead*)l.head, (const int*)l.steps_done,
-                     (const int*)l.pair_count, l.stepped, l.go);
-  DEM_CHECK_LAUNCH("k_b_go");
   auto force = [&] {
     hipLaunchKernelGGL(k_b_force<BW>, dim3(gi), dim3(BW), 0, st, (const BEns*)l.ens,
                        (const int4*)l.items, (const BHead*)l.head, (const int*)l.go,
@@ -1170,10 +1224,6 @@ int enqueue_advance_sym(dem_batch* b, hipStream_t st, bool prof) {
   const unsigned gi = (unsigned)std::max<long long>(b->plan.items_cap, 1);
   const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
   constexpr int TF = dem::kSymTplForce, TC = dem::kSymTplConv;
-  hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
-                     (const BEns*)l.ens, (const BHead*)l.head, (const int*)l.steps_done,
-                     (const int*)l.pair_count, l.stepped, l.go);
-  DEM_CHECK_LAUNCH("k_b_go");
   auto force = [&] {
     hipLaunchKernelGGL(k_b_force_sym<TF>, dim3(gi), dim3(dem::kSymB / TF), 0, st,
                        (const BEns*)l.ens, (const int4*)l.items, (const BHead*)l.head,
@@ -1248,7 +1298,32 @@ int capture_one(dem_batch* b, int k) {
   return DEM_OK;
 }
 
+// Which systems an advance moves ("go"), then an event that a second stream
+// can wait on before it re-checks and merges the others.
+int launch_go(dem_batch* b, hipStream_t st) {
+  const BLayout& l = b->l;
+  hipLaunchKernelGGL(k_b_go, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
+                     (const BEns*)l.ens, (const BHead*)l.head, (const int*)l.steps_done,
+                     (const int*)l.pair_count, l.stepped, l.go);
+  DEM_CHECK_LAUNCH("k_b_go");
+  if (!b->go_done) DEM_CHECK_HIP(hipEventCreateWithFlags(&b->go_done, hipEventDisableTiming));
+  DEM_CHECK_HIP(hipEventRecord(b->go_done, st));
+  return DEM_OK;
+}
+
+// k = 0: first check of a global step, 1: check again, 2: advance.  Work
+// tables left stale by dem_batch_merge_async are rebuilt before a first check
+// or an advance (a re-check needs only the merged systems' descriptors).
 int launch(dem_batch* b, int k, hipStream_t st) {
+  if (k != 1 && b->tables_dirty) {
+    int rc = upload_tables(b, st);
+    if (rc) return rc;
+    b->tables_dirty = false;
+  }
+  if (k == 2) {
+    int rc = launch_go(b, st);
+    if (rc) return rc;
+  }
   if (b->exec[k]) {
     DEM_CHECK_HIP(hipGraphLaunch(b->exec[k], st));
     return DEM_OK;
@@ -1259,6 +1334,61 @@ int launch(dem_batch* b, int k, hipStream_t st) {
 }  // namespace
 
 extern "C" {
+
+// Greedy merge plans of many systems on the host (merge_plan_rows /
+// droplet_solution.py:208-262 per system).  The IEEE operations are those of
+// the reference's float64 scalars: r^3 = pow(r, 3), r = pow(c1 + c2, 1/3),
+// centre (x1 c1 + x2 c2) / (c1 + c2) per axis, no contraction.
+int dem_merge_plan(int count, const int32_t* pair_off, const int32_t* pairs,
+                   const double* values, const int32_t* order, int32_t* removed_off,
+                   int32_t* removed, int32_t* merged_off, double* merged) {
+#pragma clang fp contract(off)
+  if (count < 0) return dem::set_error(DEM_ERR_INVALID, "count = %d < 0", count);
+  if (count == 0) {
+    if (removed_off) removed_off[0] = 0;
+    if (merged_off) merged_off[0] = 0;
+    return DEM_OK;
+  }
+  if (!pair_off || !pairs || !values || !order || !removed_off || !removed || !merged_off ||
+      !merged)
+    return dem::set_error(DEM_ERR_INVALID, "null argument");
+  if (pair_off[0] != 0) return dem::set_error(DEM_ERR_INVALID, "pair_off[0] must be 0");
+  std::vector<int> done;
+  int nr = 0, nm = 0;
+  removed_off[0] = 0;
+  merged_off[0] = 0;
+  for (int k = 0; k < count; ++k) {
+    const int a = pair_off[k], c = pair_off[k + 1] - a;
+    if (c < 0) return dem::set_error(DEM_ERR_INVALID, "pair offsets must not decrease");
+    done.clear();
+    for (int q = 0; q < c; ++q) {
+      const int o = order[a + q];
+      if (o < 0 || o >= c) return dem::set_error(DEM_ERR_INVALID, "order out of range");
+      const int p = a + o;
+      const int i = pairs[2 * (size_t)p], j = pairs[2 * (size_t)p + 1];
+      if (std::find(done.begin(), done.end(), i) != done.end() ||
+          std::find(done.begin(), done.end(), j) != done.end())
+        continue;
+      const double* v = values + 8 * (size_t)p;
+      const double c1 = pow(v[3], 3.0);
+      const double c2 = pow(v[7], 3.0);
+      const double sm = c1 + c2;
+      double* m = merged + 4 * (size_t)nm;
+      m[0] = (v[0] * c1 + v[4] * c2) / sm;
+      m[1] = (v[1] * c1 + v[5] * c2) / sm;
+      m[2] = (v[2] * c1 + v[6] * c2) / sm;
+      m[3] = pow(sm, 1.0 / 3.0);
+      ++nm;
+      done.push_back(i);
+      done.push_back(j);
+    }
+    std::sort(done.begin(), done.end());
+    for (int d : done) removed[nr++] = d;
+    removed_off[k + 1] = nr;
+    merged_off[k + 1] = nm;
+  }
+  return DEM_OK;
+}
 
 int dem_batch_split(int n, int jlen) { return batch_split(n, jlen); }
 
@@ -1298,6 +1428,7 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
                    [&](int x, int y) { return ens[x].capacity > ens[y].capacity; });
   b->plan = p;
   b->l = l;
+  for (int e = 0; e < n_ens; ++e) b->cap_max = std::max(b->cap_max, ens[e].capacity);
   b->pos = pos;
   b->radii = radii;
   b->F = forces;
@@ -1336,13 +1467,15 @@ int dem_batch_resize(dem_batch_t b, int count, const int32_t* systems, const int
   DEM_CHECK_HIP(hipStreamSynchronize(st));
   for (int k = 0; k < count; ++k) b->desc[systems[k]].n = n[k];
   int rc = upload_tables(b, st);
+  if (!rc) b->tables_dirty = false;
   if (rc) return rc;
   return refresh(b, st);
 }
 
-int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int32_t* removed_off,
-                    const int32_t* removed, const int32_t* merged_off, const double* merged,
-                    dem_stream_t stream) {
+static int merge_systems(dem_batch_t b, int count, const int32_t* systems,
+                         const int32_t* removed_off, const int32_t* removed,
+                         const int32_t* merged_off, const double* merged, dem_stream_t stream,
+                         bool defer) {
   if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
   if (count < 0 || count > b->n_ens) return dem::set_error(DEM_ERR_INVALID, "bad count %d", count);
   if (count == 0) return DEM_OK;
@@ -1388,9 +1521,59 @@ int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int3
                      b->l.spos);
   DEM_CHECK_LAUNCH("k_b_compact");
   for (int k = 0; k < count; ++k) b->desc[systems[k]].n = new_n[k];
-  int rc = upload_tables(b, st);  // synchronises: the host lists may be freed after return
-  if (rc) return rc;
-  return refresh(b, st);
+  if (!defer) {
+    int rc = upload_tables(b, st);  // synchronises: the host lists may be freed after return
+    if (rc) return rc;
+    b->tables_dirty = false;
+    return refresh(b, st);
+  }
+  // only the merged systems' descriptors, hash cells and records change; the
+  // work tables are rebuilt before the next first check or advance
+  const BLayout& l = b->l;
+  b->ens_stage_h.resize(count);
+  for (int k = 0; k < count; ++k) {
+    const int e = systems[k];
+    b->ens_h[e] = ens_entry(b, e);
+    b->ens_stage_h[k] = b->ens_h[e];
+  }
+  DEM_CHECK_HIP(hipMemcpyAsync(l.ens_stage, b->ens_stage_h.data(), sizeof(BEns) * count,
+                               hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_b_ens_scatter, dim3((count + 255) / 256), dim3(256), 0, st, count,
+                     (const int*)l.stage_i, (const BEns*)l.ens_stage, l.ens);
+  DEM_CHECK_LAUNCH("k_b_ens_scatter");
+  hipLaunchKernelGGL(k_b_cells_list, dim3(count), dim3(256), 0, st, (const BEns*)l.ens, count,
+                     (const int*)l.stage_i, (const double*)b->radii, l.dev);
+  DEM_CHECK_LAUNCH("k_b_cells_list");
+  const int rows_max = std::max(1, (b->cap_max + kRow - 1) / kRow);
+  const long long blocks = (long long)count * rows_max;
+  if (blocks > 0x7fffffffLL) return dem::set_error(DEM_ERR_CAPACITY, "too many rows to pack");
+  hipLaunchKernelGGL(k_b_pack_list, dim3((unsigned)blocks), dim3(kRow), 0, st,
+                     (const BEns*)l.ens, (const BDev*)l.dev, count, rows_max,
+                     (const int*)l.stage_i, (const double*)b->pos, (const double*)b->radii, l.rec,
+                     l.recc);
+  DEM_CHECK_LAUNCH("k_b_pack_list");
+  b->tables_dirty = true;
+  DEM_CHECK_HIP(hipStreamSynchronize(st));  // the host lists may be freed after return
+  return DEM_OK;
+}
+
+int dem_batch_merge(dem_batch_t b, int count, const int32_t* systems, const int32_t* removed_off,
+                    const int32_t* removed, const int32_t* merged_off, const double* merged,
+                    dem_stream_t stream) {
+  return merge_systems(b, count, systems, removed_off, removed, merged_off, merged, stream, false);
+}
+
+int dem_batch_merge_async(dem_batch_t b, int count, const int32_t* systems,
+                          const int32_t* removed_off, const int32_t* removed,
+                          const int32_t* merged_off, const double* merged, dem_stream_t stream) {
+  return merge_systems(b, count, systems, removed_off, removed, merged_off, merged, stream, true);
+}
+
+int dem_batch_wait_go(dem_batch_t b, dem_stream_t stream) {
+  if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
+  if (!b->go_done) return dem::set_error(DEM_ERR_INVALID, "no advance has been launched");
+  DEM_CHECK_HIP(hipStreamWaitEvent(dem::as_stream(stream), b->go_done, 0));
+  return DEM_OK;
 }
 
 int dem_batch_check(dem_batch_t b, int new_step, dem_stream_t stream) {
@@ -1453,6 +1636,7 @@ int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes) 
 int dem_batch_destroy(dem_batch_t b) {
   if (!b) return DEM_OK;
   drop_graphs(b);
+  if (b->go_done) (void)hipEventDestroy(b->go_done);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   delete b;
   return DEM_OK;

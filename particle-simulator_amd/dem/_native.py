@@ -143,6 +143,9 @@ SIGNATURES: dict[str, tuple] = {
     "dem_batch_check": (_I, [_P, _I, _P]),
     "dem_batch_advance": (_I, [_P, _P]),
     "dem_batch_merge": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "dem_batch_merge_async": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "dem_batch_wait_go": (_I, [_P, _P]),
+    "dem_merge_plan": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dem_batch_capture": (_I, [_P]),
     "dem_batch_run": (_I, [_P, _I, _P]),
     "dem_batch_field": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_S)]),

@@ -150,6 +150,10 @@ class BatchedEnsembles:
         self.merge_rounds = 0
         self.global_steps = 0
         self.phase_ms: dict | None = None
+        # merge rounds on a second stream beside the advance of the
+        # contact-free systems (default priority, like every stream here)
+        self.overlap = True
+        self._side = t.cuda.Stream()
 
     def _field(self, name, dtype):
         p = ctypes.c_void_p()
@@ -185,10 +189,40 @@ class BatchedEnsembles:
     def step(self, merge: bool = True) -> int:
         """One global step; returns the number of system merges in it.
 
-        check -> (advance the systems without contacts | host merge plans of
-        the others) -> merge, check again, ... -> advance the rest.  With
-        ``self.phase_ms`` set to a dict, phases are synchronised and their
-        wall times (ms) accumulated under "check", "advance", "merge"."""
+        check -> (advance the systems without contacts | merge the others,
+        check them again, ... on a second stream) -> advance the rest.  The
+        merge rounds touch only the systems in contact, whose descriptors,
+        cells and records ``dem_batch_merge_async`` rewrites, and start once
+        the advance has fixed which systems it moves (``dem_batch_wait_go``);
+        the work tables are rebuilt before the second advance.  With
+        ``self.phase_ms`` set to a dict (or ``self.overlap`` false) the
+        phases run one after another, synchronised, and their wall times (ms)
+        are accumulated under "check", "advance", "merge"."""
+        if self.phase_ms is not None or not self.overlap or not merge:
+            return self._step_serial(merge)
+        t = _device.torch()
+        main = t.cuda.current_stream()
+        self.check(new_step=True)
+        self._stage()
+        self.advance()
+        _native.call("dem_batch_wait_go", self._h, ctypes.c_void_p(self._side.cuda_stream))
+        merged = 0
+        with t.cuda.stream(self._side):
+            while True:
+                self._staged.synchronize()
+                k = self._merge_staged(defer_tables=True)
+                if k == 0:
+                    break
+                merged += k
+                self.check(new_step=False)
+                self._stage()
+        main.wait_stream(self._side)
+        if merged:
+            self.advance()
+        self.global_steps += 1
+        return merged
+
+    def _step_serial(self, merge: bool = True) -> int:
         t0 = time.perf_counter()
         self.check(new_step=True)
         if merge:
@@ -245,15 +279,19 @@ class BatchedEnsembles:
         v = t.cat([self.pos.index_select(0, g), self.radii.index_select(0, g)[:, None]], 1)
         return pairs, v.cpu().numpy().reshape(-1, 8)
 
-    def _merge_staged(self) -> int:
+    def _merge_staged(self, defer_tables: bool = False) -> int:
         """Merge every system the last check found in contact; returns how
         many systems were merged.
 
-        The merge decisions and merged droplets come from ``merge_plan_rows``
-        (the reference's float64 arithmetic) on the droplets named in the
-        pairs, whose values the check packed next to them; ``dem_batch_merge`` then
-        compacts each system on the device (survivors in order, merged
-        droplets appended) — exactly ``merge_collided``'s result."""
+        The merge decisions and merged droplets come from the reference's
+        greedy merge with its float64 arithmetic, on the droplets named in
+        the pairs, whose values the check packed next to them: for all
+        systems at once by ``dem_merge_plan`` (pairs visited in np.argsort
+        order of their first droplet, as ``merge_plan_rows`` does), or, when
+        the packed pair buffer overflowed, system by system with
+        ``merge_plan_rows``.  ``dem_batch_merge`` then compacts each system on
+        the device (survivors in order, merged droplets appended) — exactly
+        ``merge_collided``'s result."""
         counts = self._h_counts.numpy().astype(np.int64)
         hit = np.nonzero(counts)[0]
         if len(hit) == 0:
@@ -266,6 +304,55 @@ class BatchedEnsembles:
         if total > k:  # rare: more pairs than the pinned staging holds
             pairs_h = self.pairs[:total].cpu().numpy()
             vals_h = self.pair_values[:total].cpu().numpy()
+        vp = ctypes.c_void_p
+        if base[-1] <= self.pair_capacity:
+            ro, rm, mo, mg = self._plan_native(hit, counts, base, pairs_h, vals_h)
+        else:
+            ro, rm, mo, mg = self._plan_python(hit, counts, base, pairs_h, vals_h)
+        es = np.ascontiguousarray(hit, dtype=np.int32)
+        _native.call("dem_batch_merge_async" if defer_tables else "dem_batch_merge", self._h,
+                     len(es), es.ctypes.data_as(vp),
+                     ro.ctypes.data_as(vp), rm.ctypes.data_as(vp), mo.ctypes.data_as(vp),
+                     mg.ctypes.data_as(vp), _device.stream())
+        self.n[hit] -= ro[1:] - ro[:-1]
+        self.n[hit] += mo[1:] - mo[:-1]
+        self.merges += len(hit)
+        self.merge_rounds += 1
+        return len(hit)
+
+    @staticmethod
+    def _plan_native(hit, counts, base, pairs_h, vals_h):
+        """Merge lists of every system in contact from one ``dem_merge_plan``
+        call.  The systems' pair segments are consecutive in the packed list;
+        a system whose first column has no repeated droplet is visited in
+        list order (the only order a sort can give), the others in np.argsort
+        order, which is what the reference's merge uses."""
+        c = counts[hit]
+        total = int(base[-1])
+        P = np.ascontiguousarray(pairs_h[:total], dtype=np.int32)
+        V = np.ascontiguousarray(vals_h[:total], dtype=np.float64)
+        seg = np.append(base[hit], total).astype(np.int32)
+        sys_of = np.repeat(np.arange(len(hit)), c)
+        order = (np.arange(total) - np.repeat(seg[:-1], c)).astype(np.int32)
+        first = P[:, 0]
+        tie = (sys_of[1:] == sys_of[:-1]) & (first[1:] == first[:-1])
+        if tie.any():
+            for q in np.unique(sys_of[1:][tie]).tolist():
+                a, b = int(seg[q]), int(seg[q + 1])
+                order[a:b] = np.argsort(first[a:b])
+        ro = np.zeros(len(hit) + 1, np.int32)
+        mo = np.zeros(len(hit) + 1, np.int32)
+        rm = np.zeros(max(2 * total, 1), np.int32)
+        mg = np.zeros(max(4 * total, 1), np.float64)
+        vp = ctypes.c_void_p
+        _native.call("dem_merge_plan", len(hit), seg.ctypes.data_as(vp), P.ctypes.data_as(vp),
+                     V.ctypes.data_as(vp), order.ctypes.data_as(vp), ro.ctypes.data_as(vp),
+                     rm.ctypes.data_as(vp), mo.ctypes.data_as(vp), mg.ctypes.data_as(vp))
+        return ro, rm, mo, mg
+
+    def _plan_python(self, hit, counts, base, pairs_h, vals_h):
+        """System by system with ``merge_plan_rows`` (overflowed pair lists
+        come from the single-system pass)."""
         roff, moff, removed, merged = [0], [0], [], []
         cap = self.pair_capacity
         for e in hit.tolist():
@@ -287,20 +374,11 @@ class BatchedEnsembles:
                 merged.extend(mg)
             roff.append(len(removed))
             moff.append(len(merged))
-        es = np.ascontiguousarray(hit, dtype=np.int32)
         ro = np.asarray(roff, dtype=np.int32)
         mo = np.asarray(moff, dtype=np.int32)
         rm = np.asarray(removed, dtype=np.int32)
         mg = np.ascontiguousarray(np.asarray(merged, dtype=np.float64).reshape(-1))
-        vp = ctypes.c_void_p
-        _native.call("dem_batch_merge", self._h, len(es), es.ctypes.data_as(vp),
-                     ro.ctypes.data_as(vp), rm.ctypes.data_as(vp), mo.ctypes.data_as(vp),
-                     mg.ctypes.data_as(vp), _device.stream())
-        self.n[hit] -= ro[1:] - ro[:-1]
-        self.n[hit] += mo[1:] - mo[:-1]
-        self.merges += len(hit)
-        self.merge_rounds += 1
-        return len(hit)
+        return ro, rm, mo, mg
 
     # ------------------------------------------------------------------ state
 

@@ -9,8 +9,9 @@
   system whose pairs overflow its slot still merges exactly.
 * Against the FP64 oracle: the first batched step's forces, velocities and
   displacements, under the direct kernels' accuracy contract.
-* Graph replay equals eager launches; finished systems stay put; resizing
-  beyond the capacity is rejected.
+* Graph replay equals eager launches; merge rounds overlapped with the
+  advance on a second stream equal serial rounds; finished systems stay put;
+  resizing beyond the capacity is rejected.
 """
 
 from __future__ import annotations
@@ -102,6 +103,35 @@ def test_pair_once_batched_equals_sequential_bit_for_bit(gpu, setup, jlen):
         assert np.array_equal(b.positions(e), s.positions(e)), f"system {e} positions"
         n = len(b.radii_of(e))
         np.testing.assert_array_equal(b.forces(e), s.F[e].cpu().numpy()[:n])
+    b.close()
+
+
+@pytest.mark.parametrize("pair_once", [False, True])
+def test_overlapped_merge_rounds_equal_serial_rounds(gpu, setup, pair_once):
+    """Merge rounds on a second stream beside the advance of the contact-free
+    systems (dem_batch_wait_go, dem_batch_merge_async, tables rebuilt before
+    the second advance) end in exactly the states of the serial rounds."""
+    systems, dts = setup
+    tune = Tuning(256, 1024, pair_once=pair_once)
+    a = BatchedEnsembles(systems, dts, STEPS, tune, capture=True)
+    c = BatchedEnsembles(systems, dts, STEPS, tune, capture=True)
+    c.overlap = False
+    a.run_until_done(200)
+    c.run_until_done(200)
+    assert a.merges == c.merges and a.merges > 0 and a.merge_rounds == c.merge_rounds
+    for e in range(len(systems)):
+        assert np.array_equal(a.radii_of(e), c.radii_of(e)), f"system {e} radii"
+        assert np.array_equal(a.positions(e), c.positions(e)), f"system {e} positions"
+    assert np.array_equal(a.time.cpu().numpy(), c.time.cpu().numpy())
+    a.close()
+    c.close()
+
+
+def test_wait_go_needs_an_advance(gpu, setup):
+    systems, dts = setup
+    b = BatchedEnsembles(systems[:2], dts[:2], [2, 2], Tuning(64, 256), capture=False)
+    rc = _native.load().dem_batch_wait_go(b._h, None)
+    assert rc == _native.DEM_ERR_INVALID and "advance" in _native.last_error()
     b.close()
 
 
