@@ -381,52 +381,63 @@ __global__ void k_b_ens_scatter(int count, const int* __restrict__ systems,
   if (k < count) dst[systems[k]] = src[k];
 }
 
-// Merges computed by the caller, one workgroup per listed system.  Stage
-// layout: ints [0,c) systems, [c, 2c+1) removed offsets, [2c+1, 3c+2) merged
-// offsets, then the removed local indices (ascending per system); doubles:
-// (x, y, z, r) of every merged droplet.  Survivors keep their order and the
-// merged droplets follow them (droplet_solution.py:263-268).  `scratch`
-// holds the survivors between the two passes.
-__global__ __launch_bounds__(256) void k_b_compact(const BEns* __restrict__ ens, int c,
-                                                   const int* __restrict__ stage_i,
-                                                   const double* __restrict__ stage_d,
-                                                   double* __restrict__ pos,
-                                                   double* __restrict__ radii,
-                                                   double4* __restrict__ scratch) {
-  const int k = blockIdx.x;
+// Merges computed by the caller.  Stage layout: ints [0,c) systems,
+// [c, 2c+1) removed offsets, [2c+1, 3c+2) merged offsets, then the removed
+// local indices (ascending per system); doubles: (x, y, z, r) of every merged
+// droplet.  Survivors keep their order and the merged droplets follow them
+// (droplet_solution.py:263-268).  Two passes over a grid of c x rows_max
+// workgroups of 256 droplets (ens still holds the sizes before the merge):
+// the survivors are gathered into `scratch` (a survivor's new index is i minus
+// the removed indices below it), then copied back and the merged droplets
+// appended.
+__global__ __launch_bounds__(256) void k_b_compact_gather(
+    const BEns* __restrict__ ens, int c, int rows_max, const int* __restrict__ stage_i,
+    const double* __restrict__ pos, const double* __restrict__ radii,
+    double4* __restrict__ scratch) {
+  const int k = (int)blockIdx.x / rows_max, r = (int)blockIdx.x % rows_max;
+  if (k >= c) return;
+  const int e = stage_i[k];
+  const int* roff = stage_i + c;
+  const int* R = stage_i + 3 * c + 2 + roff[k];
+  const int m = roff[k + 1] - roff[k];
+  const BEns& E = ens[e];
+  const int i = r * 256 + threadIdx.x;
+  if (i >= E.n) return;
+  int lo = 0, hi = m;  // removed indices below i
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (R[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < m && R[lo] == i) return;
+  const size_t off = (size_t)E.off, g = off + i;
+  scratch[off + (i - lo)] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
+}
+
+__global__ __launch_bounds__(256) void k_b_compact_scatter(
+    const BEns* __restrict__ ens, int c, int rows_max, const int* __restrict__ stage_i,
+    const double* __restrict__ stage_d, const double4* __restrict__ scratch,
+    double* __restrict__ pos, double* __restrict__ radii) {
+  const int k = (int)blockIdx.x / rows_max, r = (int)blockIdx.x % rows_max;
   if (k >= c) return;
   const int e = stage_i[k];
   const int* roff = stage_i + c;
   const int* moff = stage_i + 2 * c + 1;
-  const int* R = stage_i + 3 * c + 2 + roff[k];
   const int m = roff[k + 1] - roff[k];
-  const BEns& E = ens[e];
-  const int n = E.n;
-  const size_t off = (size_t)E.off;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    int lo = 0, hi = m;  // removed indices below i
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (R[mid] < i) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo < m && R[lo] == i) continue;
-    const size_t g = off + i;
-    scratch[off + (i - lo)] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
-  }
-  __syncthreads();
-  const int ns = n - m;
-  for (int q = threadIdx.x; q < ns; q += 256) {
-    const double4 v = scratch[off + q];
-    pos[3 * (off + q)] = v.x;
-    pos[3 * (off + q) + 1] = v.y;
-    pos[3 * (off + q) + 2] = v.z;
-    radii[off + q] = v.w;
-  }
   const int m0 = moff[k], mc = moff[k + 1] - moff[k];
-  for (int q = threadIdx.x; q < mc; q += 256) {
-    const double* v = stage_d + 4 * (size_t)(m0 + q);
-    const size_t g = off + ns + q;
+  const BEns& E = ens[e];
+  const int ns = E.n - m;
+  const int q = r * 256 + threadIdx.x;
+  if (q >= ns + mc) return;
+  const size_t off = (size_t)E.off, g = off + q;
+  if (q < ns) {
+    const double4 v = scratch[off + q];
+    pos[3 * g] = v.x;
+    pos[3 * g + 1] = v.y;
+    pos[3 * g + 2] = v.z;
+    radii[g] = v.w;
+  } else {
+    const double* v = stage_d + 4 * (size_t)(m0 + q - ns);
     pos[3 * g] = v[0];
     pos[3 * g + 1] = v[1];
     pos[3 * g + 2] = v[2];
@@ -1516,10 +1527,17 @@ static int merge_systems(dem_batch_t b, int count, const int32_t* systems,
   if (nm)
     DEM_CHECK_HIP(hipMemcpyAsync(b->l.stage_d, merged, sizeof(double) * 4 * nm,
                                  hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_b_compact, dim3(count), dim3(256), 0, st, (const BEns*)b->l.ens, count,
-                     (const int*)b->l.stage_i, (const double*)b->l.stage_d, b->pos, b->radii,
-                     b->l.spos);
-  DEM_CHECK_LAUNCH("k_b_compact");
+  const int rows_max = std::max(1, (b->cap_max + kRow - 1) / kRow);
+  const long long blocks = (long long)count * rows_max;
+  if (blocks > 0x7fffffffLL) return dem::set_error(DEM_ERR_CAPACITY, "too many rows to merge");
+  hipLaunchKernelGGL(k_b_compact_gather, dim3((unsigned)blocks), dim3(256), 0, st,
+                     (const BEns*)b->l.ens, count, rows_max, (const int*)b->l.stage_i,
+                     (const double*)b->pos, (const double*)b->radii, b->l.spos);
+  DEM_CHECK_LAUNCH("k_b_compact_gather");
+  hipLaunchKernelGGL(k_b_compact_scatter, dim3((unsigned)blocks), dim3(256), 0, st,
+                     (const BEns*)b->l.ens, count, rows_max, (const int*)b->l.stage_i,
+                     (const double*)b->l.stage_d, (const double4*)b->l.spos, b->pos, b->radii);
+  DEM_CHECK_LAUNCH("k_b_compact_scatter");
   for (int k = 0; k < count; ++k) b->desc[systems[k]].n = new_n[k];
   if (!defer) {
     int rc = upload_tables(b, st);  // synchronises: the host lists may be freed after return
@@ -1544,9 +1562,6 @@ static int merge_systems(dem_batch_t b, int count, const int32_t* systems,
   hipLaunchKernelGGL(k_b_cells_list, dim3(count), dim3(256), 0, st, (const BEns*)l.ens, count,
                      (const int*)l.stage_i, (const double*)b->radii, l.dev);
   DEM_CHECK_LAUNCH("k_b_cells_list");
-  const int rows_max = std::max(1, (b->cap_max + kRow - 1) / kRow);
-  const long long blocks = (long long)count * rows_max;
-  if (blocks > 0x7fffffffLL) return dem::set_error(DEM_ERR_CAPACITY, "too many rows to pack");
   hipLaunchKernelGGL(k_b_pack_list, dim3((unsigned)blocks), dim3(kRow), 0, st,
                      (const BEns*)l.ens, (const BDev*)l.dev, count, rows_max,
                      (const int*)l.stage_i, (const double*)b->pos, (const double*)b->radii, l.rec,
