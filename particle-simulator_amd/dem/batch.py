@@ -35,7 +35,7 @@ import numpy as np
 from . import _device, _native
 from .collision_detector import SpatialHashCollisionDetector
 from .force_calculator import DirectDropletForceCalculator
-from .solution.droplet_solution import merge_collided, merge_plan_rows
+from .solution.droplet_solution import merge_plan, merge_plan_rows
 
 BLOCKS = (64, 128, 256)
 JLENS = (1024, 2048, 4096, 8192)
@@ -421,7 +421,10 @@ class SequentialEnsembles:
 
     Per step of system e: contact pairs (``dem_hash_pairs``; the pair count
     crosses to the host), merge and re-check while there are contacts, then
-    ``dem_direct_forces_and_convection`` and ``dem_euler_step``.  With
+    ``dem_direct_forces_and_convection`` and ``dem_euler_step``.  A merge is
+    the device-resident loop's: the greedy plan on the host from the pair
+    list and the pair droplets' values (``merge_plan``), the compaction on
+    the device (``dem_merge_compact``).  With
     ``jlen`` set, each force evaluation uses the batched split
     (``split_for(n, jlen, pair_once)``; pair-once mode 2 when ``pair_once``),
     which makes its sums bit-identical to ``BatchedEnsembles`` with the same
@@ -463,10 +466,8 @@ class SequentialEnsembles:
             k, pairs = self.det[e].detect_device(self.pos[e], self.radii[e], L, True)
             if k == 0:
                 break
-            new_p, new_r, _ = merge_collided(self.pos[e].cpu().numpy(),
-                                             self.radii[e].cpu().numpy(), pairs.cpu().numpy())
-            self.pos[e] = _device.to_device(new_p)
-            self.radii[e] = _device.to_device(new_r)
+            self.pos[e], self.radii[e] = self._merge(self.pos[e], self.radii[e],
+                                                     pairs.cpu().numpy())
             self.merges += 1
         n = int(self.radii[e].shape[0])
         prev = prev_sym = None
@@ -486,6 +487,37 @@ class SequentialEnsembles:
                      _device.stream())
         self.F[e], self.V[e] = F, V
         self.steps_done[e] += 1
+
+    @staticmethod
+    def _merge(pos_d, r_d, pairs):
+        """``merge_collided`` of the device state: plan on the host over the
+        droplets named in the pairs (the reference's float64 arithmetic),
+        compaction on the device."""
+        t = _device.torch()
+        ids = np.unique(pairs)
+        sel = t.from_numpy(ids.astype(np.int64)).to(pos_d.device)
+        p_h = pos_d.index_select(0, sel).cpu().numpy()
+        r_h = r_d.index_select(0, sel).cpu().numpy()
+        at = {int(i): q for q, i in enumerate(ids.tolist())}
+        removed, new_r, new_p, _ = merge_plan(pairs, lambda i: r_h[at[int(i)]],
+                                              lambda i: p_h[at[int(i)]])
+        rem = np.asarray(sorted(removed), dtype=np.int32)
+        m = len(new_r)
+        merged = np.empty((m, 4), dtype=np.float64)
+        merged[:, :3] = np.asarray(new_p, dtype=np.float64).reshape(m, 3)
+        merged[:, 3] = np.asarray(new_r, dtype=np.float64)
+        n = int(pos_d.shape[0])
+        n_out = n - len(rem) + m
+        rem_d = _device.to_device(rem, np.int32)
+        merged_d = _device.to_device(merged)
+        out_p = _device.empty((n_out, 3))
+        out_r = _device.empty((n_out,))
+        _native.call("dem_merge_compact", _device.ptr(pos_d), _device.ptr(r_d), n,
+                     _device.ptr(rem_d), len(rem), _device.ptr(merged_d), m,
+                     _device.ptr(out_p), _device.ptr(out_r), _device.stream())
+        # the inputs of the launch stay referenced until it has run
+        t.cuda.current_stream().synchronize()
+        return out_p, out_r
 
     def step_all(self):
         for e in range(self.n_ens):
