@@ -202,6 +202,12 @@ int dem_merge_compact(const double* pos, const double* radii, int n,
                       int n_merged, double* out_pos, double* out_radii,
                       dem_stream_t stream);
 
+/* out[4 q ..] = (x, y, z, r) of droplet idx[q], q < k (NaN for an index
+ * outside [0, n)): the droplet values a host-side merge plan reads
+ * (droplet_solution.py:230-250).  Device pointers. */
+int dem_gather_droplets(const double* pos, const double* radii, int n, const int32_t* idx,
+                        int k, double* out, dem_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Flat Barnes-Hut octree                                                     */
 /* ------------------------------------------------------------------------ */

@@ -71,9 +71,35 @@ __global__ __launch_bounds__(256) void k_merge_compact(
   }
 }
 
+// (x, y, z, r) of the droplets idx[0 .. k) (the values a merge plan reads).
+__global__ __launch_bounds__(256) void k_gather_droplets(const double* __restrict__ pos,
+                                                         const double* __restrict__ r, int n,
+                                                         const int32_t* __restrict__ idx, int k,
+                                                         double* __restrict__ out) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= k) return;
+  const int i = idx[q];
+  const bool ok = i >= 0 && i < n;
+  out[4 * (size_t)q] = ok ? pos[3 * (size_t)i] : NAN;
+  out[4 * (size_t)q + 1] = ok ? pos[3 * (size_t)i + 1] : NAN;
+  out[4 * (size_t)q + 2] = ok ? pos[3 * (size_t)i + 2] : NAN;
+  out[4 * (size_t)q + 3] = ok ? r[i] : NAN;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dem_gather_droplets(const double* pos, const double* radii, int n, const int32_t* idx, int k,
+                        double* out, dem_stream_t stream) {
+  if (n < 0 || k < 0) return dem::set_error(DEM_ERR_INVALID, "bad counts (n %d, k %d)", n, k);
+  if (k == 0) return DEM_OK;
+  if (!pos || !radii || !idx || !out) return dem::set_error(DEM_ERR_INVALID, "null argument");
+  hipLaunchKernelGGL(k_gather_droplets, dim3((k + 255) / 256), dim3(256), 0,
+                     dem::as_stream(stream), pos, radii, n, idx, k, out);
+  DEM_CHECK_LAUNCH("k_gather_droplets");
+  return DEM_OK;
+}
 
 int dem_merge_compact(const double* pos, const double* radii, int n, const int32_t* removed,
                       int n_removed, const double* merged, int n_merged, double* out_pos,

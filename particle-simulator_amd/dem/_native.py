@@ -127,6 +127,7 @@ SIGNATURES: dict[str, tuple] = {
     "dem_euler_step": (_I, [_P, _P, _P, _P, _I, _D, _D, _D, _I, _P]),
     "dem_euler_step_total": (_I, [_P, _P, _I, _D, _D, _I, _P]),
     "dem_merge_compact": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P]),
+    "dem_gather_droplets": (_I, [_P, _P, _I, _P, _I, _P, _P]),
     "dem_octree_workspace_bytes": (_S, [_I, _I]),
     "dem_octree_build": (_I, [_P, _P, _I, _D, _I, _I, _D, _P, _P, _S, _P]),
     "dem_octree_forces": (_I, [_P, _D, _D, _P, _P, _S, _P]),

@@ -494,10 +494,14 @@ class SequentialEnsembles:
         droplets named in the pairs (the reference's float64 arithmetic),
         compaction on the device."""
         t = _device.torch()
-        ids = np.unique(pairs)
-        sel = t.from_numpy(ids.astype(np.int64)).to(pos_d.device)
-        p_h = pos_d.index_select(0, sel).cpu().numpy()
-        r_h = r_d.index_select(0, sel).cpu().numpy()
+        ids = np.unique(pairs).astype(np.int32)
+        n = int(pos_d.shape[0])
+        ids_d = _device.to_device(ids, np.int32)
+        vals_d = _device.empty((len(ids), 4))
+        _native.call("dem_gather_droplets", _device.ptr(pos_d), _device.ptr(r_d), n,
+                     _device.ptr(ids_d), len(ids), _device.ptr(vals_d), _device.stream())
+        vals = vals_d.cpu().numpy()
+        p_h, r_h = vals[:, :3], vals[:, 3]
         at = {int(i): q for q, i in enumerate(ids.tolist())}
         removed, new_r, new_p, _ = merge_plan(pairs, lambda i: r_h[at[int(i)]],
                                               lambda i: p_h[at[int(i)]])
@@ -506,7 +510,6 @@ class SequentialEnsembles:
         merged = np.empty((m, 4), dtype=np.float64)
         merged[:, :3] = np.asarray(new_p, dtype=np.float64).reshape(m, 3)
         merged[:, 3] = np.asarray(new_r, dtype=np.float64)
-        n = int(pos_d.shape[0])
         n_out = n - len(rem) + m
         rem_d = _device.to_device(rem, np.int32)
         merged_d = _device.to_device(merged)

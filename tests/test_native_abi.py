@@ -60,6 +60,11 @@ def test_invalid_arguments_are_rejected_before_any_launch():
     rc = lib.dem_merge_compact(ctypes.c_void_p(8), ctypes.c_void_p(8), 4, ctypes.c_void_p(8), 2,
                                ctypes.c_void_p(8), 1, None, None, None)
     assert rc == _native.DEM_ERR_INVALID and "null" in _native.last_error()
+    rc = lib.dem_gather_droplets(ctypes.c_void_p(8), ctypes.c_void_p(8), 4, None, 3,
+                                 ctypes.c_void_p(8), None)
+    assert rc == _native.DEM_ERR_INVALID and "null" in _native.last_error()
+    rc = lib.dem_gather_droplets(None, None, -1, None, 0, None, None)
+    assert rc == _native.DEM_ERR_INVALID and "bad counts" in _native.last_error()
 
 
 def test_zero_particles_is_a_no_op():
