@@ -112,6 +112,20 @@ def differentiable() -> System:
     return make_system(4096, seed=0, phi=0.02)
 
 
+def differentiable_dt(s: System, E: float, mu: float, eps_oil: float = 2.85,
+                      mu_water: float = 0.001) -> float:
+    """dt of the differentiable workload: the fastest droplet moves 0.1 r_min
+    in the first step (velocities of the FP32 direct path)."""
+    from .force_calculator import DirectDropletForceCalculator
+
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=eps_oil, eta_oil=mu,
+                                      eta_water=mu_water, E=E, L=s.L)
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    sf = 2 * np.pi * mu * (2 * mu + 3 * mu_water) / (mu + mu_water)
+    v = np.linalg.norm(F / (sf * s.radii[:, None]) + V, axis=1)
+    return float(0.1 * s.radii.min() / v.max())
+
+
 def ensemble_specs(count: int = 256, base_seed: int = 0):
     """(N_e, E_e, phi_e) of the 256-ensemble batch: N_e ~ U{500..20000},
     E_e ~ U[1e5, 5e5] V/m, phi_e ~ U[0.01, 0.05], from default_rng(base_seed)."""

@@ -22,7 +22,7 @@ import torch
 from dem import _native
 from dem.adjoint import DirectTrajectory, contact_loss, rollout
 from dem.force_calculator import DirectDropletForceCalculator
-from dem.workloads import make_system
+from dem.workloads import differentiable, differentiable_dt, make_system
 from oracle import rollout64 as r64
 
 pytestmark = pytest.mark.gpu
@@ -164,3 +164,42 @@ def test_trajectory_gradients_match_autograd_and_fd(gpu):
     assert _rel(E.grad.item(), fE) <= 1e-4
     assert _rel(mu.grad.item(), fmu) <= 1e-4
     assert _rel(float((x0.grad * v).sum()), fx) <= 1e-4
+
+
+def test_full_workload_gradients_match_autograd_and_fd(gpu):
+    """The differentiable workload itself (N = 4096, T = 200, checkpoints
+    every 20 steps): gradients w.r.t. E, mu_oil and x0 within 1e-4 of
+    autograd through the plain FP64 path and of FP64 central differences."""
+    s = differentiable()
+    dev = torch.device("cuda")
+    E0, mu0, T = 3e5, 0.05, 200
+    dt = differentiable_dt(s, E0, mu0, EPS, MU_W)
+    traj = DirectTrajectory(s.radii, s.L, dt, T, eps_oil=EPS, eta_water=MU_W,
+                            checkpoint_every=20)
+    x0 = torch.tensor(s.positions, device=dev, requires_grad=True)
+    E = torch.tensor(E0, dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(mu0, dtype=torch.float64, requires_grad=True)
+    loss = contact_loss(rollout(x0, E, mu, traj), traj)
+    loss.backward()
+    r = torch.tensor(s.radii, device=dev)
+    x64 = torch.tensor(s.positions, device=dev, requires_grad=True)
+    E64 = torch.tensor(E0, dtype=torch.float64, device=dev, requires_grad=True)
+    mu64 = torch.tensor(mu0, dtype=torch.float64, device=dev, requires_grad=True)
+    L64 = _fp64_loss(x64, r, E64, mu64, dt, T, s.L)
+    L64.backward()
+    errs = {"loss": _rel(loss.item(), L64.item()),
+            "E": _rel(E.grad.item(), E64.grad.item()),
+            "mu": _rel(mu.grad.item(), mu64.grad.item()),
+            "x0_field": _field_err(x0.grad.cpu().numpy(), x64.grad.cpu().numpy())}
+    x0d = torch.tensor(s.positions, device=dev)
+    g = torch.Generator(dev).manual_seed(3)
+    v = torch.randn(s.n, 3, dtype=torch.float64, device=dev, generator=g)
+    args = (x0d, r, E0, mu0, EPS, MU_W, dt, T, s.L)
+    fE, hE = r64.switch_free_derivative(*args, "E")
+    fmu, hmu = r64.switch_free_derivative(*args, "mu")
+    fx, hx = r64.switch_free_derivative(*args, "x0", direction=v)
+    assert None not in (fE, fmu, fx), "every step straddles a contact or image switch"
+    errs.update({"fd_E": _rel(E.grad.item(), fE), "fd_mu": _rel(mu.grad.item(), fmu),
+                 "fd_x0_dir": _rel(float((x0.grad * v).sum()), fx)})
+    print(f"\n[adjoint N={s.n} T={T}] h E {hE:g}, mu {hmu:g}, x0 {hx:g}; errors {errs}")
+    assert max(errs.values()) <= 1e-4, errs
