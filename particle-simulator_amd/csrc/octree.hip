@@ -773,11 +773,12 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
           for (int u = 0; u < 2; ++u) {
             const RecF& sj = u == 0 ? two.a : two.b;
             const int jj = j + u;
-            // tree leaves keep r^2 >= (r_i + r_j)^2 (:835); j == i adds nothing
+            // tree leaves keep r^2 >= (r_i + r_j)^2 (:835), which also drops
+            // j == i (r^2 = 0); the spare past the leaf adds nothing
             for (int h = 0; h < TPL; ++h)
               dem::dipole_pair<true>(dem::diff32(sj.x, me[h].x), dem::diff32(sj.y, me[h].y),
-                                     dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3,
-                                     jj < e && jj != t[h], f[h][0], f[h][1], f[h][2]);
+                                     dem::diff32(sj.z, me[h].z), me[h].r, sj.r, sj.r3, jj < e,
+                                     f[h][0], f[h][1], f[h][2]);
           }
         }
         for (int h = 0; h < TPL; ++h)
@@ -842,6 +843,7 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
   const int t = live ? t0 : wa.n - 1;
   const Scales sc = *scp;
   const RecF me = wa.srec[t];
+  const float rif = me.r * 0x1p22f;  // radius, fixed units
   const double px = wa.spos[3 * (size_t)t], py = wa.spos[3 * (size_t)t + 1],
                pz = wa.spos[3 * (size_t)t + 2];
   double ax = 0.0, ay = 0.0, az = 0.0;  // Oseen sum, fixed units
@@ -873,11 +875,10 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
             const float dx = dem::diff32(sj.x, me.x);
             const float dy = dem::diff32(sj.y, me.y);
             const float dz = dem::diff32(sj.z, me.z);
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            // >= (:1055); j == i, contacts and the spare past the leaf add nothing
-            const bool keep =
-                jj < e && jj != t && dem::separated<true>(r2 * dem::kSigma2f, me.r + sj.r);
-            dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, keep, vx, vy, vz);
+            // >= (:1055); j == i, contacts and the spare past the leaf add
+            // nothing.  2^22 (r_j + r_i) in one rounding, as 2^22 fl(r_i + r_j).
+            const bool keep = dem::oseen_member(dx, dy, dz, Fx, Fy, Fz,
+                                                fmaf(sj.r, 0x1p22f, rif), jj < e, vx, vy, vz);
             if (CORR && keep) dem::corr_lookup(ca.g, dx, dy, dz, Fx, Fy, Fz, ux, uy, uz);
           }
         }

@@ -40,16 +40,18 @@ __device__ __forceinline__ void dipole_pair(float dx, float dy, float dz, float 
                                             float& fz) {
   const float dz2 = dz * dz;
   const float dxy2 = dx * dx + dy * dy;
-  const float r2 = (dxy2 + dz2) * kSigma2f;
+  const float rr = dxy2 + dz2;  // |D|^2, fixed units; r2_work's rounding
+  const float r2 = rr * kSigma2f;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float ir2 = ir * ir;
   const float ir7 = (ir2 * ir2) * (ir2 * ir);
   // select, never 0*inf
   const float s7 = (keep && separated<INCLUSIVE>(r2, ri + rj)) ? rj3 * ir7 : 0.f;
-  const float txy = s7 * (4.f * dz2 - dxy2);
+  const float A = 4.f * dz2 - dxy2;
+  const float txy = s7 * A;
   fx += txy * dx;
   fy += txy * dy;
-  fz += (s7 * (2.f * dz2 - 3.f * dxy2)) * dz;
+  fz += (s7 * fmaf(-2.f, rr, A)) * dz;  // 2 dz^2 - 3 (dx^2 + dy^2)
 }
 
 template <bool INCLUSIVE>
@@ -63,14 +65,15 @@ __device__ __forceinline__ void dipole_monopole(float dx, float dy, float dz, fl
                                                 float& fy, float& fz) {
   const float dz2 = dz * dz;
   const float dxy2 = dx * dx + dy * dy;
-  const float r2 = (dxy2 + dz2) * kSigma2f;
-  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float rr = dxy2 + dz2;
+  const float ir = __builtin_amdgcn_rsqf(rr * kSigma2f);
   const float ir2 = ir * ir;
   const float s7 = w3 * ((ir2 * ir2) * (ir2 * ir));
-  const float txy = s7 * (4.f * dz2 - dxy2);
+  const float A = 4.f * dz2 - dxy2;
+  const float txy = s7 * A;
   fx += txy * dx;
   fy += txy * dy;
-  fz += (s7 * (2.f * dz2 - 3.f * dxy2)) * dz;
+  fz += (s7 * fmaf(-2.f, rr, A)) * dz;
 }
 
 // Oseen tensor applied to source force (Fx,Fy,Fz); keep == false adds nothing.
@@ -82,6 +85,23 @@ __device__ __forceinline__ void oseen_pair(float dx, float dy, float dz, float F
   vx += ir * (Fx + t * dx);
   vy += ir * (Fy + t * dy);
   vz += ir * (Fz + t * dz);
+}
+
+// Oseen term of a tree-leaf member (fixed units) with the leaf contact rule
+// |D| >= r_i + r_j tested on fixed-unit radii (rsf = (r_i + r_j) 2^22): the
+// work-unit test scaled by a power of two, so it decides the same.  The
+// target itself (D = 0) fails it.  Returns whether the member was kept.
+__device__ __forceinline__ bool oseen_member(float dx, float dy, float dz, float Fx, float Fy,
+                                             float Fz, float rsf, bool keep, float& vx, float& vy,
+                                             float& vz) {
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const bool sep = keep && r2 >= rsf * rsf;
+  const float ir = sep ? __builtin_amdgcn_rsqf(r2) : 0.f;
+  const float t = (dx * Fx + dy * Fy + dz * Fz) * (ir * ir);
+  vx += ir * (Fx + t * dx);
+  vy += ir * (Fy + t * dy);
+  vz += ir * (Fz + t * dz);
+  return sep;
 }
 
 // High word (floor) of the 64-bit minimum-image difference qj - qi, as float.
