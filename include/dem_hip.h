@@ -239,6 +239,14 @@ typedef struct dem_tree_info {
 
 size_t dem_octree_workspace_bytes(int n_capacity, int max_nodes);
 
+/* Opening tests of the walks: by default an FP32 test on fixed-point
+ * separations decides wherever its error band allows and the reference's
+ * float64 test (flat_tree.py:871, :1154, :1479) decides the rest, so the
+ * interaction sets are the float64 test's.  on = 1 sends every test to the
+ * float64 path (a checking switch for tests).  Returns the previous setting.
+ * No counterpart in the reference. */
+int dem_octree_set_exact_mac(int on);
+
 /* Builds the tree of the root cube [0, L]^3 (FlatOctree.build,
  * flat_tree.py:180-246): same node numbering (breadth-first, children in
  * octant order, split when count > mpl, capacity clamp), leaf slots, next

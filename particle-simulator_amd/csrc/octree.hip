@@ -48,18 +48,31 @@ constexpr double kSigma3 = 0x1p-66;
 constexpr double kR3Skip = 1e-30;   // flat_tree.py:29
 constexpr double kFsqSkip = 1e-60;  // flat_tree.py:32
 
-enum NodeFlags { kSkipR3 = 1, kSkipF = 2 };
+// kExactMac: a node smaller than 2^22 fixed units, whose opening test always
+// takes the float64 path (its fixed-point centre is not precise enough for
+// the far-field separation).
+int g_exact_mac = 0;  // test switch: every opening test on the float64 path
 
-// Per-node walk record (64 B).
+enum NodeFlags { kSkipR3 = 1, kSkipF = 2, kExactMac = 4 };
+
+// Per-node walk record (64 B): what the walks read at every visit.
 struct alignas(16) NodeW {
-  double cx, cy, cz;  // R^3-weighted centre (geometric centre if R3_sum == 0)
-  double s2;          // s*s, s = largest bbox extent
+  uint32_t qx, qy, qz;  // centre in fixed point, quantized as the particles
+  float s2f;            // s^2 in fixed units^2, s = largest bbox extent
+  float sf;             // s in fixed units
   int first_child, next, count, mstart;
-  float r3w;          // R3_sum in work units^3
+  float r3w;            // R3_sum in work units^3
   int flags;
-  int pad0, pad1;
+  int pad[5];
 };
 static_assert(sizeof(NodeW) == 64, "node record");
+
+// Per-node float64 centre and s^2: the reference's opening test, read when
+// the FP32 test cannot decide.
+struct alignas(16) NodeC {
+  double cx, cy, cz;  // R^3-weighted centre (geometric centre if R3_sum == 0)
+  double s2;
+};
 
 // Per-node Stokeslet data: force sum / Fs and the centred dipole tensor
 // D_ab = P_ab - cm_a F_b divided by Fs*U (fixed units), kept as the parts the
@@ -74,6 +87,27 @@ struct alignas(16) NodeF {
   float pad[3];
 };
 static_assert(sizeof(NodeF) == 64, "node force record");
+
+// FP32 opening test of a node at a target from the fixed-point separation R
+// (centre - x_i, fixed units): 1 take, 0 open, -1 undecided.  With both
+// points quantized to within half a unit, |R|^2 is within 2^-21 |D|^2 +
+// 3.5 |D| + 3.1 of the exact |D|^2, so m = theta^2 |R|^2 - s^2 is within
+// band = 2^-17 s^2 + 8 theta s + 8 of the exact value wherever
+// |D| <= 2 s / theta; farther out m > 0 with a margin far above the error.
+// A decision outside the band is therefore the float64 test's (s^2 <
+// theta^2 d^2, d^2 > 0).  A component within 2^11 units of half the box
+// could take the other periodic image than the float64 minimum image:
+// undecided too.
+__device__ __forceinline__ int fast_open_test(float Rx, float Ry, float Rz, float th2f,
+                                              float band_s, float s2f, float sf) {
+  constexpr float kHalfBand = 2147481600.f;  // 2^31 - 2^11
+  const float d2 = Rx * Rx + Ry * Ry + Rz * Rz;
+  const float m = fmaf(th2f, d2, -s2f);
+  const float band = fmaf(0x1p-17f, s2f, fmaf(band_s, sf, 8.f));
+  const float amax = fmaxf(fabsf(Rx), fmaxf(fabsf(Ry), fabsf(Rz)));
+  if (!(fabsf(m) > band) || amax >= kHalfBand) return -1;
+  return m > 0.f ? 1 : 0;
+}
 
 // Far-field velocity of a node at a target (FP32, fixed units): the Oseen
 // monopole of the force sum and the dipole correction (flat_tree.py:
@@ -163,6 +197,7 @@ struct TreeLayout {
   int* rank;                     // [M]
   int* leafcnt;                  // [M]
   NodeW* nw;                     // [M]
+  NodeC* nc;                     // [M]
   NodeF* nf;                     // [M]
   int* counter;                  // [16]
   // particles (N = capacity)
@@ -205,6 +240,7 @@ TreeLayout carve(void* ws, int ncap, int M) {
   l.rank = c.take<int>(m);
   l.leafcnt = c.take<int>(m);
   l.nw = c.take<NodeW>(m);
+  l.nc = c.take<NodeC>(m);
   l.nf = c.take<NodeF>(m);
   l.counter = c.take<int>(16);
   l.keys = c.take<unsigned long long>(N);
@@ -560,6 +596,19 @@ __global__ void k_scales(const double* __restrict__ bbox, int nblk, double L, in
   *sc = s;
 }
 
+// Fixed-point coordinate of x along axis a (the sorted records' and the node
+// centres' quantization): periodic, the fraction of the box times 2^32;
+// otherwise units of U from the bounding-box origin.
+__device__ __forceinline__ uint32_t quantize(double x, int a, const Scales& sc) {
+#pragma clang fp contract(off)
+  if (sc.periodic) {
+    double u = x * sc.inv_U * 0x1p-32;
+    u -= floor(u);
+    return (uint32_t)(unsigned long long)rint(u * kTwo32);
+  }
+  return (uint32_t)(long long)rint((x - sc.origin[a]) * sc.inv_U);
+}
+
 // Sorted particle data: float64 positions / radii and fixed-point records.
 __global__ __launch_bounds__(256) void k_sorted_particles(
     const double* __restrict__ pos, const double* __restrict__ radii, int n,
@@ -574,13 +623,7 @@ __global__ __launch_bounds__(256) void k_sorted_particles(
   for (int a = 0; a < 3; ++a) {
     const double x = pos[3 * (size_t)o + a];
     spos[3 * (size_t)p + a] = x;
-    if (sc.periodic) {
-      double u = x * sc.inv_U * 0x1p-32;
-      u -= floor(u);
-      q[a] = (uint32_t)(unsigned long long)rint(u * kTwo32);
-    } else {
-      q[a] = (uint32_t)(long long)rint((x - sc.origin[a]) * sc.inv_U);
-    }
+    q[a] = quantize(x, a, sc);
   }
   const double r = radii[o];
   srad[p] = r;
@@ -600,26 +643,36 @@ __global__ void k_node_walk(int nn, const double* __restrict__ bmin, const doubl
                             const double* __restrict__ r3sum, const double* __restrict__ r3c,
                             const int* __restrict__ first_child, const int* __restrict__ next,
                             const int* __restrict__ count, const int* __restrict__ mstart,
-                            const Scales* __restrict__ scp, NodeW* __restrict__ nw) {
+                            const Scales* __restrict__ scp, NodeW* __restrict__ nw,
+                            NodeC* __restrict__ nc) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
+  const Scales sc = *scp;
   NodeW w;
+  NodeC c;
   const double s3 = r3sum[i];
   if (s3 > 0.0) {
     const double inv = 1.0 / s3;
-    w.cx = r3c[3 * i] * inv;
-    w.cy = r3c[3 * i + 1] * inv;
-    w.cz = r3c[3 * i + 2] * inv;
+    c.cx = r3c[3 * i] * inv;
+    c.cy = r3c[3 * i + 1] * inv;
+    c.cz = r3c[3 * i + 2] * inv;
   } else {
-    w.cx = (bmin[3 * i] + bmax[3 * i]) * 0.5;
-    w.cy = (bmin[3 * i + 1] + bmax[3 * i + 1]) * 0.5;
-    w.cz = (bmin[3 * i + 2] + bmax[3 * i + 2]) * 0.5;
+    c.cx = (bmin[3 * i] + bmax[3 * i]) * 0.5;
+    c.cy = (bmin[3 * i + 1] + bmax[3 * i + 1]) * 0.5;
+    c.cz = (bmin[3 * i + 2] + bmax[3 * i + 2]) * 0.5;
   }
   const double ex = bmax[3 * i] - bmin[3 * i], ey = bmax[3 * i + 1] - bmin[3 * i + 1],
                ez = bmax[3 * i + 2] - bmin[3 * i + 2];
   const double s = fmax(fmax(ex, ey), ez);
-  w.s2 = s * s;
+  c.s2 = s * s;
+  nc[i] = c;
+  w.qx = quantize(c.cx, 0, sc);
+  w.qy = quantize(c.cy, 1, sc);
+  w.qz = quantize(c.cz, 2, sc);
+  const double su = s * sc.inv_U;
+  w.sf = (float)su;
+  w.s2f = (float)(su * su);
   w.first_child = first_child[i];
   w.next = next[i];
   w.count = count[i];
@@ -627,14 +680,16 @@ __global__ void k_node_walk(int nn, const double* __restrict__ bmin, const doubl
   const double lam = scp->lambda;
   w.r3w = (float)(s3 / (lam * lam * lam));
   w.flags = (count[i] == 0 || s3 < kR3Skip) ? kSkipR3 : 0;
-  w.pad0 = w.pad1 = 0;
+  if (!(su >= 0x1p22)) w.flags |= kExactMac;
+  for (int k = 0; k < 5; ++k) w.pad[k] = 0;
   nw[i] = w;
 }
 
 // Stokeslet node data + the skip flag on the force sum (:1021-1029, :1215-1240).
 __global__ void k_node_force(int nn, const double* __restrict__ fsum, const double* __restrict__ fmom,
-                             const NodeW* __restrict__ nwr, const Scales* __restrict__ scp,
-                             NodeF* __restrict__ nf, int* __restrict__ flags_out) {
+                             const NodeW* __restrict__ nwr, const NodeC* __restrict__ ncr,
+                             const Scales* __restrict__ scp, NodeF* __restrict__ nf,
+                             int* __restrict__ flags_out) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
@@ -642,11 +697,12 @@ __global__ void k_node_force(int nn, const double* __restrict__ fsum, const doub
   const double Fx = fsum[3 * (size_t)i], Fy = fsum[3 * (size_t)i + 1], Fz = fsum[3 * (size_t)i + 2];
   const double fs2 = Fx * Fx + Fy * Fy + Fz * Fz;
   const NodeW w = nwr[i];
+  const NodeC c = ncr[i];
   NodeF o;
   o.fx = (float)(Fx / sc.Fs);
   o.fy = (float)(Fy / sc.Fs);
   o.fz = (float)(Fz / sc.Fs);
-  const double cm[3] = {w.cx, w.cy, w.cz};
+  const double cm[3] = {c.cx, c.cy, c.cz};
   const double F[3] = {Fx, Fy, Fz};
   const double k = 1.0 / (sc.Fs * sc.U);
   double D[9];
@@ -693,6 +749,7 @@ struct CorrArgs {
 
 struct WalkArgs {
   const NodeW* nw;
+  const NodeC* nc;
   const NodeF* nf;
   const RecF* srec;
   const double* spos;
@@ -703,6 +760,9 @@ struct WalkArgs {
   double L;
   int periodic;
   double theta_sq;
+  float th2f;     // fl32(theta^2)
+  float band_s;   // 8 theta (rounded up): the band's term in s
+  int exact_mac;  // every opening test on the float64 path
 };
 
 // Wave-coherent walks.  The 64 lanes of a wave (targets consecutive in
@@ -789,18 +849,30 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
       continue;
     }
     bool open = false;
+    const bool exact = wa.exact_mac || (nd.flags & kExactMac);
     for (int h = 0; h < TPL; ++h) {
       if (!act[h]) continue;
-      double dx = mic64(nd.cx - p[h][0], wa.L, wa.periodic);
-      double dy = mic64(nd.cy - p[h][1], wa.L, wa.periodic);
-      double dz = mic64(nd.cz - p[h][2], wa.L, wa.periodic);
-      double d2 = dx * dx;
-      d2 = d2 + dy * dy;
-      d2 = d2 + dz * dz;
-      if (nd.s2 < wa.theta_sq * d2 && d2 > 0.0) {
+      // R = centre - x_i (fixed units)
+      float Rx = dem::diff32(nd.qx, me[h].x), Ry = dem::diff32(nd.qy, me[h].y),
+            Rz = dem::diff32(nd.qz, me[h].z);
+      int take = exact ? -1 : fast_open_test(Rx, Ry, Rz, wa.th2f, wa.band_s, nd.s2f, nd.sf);
+      if (take < 0) {
+        // the reference's float64 test (:871)
+        const NodeC c = wa.nc[node];
+        double dx = mic64(c.cx - p[h][0], wa.L, wa.periodic);
+        double dy = mic64(c.cy - p[h][1], wa.L, wa.periodic);
+        double dz = mic64(c.cz - p[h][2], wa.L, wa.periodic);
+        double d2 = dx * dx;
+        d2 = d2 + dy * dy;
+        d2 = d2 + dz * dz;
+        take = c.s2 < wa.theta_sq * d2 && d2 > 0.0;
+        Rx = (float)(dx * sc.inv_U);
+        Ry = (float)(dy * sc.inv_U);
+        Rz = (float)(dz * sc.inv_U);
+      }
+      if (take) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        dem::dipole_monopole((float)(dx * sc.inv_U), (float)(dy * sc.inv_U),
-                             (float)(dz * sc.inv_U), nd.r3w, fx, fy, fz);
+        dem::dipole_monopole(Rx, Ry, Rz, nd.r3w, fx, fy, fz);
         acc[h][0] += (double)fx;
         acc[h][1] += (double)fy;
         acc[h][2] += (double)fz;
@@ -894,19 +966,29 @@ __global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* 
       node = wave_node(nd.next);
       continue;
     }
-    // x_i - centre (:1139-1150)
-    double dx = mic64(px - nd.cx, wa.L, wa.periodic);
-    double dy = mic64(py - nd.cy, wa.L, wa.periodic);
-    double dz = mic64(pz - nd.cz, wa.L, wa.periodic);
-    double d2 = dx * dx;
-    d2 = d2 + dy * dy;
-    d2 = d2 + dz * dz;
-    const bool approx = act && nd.s2 < wa.theta_sq * d2 && d2 > 0.0;
+    // R = centre - x_i (fixed units)
+    float Rx = dem::diff32(nd.qx, me.x), Ry = dem::diff32(nd.qy, me.y),
+          Rz = dem::diff32(nd.qz, me.z);
+    int take = (wa.exact_mac || (nd.flags & kExactMac))
+                   ? -1
+                   : fast_open_test(Rx, Ry, Rz, wa.th2f, wa.band_s, nd.s2f, nd.sf);
+    if (act && take < 0) {
+      // the reference's float64 test, x_i - centre (:1139-1154, :1479)
+      const NodeC c = wa.nc[node];
+      double dx = mic64(px - c.cx, wa.L, wa.periodic);
+      double dy = mic64(py - c.cy, wa.L, wa.periodic);
+      double dz = mic64(pz - c.cz, wa.L, wa.periodic);
+      double d2 = dx * dx;
+      d2 = d2 + dy * dy;
+      d2 = d2 + dz * dz;
+      take = c.s2 < wa.theta_sq * d2 && d2 > 0.0;
+      Rx = (float)(-dx * sc.inv_U);
+      Ry = (float)(-dy * sc.inv_U);
+      Rz = (float)(-dz * sc.inv_U);
+    }
+    const bool approx = act && take > 0;
     const NodeF nf = wa.nf[node];  // wave-uniform: loaded for every lane
     if (approx) {
-      // R = centre - x_i in fixed units (sign irrelevant for the monopole)
-      const float Rx = (float)(-dx * sc.inv_U), Ry = (float)(-dy * sc.inv_U),
-                  Rz = (float)(-dz * sc.inv_U);
       float vx = 0.f, vy = 0.f, vz = 0.f;
       node_velocity(Rx, Ry, Rz, nf, vx, vy, vz);
       ax += (double)vx;
@@ -1065,7 +1147,8 @@ int propagate_forces(const dem_tree_info* info, const TreeLayout& l, const doubl
     }
   }
   hipLaunchKernelGGL(k_node_force, dim3(nb256(nn)), dim3(256), 0, st, nn, (const double*)l.fsum,
-                     (const double*)l.fmom, (const NodeW*)l.nw, (const Scales*)l.sc, l.nf, l.flag);
+                     (const double*)l.fmom, (const NodeW*)l.nw, (const NodeC*)l.nc,
+                     (const Scales*)l.sc, l.nf, l.flag);
   DEM_CHECK_LAUNCH("k_node_force");
   hipLaunchKernelGGL(k_apply_flags, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.flag,
                      l.nw);
@@ -1077,8 +1160,9 @@ int propagate_forces(const dem_tree_info* info, const TreeLayout& l, const doubl
 }
 
 WalkArgs walk_args(const dem_tree_info* info, const TreeLayout& l, double theta_sq) {
-  WalkArgs w;
+  WalkArgs w{};
   w.nw = l.nw;
+  w.nc = l.nc;
   w.nf = l.nf;
   w.srec = l.srec;
   w.spos = l.spos;
@@ -1089,6 +1173,9 @@ WalkArgs walk_args(const dem_tree_info* info, const TreeLayout& l, double theta_
   w.L = info->L;
   w.periodic = info->periodic;
   w.theta_sq = theta_sq;
+  w.th2f = (float)theta_sq;
+  w.band_s = (float)(8.0 * sqrt(theta_sq) * (1.0 + 0x1p-20));
+  w.exact_mac = g_exact_mac;
   return w;
 }
 
@@ -1120,6 +1207,12 @@ __global__ void k_corr_scale(const Scales* __restrict__ scp, double L_ratio, dou
 }  // namespace
 
 extern "C" {
+
+int dem_octree_set_exact_mac(int on) {
+  const int prev = g_exact_mac;
+  g_exact_mac = on ? 1 : 0;
+  return prev;
+}
 
 size_t dem_octree_workspace_bytes(int n_capacity, int max_nodes) {
   return carve(nullptr, n_capacity < 0 ? 0 : n_capacity, max_nodes < 1 ? 1 : max_nodes).bytes;
@@ -1253,7 +1346,7 @@ int dem_octree_build(const double* pos, const double* radii, int n, double L, in
   hipLaunchKernelGGL(k_node_walk, dim3(nb256(nn)), dim3(256), 0, st, nn, (const double*)l.bmin,
                      (const double*)l.bmax, (const double*)l.r3sum, (const double*)l.r3c,
                      (const int*)l.first_child, (const int*)l.next, (const int*)l.count,
-                     (const int*)l.mstart, (const Scales*)l.sc, l.nw);
+                     (const int*)l.mstart, (const Scales*)l.sc, l.nw, l.nc);
   DEM_CHECK_LAUNCH("k_node_walk");
   int lsc = 0;
   DEM_CHECK_HIP(hipMemcpyAsync(&lsc, l.counter + 1, sizeof(int), hipMemcpyDeviceToHost, st));
