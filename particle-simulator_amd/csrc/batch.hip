@@ -506,39 +506,50 @@ __global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
 }
 
 // Exclusive scan of in[0, len) into out[0, len) (and out[len] = total when
-// `total`) by one kScanWG-thread workgroup, in contiguous per-thread chunks.
+// `total`) by one kScanWG-thread workgroup, in tiles of 4 kScanWG values:
+// thread t takes values 4t .. 4t + 3 of a tile (consecutive threads read
+// consecutive 16-byte runs), a wave scan and a scan of the wave totals give
+// its offset, and the tile total carries into the next tile.
 __device__ void block_scan(const int* __restrict__ in, int* __restrict__ out, int len,
                            bool total) {
-  __shared__ int s_w[kScanWG / 64];
-  const int chunk = (len + kScanWG - 1) / kScanWG;
-  const int b0 = min(len, (int)threadIdx.x * chunk), b1 = min(len, b0 + chunk);
-  int local = 0;
-  for (int b = b0; b < b1; ++b) local += in[b];
+  constexpr int kW = kScanWG / 64;
+  __shared__ int s_w[kW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int v = local;
+  int carry = 0;
+  for (int base = 0; base < len; base += 4 * kScanWG) {
+    const int b = base + 4 * (int)threadIdx.x;
+    int x[4];
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int u = __shfl_up(v, d, 64);
-    if (lane >= d) v += u;
-  }
-  if (lane == 63) s_w[wave] = v;
-  __syncthreads();
-  if (wave == 0) {
-    int w = lane < kScanWG / 64 ? s_w[lane] : 0;
+    for (int k = 0; k < 4; ++k) x[k] = b + k < len ? in[b + k] : 0;
+    const int local = x[0] + x[1] + x[2] + x[3];
+    int v = local;
 #pragma unroll
-    for (int d = 1; d < kScanWG / 64; d <<= 1) {
-      const int u = __shfl_up(w, d, 64);
-      if (lane >= d) w += u;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(v, d, 64);
+      if (lane >= d) v += u;
     }
-    if (lane < kScanWG / 64) s_w[lane] = w;
+    if (lane == 63) s_w[wave] = v;
+    __syncthreads();
+    if (wave == 0) {
+      int w = lane < kW ? s_w[lane] : 0;
+#pragma unroll
+      for (int d = 1; d < kW; d <<= 1) {
+        const int u = __shfl_up(w, d, 64);
+        if (lane >= d) w += u;
+      }
+      if (lane < kW) s_w[lane] = w;
+    }
+    __syncthreads();
+    int run = carry + v - local + (wave > 0 ? s_w[wave - 1] : 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (b + k < len) out[b + k] = run;
+      run += x[k];
+    }
+    carry += s_w[kW - 1];
+    __syncthreads();  // s_w is rewritten by the next tile
   }
-  __syncthreads();
-  int run = v - local + (wave > 0 ? s_w[wave - 1] : 0);
-  for (int b = b0; b < b1; ++b) {
-    out[b] = run;
-    run += in[b];
-  }
-  if (total && threadIdx.x == kScanWG - 1) out[len] = run;
+  if (total && threadIdx.x == 0) out[len] = carry;
 }
 
 __global__ __launch_bounds__(kScanWG) void k_b_bucket_scan(const BEns* __restrict__ ens,
