@@ -152,7 +152,8 @@ class BatchedEnsembles:
         self.phase_ms: dict | None = None
         # merge rounds on a second stream beside the advance of the
         # contact-free systems (default priority, like every stream here)
-        self.overlap = True
+        self.overlap = True      # False: check, advance and merge rounds one after another
+        self.merge_first = False  # merge rounds first, then one advance of every system
         self._side = t.cuda.Stream()
 
     def _field(self, name, dtype):
@@ -189,7 +190,12 @@ class BatchedEnsembles:
     def step(self, merge: bool = True) -> int:
         """One global step; returns the number of system merges in it.
 
-        check -> (advance the systems without contacts | merge the others,
+        With ``self.merge_first``: check -> merge the systems in contact,
+        check them again, ... -> one advance of every system.  The systems
+        are independent, so each goes through the same check / merge /
+        advance sequence as in the other schedules and ends in the same
+        state; the advance is one launch over all systems.  Otherwise (the
+        default): check -> (advance the systems without contacts | merge the others,
         check them again, ... on a second stream) -> advance the rest.  The
         merge rounds touch only the systems in contact, whose descriptors,
         cells and records ``dem_batch_merge_async`` rewrites, and start once
@@ -200,6 +206,8 @@ class BatchedEnsembles:
         are accumulated under "check", "advance", "merge"."""
         if self.phase_ms is not None or not self.overlap or not merge:
             return self._step_serial(merge)
+        if self.merge_first:
+            return self._step_merge_first()
         t = _device.torch()
         main = t.cuda.current_stream()
         self.check(new_step=True)
@@ -219,6 +227,22 @@ class BatchedEnsembles:
         main.wait_stream(self._side)
         if merged:
             self.advance()
+        self.global_steps += 1
+        return merged
+
+    def _step_merge_first(self) -> int:
+        self.check(new_step=True)
+        self._stage()
+        merged = 0
+        while True:
+            self._staged.synchronize()
+            k = self._merge_staged(defer_tables=True)
+            if k == 0:
+                break
+            merged += k
+            self.check(new_step=False)
+            self._stage()
+        self.advance()  # the work tables are rebuilt here when merges changed them
         self.global_steps += 1
         return merged
 

@@ -106,14 +106,17 @@ def test_pair_once_batched_equals_sequential_bit_for_bit(gpu, setup, jlen):
     b.close()
 
 
-@pytest.mark.parametrize("pair_once", [False, True])
-def test_overlapped_merge_rounds_equal_serial_rounds(gpu, setup, pair_once):
+@pytest.mark.parametrize("pair_once,merge_first", [(False, False), (True, False), (False, True),
+                                                  (True, True)])
+def test_overlapped_merge_rounds_equal_serial_rounds(gpu, setup, pair_once, merge_first):
     """Merge rounds on a second stream beside the advance of the contact-free
     systems (dem_batch_wait_go, dem_batch_merge_async, tables rebuilt before
-    the second advance) end in exactly the states of the serial rounds."""
+    the second advance), or all merge rounds first and then one advance of
+    every system, end in exactly the states of the serial rounds."""
     systems, dts = setup
     tune = Tuning(256, 1024, pair_once=pair_once)
     a = BatchedEnsembles(systems, dts, STEPS, tune, capture=True)
+    a.merge_first = merge_first
     c = BatchedEnsembles(systems, dts, STEPS, tune, capture=True)
     c.overlap = False
     a.run_until_done(200)
