@@ -779,17 +779,21 @@ constexpr int kActive = -2;  // lane state: in the walk; otherwise the resume no
 
 __device__ __forceinline__ int wave_node(int node) { return __builtin_amdgcn_readfirstlane(node); }
 
+// Walk workgroup: one wave, so the dispatcher places waves one at a time and
+// the launch's tail is balanced per wave.
+constexpr int kWalkWG = 64;
+
 // Force walk (:783-906).  Writes forces (original order) and the stored copy.
 // TPL targets per lane: a wave walks for 64 TPL consecutive (Morton-ordered)
 // targets, lane l holding targets base + 64 h + l; node loads, leaf-member
 // loads and the walk's control are shared by all of them.
 template <int TPL>
-__global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* __restrict__ scp,
+__global__ __launch_bounds__(kWalkWG) void k_walk_forces(WalkArgs wa, const Scales* __restrict__ scp,
                                                      double m_const, double* __restrict__ F,
                                                      double* __restrict__ pforces) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
-  const int base = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64 * TPL;
+  const int base = (blockIdx.x * (kWalkWG / 64) + (threadIdx.x >> 6)) * 64 * TPL;
   const Scales sc = *scp;
   int t[TPL], resume[TPL];
   bool live[TPL];
@@ -904,13 +908,13 @@ __global__ __launch_bounds__(256) void k_walk_forces(WalkArgs wa, const Scales* 
 // MODE 0: Stokeslet (+ COMSOL self term); MODE 1: total velocity (migration
 // F_i / (stokes r_i) added, no self term — as the reference).
 template <int MODE, bool CORR>
-__global__ __launch_bounds__(256) void k_walk_stokes(WalkArgs wa, const Scales* __restrict__ scp,
+__global__ __launch_bounds__(kWalkWG) void k_walk_stokes(WalkArgs wa, const Scales* __restrict__ scp,
                                                      CorrArgs ca, double eta_const, double common,
                                                      double self_coeff, double stokes_factor,
                                                      const double* __restrict__ Fself,
                                                      double* __restrict__ V) {
 #pragma clang fp contract(off)
-  const int t0 = blockIdx.x * 256 + threadIdx.x;
+  const int t0 = blockIdx.x * kWalkWG + threadIdx.x;
   const bool live = t0 < wa.n;
   const int t = live ? t0 : wa.n - 1;
   const Scales sc = *scp;
@@ -1117,7 +1121,6 @@ __global__ __launch_bounds__(256) void k_walk_collisions(
 
 inline int nb256(long long n) { return (int)((n + 255) / 256); }
 // workgroups of 256 lanes walking `tpl` targets each
-inline int nb256w(long long n, int tpl) { return (int)((n + 256LL * tpl - 1) / (256LL * tpl)); }
 constexpr int kWalkTpl = 1;  // 2 measured slower: the wider union of the walks costs more than it hides
 
 int check_info(const dem_tree_info* info, const void* ws, size_t ws_bytes, TreeLayout& l) {
@@ -1366,8 +1369,10 @@ int dem_octree_forces(dem_tree_info* info, double theta_sq, double m_const, doub
   hipStream_t st = dem::as_stream(stream);
   WalkArgs wa = walk_args(info, l, theta_sq);
   DEM_PROF_LAUNCH("k_walk_forces", st,
-                  hipLaunchKernelGGL(k_walk_forces<kWalkTpl>, dim3(nb256w(info->n, kWalkTpl)),
-                                     dim3(256), 0, st, wa,
+                  hipLaunchKernelGGL(k_walk_forces<kWalkTpl>,
+                                     dim3((unsigned)((info->n + (long long)kWalkWG * kWalkTpl - 1) /
+                                                     ((long long)kWalkWG * kWalkTpl))),
+                                     dim3(kWalkWG), 0, st, wa,
                                      (const Scales*)l.sc, m_const, forces, l.pforces));
   DEM_CHECK_LAUNCH("k_walk_forces");
   info->forces_valid = 1;
@@ -1402,7 +1407,7 @@ static int stokes_common(int mode, const dem_tree_info* info, double theta_sq, c
     ca.g.ga = ga;
   }
   WalkArgs wa = walk_args(info, l, theta_sq);
-  const dim3 g(nb256(info->n)), b(256);
+  const dim3 g((unsigned)((info->n + kWalkWG - 1) / kWalkWG)), b(kWalkWG);
   const char* name = mode == 0 ? "k_walk_stokeslet" : "k_walk_total";
   int slot = dem::prof_begin(name, st);
   if (mode == 0 && ca.enabled)
