@@ -25,6 +25,7 @@ import os
 
 import numpy as np
 from scipy.interpolate import LinearNDInterpolator, NearestNDInterpolator, RegularGridInterpolator
+from scipy.spatial import Delaunay
 
 
 class COMSOLLatticeCorrection:
@@ -97,13 +98,19 @@ class COMSOLLatticeCorrection:
         axis = np.linspace(-h, h, n)
         gx, gy, gz = np.meshgrid(axis, axis, axis, indexing="ij")
         q = np.column_stack([gx.ravel(), gy.ravel(), gz.ravel()])
+        # One triangulation and one simplex search serve all three components
+        # (the barycentric weights are per point, so each column gets exactly
+        # the values a single-column interpolator would give).
+        tri = Delaunay(pts)
+        vals = np.column_stack([u, v, w])
+        lin = LinearNDInterpolator(tri, vals)(q)
         grids = []
-        for vals in (u, v, w):
-            g = LinearNDInterpolator(pts, vals)(q).reshape(n, n, n)
+        for k in range(3):
+            g = lin[:, k].reshape(n, n, n).copy()
             bad = np.isnan(g)
             if np.any(bad):
                 idx = np.argwhere(bad)
-                g[bad] = NearestNDInterpolator(pts, vals)(
+                g[bad] = NearestNDInterpolator(pts, vals[:, k])(
                     np.column_stack([axis[idx[:, 0]], axis[idx[:, 1]], axis[idx[:, 2]]])
                 )
             grids.append(g)
