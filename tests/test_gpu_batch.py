@@ -165,6 +165,38 @@ def test_first_step_against_fp64_oracle(gpu, setup, pair_once):
     b.close()
 
 
+@pytest.mark.parametrize("jlen", [1024, 8192])
+def test_pair_once_first_step_at_ensemble_sizes_against_fp64_oracle(gpu, jlen):
+    """Systems of the 256-ensemble workload's sizes (10k-20k droplets, 40-79
+    blocks of 256, own seed, E and volume fraction) through the pair-once
+    batched step: forces, velocities and displacements of the first step
+    against the FP64 oracle under the direct kernels' accuracy contract."""
+    specs = [(10240, 0.012, 1.3e5), (15001, 0.035, 4.1e5), (19999, 0.049, 2.6e5)]
+    sub = [make_system(n, seed=300 + e, phi=phi, E=E) for e, (n, phi, E) in enumerate(specs)]
+    sdt = _dts(sub, frac=0.1)
+    b = BatchedEnsembles(sub, sdt, [1] * len(sub), Tuning(256, jlen, pair_once=True),
+                         capture=True)
+    b.run(1, merge=False)
+    assert np.array_equal(b.steps_done.cpu().numpy(), np.ones(len(sub), dtype=np.int32))
+    for e, sy in enumerate(sub):
+        F_ref = oracle.direct_forces(sy.positions, sy.radii, sy.m_const, sy.L, True)
+        V_ref = oracle.direct_convection(sy.positions, F_ref, sy.eta_const, sy.L, True)
+        ef = summary(b.forces(e), F_ref, FLOOR)
+        ev = summary(b.velocities(e), V_ref, FLOOR)
+        print(f"\n[batched pair-once n={sy.n} jlen={jlen}] force {ef}\n  velocity {ev}")
+        assert ef["max"] <= TOL_F, ef
+        assert ev["norm"] <= TOL_V_NORM and ev["p999"] <= TOL_V_P999 and ev["max"] <= TOL_V_MAX, ev
+        x_ref = oracle.euler_step(sy.positions, F_ref, V_ref, sy.radii, sy.stokes_factor, sdt[e],
+                                  sy.L, True)
+        d = b.positions(e) - sy.positions
+        d_ref = x_ref - sy.positions
+        d -= sy.L * np.round(d / sy.L)
+        d_ref -= sy.L * np.round(d_ref / sy.L)
+        ed = summary(d, d_ref, FLOOR)
+        assert ed["norm"] <= TOL_V_NORM and ed["max"] <= TOL_V_MAX, ed
+    b.close()
+
+
 def test_graph_replay_equals_eager_launches(gpu, setup):
     systems, dts = setup
     a = BatchedEnsembles(systems[:3], dts[:3], [6, 6, 6], Tuning(128, 1024), capture=True)

@@ -9,6 +9,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import oracle
+
 from dem.collision_detector import SpatialHashCollisionDetector
 from dem.force_calculator import DirectDropletForceCalculator
 from dem.octree import TreeDropletForceCalculator
@@ -110,3 +112,59 @@ def test_merge_compact_matches_merge_collided(gpu):
                      _device.ptr(out_r), _device.stream())
         assert np.array_equal(out_p.cpu().numpy(), ref_p)
         assert np.array_equal(out_r.cpu().numpy(), ref_r)
+        # the oracle's merge, visiting the pairs in the same (np.argsort) order
+        op, orr, _ = oracle.merge(pos, r, pairs[np.argsort(pairs[:, 0])])
+        assert np.array_equal(out_p.cpu().numpy(), op)
+        assert np.array_equal(out_r.cpu().numpy(), orr)
+
+
+def test_config0_trajectory_against_fp64_oracle(gpu):
+    """The plain direct workload on the CPU-path size: N = 2000, seed 0,
+    phi = 0.02, E = 3e5 V/m, no coalescence, dt = 0.1 r_min / max|v(0)| from
+    the FP64 oracle, 100 steps of the device-resident loop against the FP64
+    oracle's rollout (C restatement of the force, Oseen and Euler kernels) on
+    the same initial state.
+
+    Tolerance, at every saved step: the displacement from the initial state,
+    d(t) = x(t) - x(0) (minimum image), has field error
+    |d - d_ref| / |d_ref| <= 1e-5 and a per-droplet error (floor 1e-3 rms)
+    of at most 1e-3; positions agree to within 1e-3 r_min."""
+    from ._errors import summary
+
+    s = make_system(2000, seed=0, phi=0.02)
+    F0 = oracle.direct_forces(s.positions, s.radii, s.m_const, s.L, True)
+    V0 = oracle.direct_convection(s.positions, F0, s.eta_const, s.L, True)
+    v0 = F0 / (s.stokes_factor * s.radii[:, None]) + V0
+    dt = 0.1 * s.radii.min() / np.max(np.linalg.norm(v0, axis=1))
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                      eta_water=s.eta_water, E=s.E, L=s.L,
+                                      boundary_mode="periodic")
+    sol = DropletSolution(DropletState(s.positions.copy(), s.radii.copy(), 0.0), length=128)
+    solver = DeviceEulerDropletSolver(force_calculator=fc, solution=sol,
+                                      post_processor=DropletPostProcessor(sol, box_size=s.L),
+                                      collision_detector=None, verbose=False)
+    solver.solve(dt, 100 * dt)
+    traj = solver.solution.get_trajectories()
+    assert solver.solution.get_first_solution() is solver.solution  # no merges
+    steps = traj.shape[0] - 1
+    assert steps >= 100
+    x = s.positions.copy()
+    worst = {"norm": 0.0, "max": 0.0, "abs": 0.0}
+    for k in range(1, steps + 1):
+        F = oracle.direct_forces(x, s.radii, s.m_const, s.L, True)
+        V = oracle.direct_convection(x, F, s.eta_const, s.L, True)
+        x = oracle.euler_step(x, F, V, s.radii, s.stokes_factor, dt, s.L, True)
+        d = traj[k] - s.positions
+        d_ref = x - s.positions
+        d -= s.L * np.round(d / s.L)
+        d_ref -= s.L * np.round(d_ref / s.L)
+        e = summary(d, d_ref)
+        gap = traj[k] - x
+        gap -= s.L * np.round(gap / s.L)
+        worst["norm"] = max(worst["norm"], e["norm"])
+        worst["max"] = max(worst["max"], e["max"])
+        worst["abs"] = max(worst["abs"], float(np.abs(gap).max() / s.radii.min()))
+    print(f"\n[config 0 trajectory, {steps} steps] worst displacement field error "
+          f"{worst['norm']:.3e}, per-droplet {worst['max']:.3e}, "
+          f"position gap {worst['abs']:.3e} r_min")
+    assert worst["norm"] <= 1e-5 and worst["max"] <= 1e-3 and worst["abs"] <= 1e-3

@@ -287,3 +287,38 @@ def test_capacity_regrowth_and_empty(gpu):
     tree = FlatOctree(num_particles=10)
     tree.build(np.zeros((0, 3)), np.zeros(0), L=1.0)
     assert tree.node_count[None] == 1 and tree.compute_forces(M).shape == (0, 3)
+
+
+# --- far-field order (the oracle's counterpart: tests/test_oracle_farfield.py)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_far_field_error_is_second_order_on_gpu(gpu, periodic):
+    """Four sources of unequal radii seen from afar as one accepted node:
+    halving the cluster cuts the GPU walks' error against the FP64 direct sum
+    by 4 (monopole at the R^3 centroid, flat_tree.py:871-886; Stokeslet
+    monopole + dipole with its trace term, :1154-1168, :1215-1265), and at each
+    size the GPU walk equals the oracle walk to FP32 accuracy."""
+    from .test_oracle_farfield import cluster_case
+
+    errs = []
+    for h in (0.02, 0.01, 0.005):
+        pos, r, F = cluster_case(h, periodic)
+        tree = FlatOctree(theta=0.5, mpl=1, num_particles=len(r))
+        tree.build(pos, r, L=1.0, periodic=periodic)
+        Ft = tree.compute_forces(1.0)
+        Vs = tree.compute_stokeslet(F, 1.0)
+        Vt = tree.compute_total_velocity(F, 1.0, 1.0)
+        Fd = oracle.direct_forces(pos, r, 1.0, 1.0, periodic)
+        Vd = oracle.direct_convection(pos, F, 1.0, 1.0, periodic)
+        ref = oracle.OracleTree(pos, r, 1.0, periodic, mpl=1)
+        np.testing.assert_allclose(Ft[0], ref.forces(0.5, 1.0)[0], rtol=1e-5)
+        np.testing.assert_allclose(Vs[0], ref.stokeslet(0.5, F, 1.0)[0], rtol=1e-5)
+        np.testing.assert_allclose(Vt[0], ref.total_velocity(0.5, F, 1.0, 1.0)[0], rtol=1e-5)
+        rel = lambda a, b: float(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]))  # noqa: E731
+        errs.append((rel(Ft, Fd), rel(Vs, Vd), rel(Vt, Vd)))
+    errs = np.array(errs)
+    ratios = errs[:-1] / errs[1:]
+    print(f"\n[far field per={periodic}] errors {errs.tolist()} ratios {ratios.tolist()}")
+    assert np.all(errs[0] > 1e-5)
+    assert np.all((ratios > 3.6) & (ratios < 4.5)), ratios

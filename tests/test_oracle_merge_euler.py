@@ -52,3 +52,41 @@ def test_euler_periodic_wrap():
     x = oracle.euler_step_total(pos, v, 0.2, 1.0, True)
     np.testing.assert_allclose(x, [[0.001, 0.9985, 0.5]], atol=1e-14)
     assert np.all((x >= 0) & (x < 1.0))
+
+
+def _pair_lists(rng, n, k):
+    """Random pair lists with chains (a droplet in several pairs) and repeats
+    of a first index, canonicalised as detect() returns them."""
+    a = rng.integers(0, n, k)
+    b = (a + 1 + rng.integers(0, n - 1, k)) % n
+    pairs = np.unique(np.sort(np.stack([a, b], 1), axis=1), axis=0).astype(np.int32)
+    return pairs[pairs[:, 0] != pairs[:, 1]]
+
+
+def test_product_merge_equals_oracle_merge_bit_for_bit():
+    """merge_collided (the product's host merge, droplet_solution.py:208-271)
+    equals the oracle's merge bit for bit: survivors, merged radii and
+    centres, and their order.  The reference visits pairs in np.argsort order
+    of the first index, which numpy does not keep stable for ties; the oracle
+    visits them in stable order, so it is handed the pairs already in
+    np.argsort order (and, for lists without repeated first indices, the raw
+    list)."""
+    from dem.solution.droplet_solution import merge_collided
+
+    rng = np.random.default_rng(21)
+    chained = 0
+    for n, k in ((2, 1), (7, 5), (40, 30), (300, 250), (3000, 900)):
+        for _ in range(5):
+            pos = rng.random((n, 3)) * 1e-3
+            r = rng.uniform(2e-6, 2e-5, n)
+            pairs = _pair_lists(rng, n, k)
+            p, rr, coll = merge_collided(pos, r, pairs)
+            op, orr, res = oracle.merge(pos, r, pairs[np.argsort(pairs[:, 0])])
+            assert np.array_equal(p, op) and np.array_equal(rr, orr)
+            assert res == list(range(len(orr) - len(res), len(orr)))
+            chained += len(pairs) - len(res)
+            first = pairs[:, 0]
+            if len(np.unique(first)) == len(first):
+                op2, orr2, _ = oracle.merge(pos, r, pairs[::-1])
+                assert np.array_equal(p, op2) and np.array_equal(rr, orr2)
+    assert chained > 100, "the lists should contain skipped (chained) pairs"
