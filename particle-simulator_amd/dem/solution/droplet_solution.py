@@ -1,18 +1,26 @@
-"""Trajectory chain and the volume-conserving merge (host, float64 numpy).
+"""Trajectory segments and the volume-conserving merge (host, float64 numpy).
 
-Drop-in for the reference's ``DropletSolution``
-(dem/solution/droplet_solution.py:10-499).  The merge
-(``generate_next_solution``, :208-287) is the reference's algorithm kept
-bit-exact on purpose — it runs on the host in float64 with the same numpy
-calls, in the same order:
-* pairs ordered by their first index with numpy's default ``argsort``;
-* greedy: a pair is skipped when either droplet already merged;
-* r_new = (r1^3 + r2^3) ** (1/3) (float power, not cbrt);
-* R^3-weighted centre from raw coordinates (no minimum image, no wrap);
-* ``np.delete`` of the merged indices keeps survivors in order and the merged
-  droplets are appended last in processing order.
-Storage (growing buffers, chain links, npz chain I/O as a list of dicts,
-time interpolation) follows the same layout so files interoperate.
+A run is a doubly linked chain of segments.  A segment records one fixed set
+of droplets (radii and count never change inside it) at successive times;
+a merge closes the segment and opens the next one with the merged set.  The
+public surface — class and method names, attributes read by the solver and
+post-processors (``times`` (rows, 1), ``trajectories`` (rows, N, 3),
+``radii``, ``num_particles``, ``length``, ``current_step``,
+``initial_droplet_state``, ``is_collision``, ``collided_droplets``,
+``resulting_droplet``, ``_prev`` / ``_next``) and the npz files — is that of
+the reference's ``DropletSolution`` (dem/solution/droplet_solution.py:10-499),
+so its callers and files work unchanged.
+
+Recording: rows [0, current_step] of ``times`` / ``trajectories`` are
+recorded; the buffers double when full (``length`` doubles with them, and is
+what a saved chain stores), and ``compact`` trims them to the recorded rows.
+
+The merge (``generate_next_solution``, reference :208-287) is kept bit-exact
+with the reference: pairs visited in numpy's default ``argsort`` order of the
+first droplet, a pair skipped when either droplet already merged, the merged
+radius (r1^3 + r2^3) ** (1/3) as a float power, the centre the r^3-weighted
+mean of the raw (unwrapped) coordinates, survivors kept in order and merged
+droplets appended in visiting order.
 """
 
 from __future__ import annotations
@@ -25,39 +33,29 @@ import numpy as np
 from ..particle_state import DropletState
 from .solution_base import Solution
 
+# ---------------------------------------------------------------- the merge
+
 
 def merge_plan(collided, radius_of, position_of):
-    """The greedy merge decisions of the reference (droplet_solution.py:
-    208-262), reading droplet values through ``radius_of(i)`` (float64
-    scalar) and ``position_of(i)`` ((3,) float64).
+    """Greedy merge decisions over the contact pairs ``collided`` (k, 2),
+    reading droplet values through ``radius_of(i)`` (float64 scalar) and
+    ``position_of(i)`` ((3,) float64).
 
-    Pairs are visited in ``np.argsort(collided[:, 0])`` order; a pair is
-    skipped when either droplet already merged; r = (r1^3 + r2^3) ** (1/3)
-    (float power); the centre is the r^3-weighted mean of the raw coordinates
-    (no minimum image).  Returns (indices to remove, descending; merged radii;
-    merged positions; collided sorted by first index).
+    Returns (indices to remove, descending; merged radii; merged centres;
+    the pairs in visiting order).  Reference: droplet_solution.py:214-262.
     """
-    sorted_indices = np.argsort(collided[:, 0])
-    collided = collided[sorted_indices]
-    new_droplets_radii = []
-    new_droplets_positions = []
-    collided_indices = set()
-    for idx1, idx2 in collided:
-        if idx1 in collided_indices or idx2 in collided_indices:
+    visit = collided[np.argsort(collided[:, 0])]  # numpy's default kind, as the reference
+    merged_away = set()
+    radii_out, centres_out = [], []
+    for a, b in visit:
+        if a in merged_away or b in merged_away:
             continue
-        radius1, radius2 = radius_of(idx1), radius_of(idx2)
-        position1, position2 = position_of(idx1), position_of(idx2)
-        radius1_cube = radius1**3
-        radius2_cube = radius2**3
-        new_radius = (radius1_cube + radius2_cube) ** (1 / 3)
-        new_position = (position1 * radius1_cube + position2 * radius2_cube) / (
-            radius1_cube + radius2_cube
-        )
-        new_droplets_radii.append(new_radius)
-        new_droplets_positions.append(new_position)
-        collided_indices.add(idx1)
-        collided_indices.add(idx2)
-    return sorted(collided_indices, reverse=True), new_droplets_radii, new_droplets_positions, collided
+        va = radius_of(a) ** 3
+        vb = radius_of(b) ** 3
+        radii_out.append((va + vb) ** (1 / 3))
+        centres_out.append((position_of(a) * va + position_of(b) * vb) / (va + vb))
+        merged_away.update((a, b))
+    return sorted(merged_away, reverse=True), radii_out, centres_out, visit
 
 
 def merge_plan_rows(pairs, values):
@@ -66,12 +64,12 @@ def merge_plan_rows(pairs, values):
     first droplet then of the second.
 
     Same decisions and the same IEEE operations in the same order as
-    ``merge_plan`` — argsort by first index, greedy skip, r^3 = r ** 3,
-    r = (r1^3 + r2^3) ** (1/3), centre (x1 r1^3 + x2 r2^3) / (r1^3 + r2^3) per
-    axis — on Python floats (C ``pow`` and unfused multiply/add/divide, as
-    numpy's float64 scalars and element-wise array operations do), which is
-    what makes it cheap for thousands of systems.  Returns (indices to remove,
-    ascending; merged (x, y, z, r) tuples in processing order).
+    ``merge_plan`` (r^3 = r ** 3, r = (r1^3 + r2^3) ** (1/3), centre
+    (x1 r1^3 + x2 r2^3) / (r1^3 + r2^3) per axis), on Python floats — C
+    ``pow`` and unfused multiply/add/divide, as numpy's float64 scalars and
+    element-wise operations do — which is what makes it cheap for thousands
+    of systems.  Returns (indices to remove, ascending; merged (x, y, z, r)
+    tuples in visiting order).
     """
     order = np.argsort(pairs[:, 0])
     pl = pairs[order].tolist()
@@ -100,19 +98,32 @@ def merge_plan_rows(pairs, values):
 def merge_collided(positions, radii, collided):
     """Volume-conserving greedy merge (reference droplet_solution.py:208-271).
 
-    Returns (new_positions, new_radii, collided sorted by first index):
-    survivors keep their order (``np.delete``) and merged droplets are
-    appended in processing order (see ``merge_plan``).
+    Returns (new_positions, new_radii, collided in visiting order): survivors
+    keep their order and merged droplets follow in visiting order.
     """
-    new_radii = np.copy(radii)
-    new_positions = np.copy(positions)
-    indices_to_remove, new_droplets_radii, new_droplets_positions, collided = merge_plan(
-        collided, lambda i: new_radii[i], lambda i: new_positions[i])
-    new_radii = np.delete(new_radii, indices_to_remove)
-    new_positions = np.delete(new_positions, indices_to_remove, axis=0)
-    new_radii = np.append(new_radii, new_droplets_radii)
-    new_positions = np.vstack([new_positions, new_droplets_positions])
-    return new_positions, new_radii, collided
+    gone, add_r, add_x, visit = merge_plan(collided, lambda i: radii[i], lambda i: positions[i])
+    keep = np.ones(len(radii), dtype=bool)
+    keep[gone] = False
+    new_radii = np.concatenate([np.asarray(radii, dtype=np.float64)[keep],
+                                np.asarray(add_r, dtype=np.float64)])
+    survivors = np.asarray(positions, dtype=np.float64)[keep]
+    new_positions = (np.concatenate([survivors, np.asarray(add_x, dtype=np.float64).reshape(-1, 3)])
+                     if add_x else survivors)
+    return new_positions, new_radii, visit
+
+
+# ---------------------------------------------------------------- storage helpers
+
+
+def _enlarged(a: np.ndarray, rows: int) -> np.ndarray:
+    """``a`` with its leading axis enlarged to ``rows``; the old rows copied,
+    the new ones left uninitialised (they are written before being read)."""
+    out = np.empty((rows,) + a.shape[1:], dtype=a.dtype)
+    out[: a.shape[0]] = a
+    return out
+
+
+_PRECISIONS = {"float32": np.float32, "float16": np.float16}
 
 
 class DropletSolution(Solution):
@@ -125,236 +136,231 @@ class DropletSolution(Solution):
         previous=None,
         filename=None,
     ):
-        self.initial_droplet_state = initial_droplet_state
-        self.num_particles = len(initial_droplet_state.radii)
-        self.radii = initial_droplet_state.radii
-        self.length = length
+        state = initial_droplet_state
+        self.initial_droplet_state = state
+        self.radii = state.radii
+        self.num_particles = len(state.radii)
         self.real_time_visualization = real_time_visualization
         self.update_interval = update_interval
-        self.times = np.zeros((self.length, 1)) + initial_droplet_state.time
-        self.trajectories = np.zeros((self.length, self.num_particles, 3))
-        self.last_update_time = time.time()
+        self.filename = filename
         self._prev = previous
         self._next = None
+        # collision record of the segment (set when it ends in a merge)
         self.is_collision = False
         self.collided_droplets = []
         self.resulting_droplet = []
+        self.last_update_time = time.time()
+        # unrecorded rows hold the start time / zeros until written
+        self.length = length
+        self.times = np.zeros((length, 1)) + state.time
+        self.trajectories = np.zeros((length, self.num_particles, 3))
         self.current_step = -1
-        self.save_step(initial_droplet_state.time, initial_droplet_state.positions)
-        self.filename = filename
+        self.save_step(state.time, state.positions)
+
+    # -- recording
 
     def save_step(self, t, positions):
-        self.current_step += 1
-        if self.current_step >= self.length:
-            new_length = self.length * 2
-            new_times = np.empty((new_length, 1), dtype=self.times.dtype)
-            new_times[: self.length] = self.times
-            self.times = new_times
-            new_traj = np.empty((new_length, self.num_particles, 3), dtype=self.trajectories.dtype)
-            new_traj[: self.length] = self.trajectories
-            self.trajectories = new_traj
-            self.length = new_length
-        self.times[self.current_step] = t
-        self.trajectories[self.current_step] = positions
+        row = self.current_step + 1
+        if row >= self.length:
+            rows = 2 * self.length
+            self.times = _enlarged(self.times, rows)
+            self.trajectories = _enlarged(self.trajectories, rows)
+            self.length = rows
+        self.times[row] = t
+        self.trajectories[row] = positions
+        self.current_step = row
+
+    def _recorded(self) -> slice:
+        return slice(0, self.current_step + 1)
 
     def get_times(self):
-        return self.times[: self.current_step + 1]
+        return self.times[self._recorded()]
 
     def get_trajectories(self):
-        return self.trajectories[: self.current_step + 1]
+        return self.trajectories[self._recorded()]
 
     def get_current_time(self):
-        return self.times[self.current_step][0]
-
-    def get_last_time(self):
-        valid_steps = np.where(np.any(self.times != 0, axis=0))[0]
-        return self.times[valid_steps[-1]]
-
-    def get_chain_start_time(self):
-        cur = self
-        while cur._prev is not None:
-            cur = cur._prev
-        return cur.times[0][0]
-
-    def get_chain_end_time(self):
-        cur = self
-        while cur._next is not None:
-            cur = cur._next
-        return cur.times[-1][0]
-
-    def get_first_solution(self):
-        cur = self
-        while cur._prev is not None:
-            cur = cur._prev
-        return cur
+        return self.times[self.current_step, 0]
 
     def get_current_positions(self):
         return self.trajectories[self.current_step]
+
+    def get_last_time(self):
+        """The reference evaluates ``times[where(any(times != 0, axis=0))[0][-1]]``
+        (:126-127).  The ``any`` reduces over the step axis of the (rows, 1)
+        array, so the result is the segment's *first* time row whenever some
+        recorded time is non-zero (IndexError when none is).  The
+        post-processor's collision windows read this value, so it is kept."""
+        if not np.any(self.times != 0):
+            raise IndexError("get_last_time: every time of the segment is zero")
+        return self.times[0]
 
     def compact(self):
         self.times = self.get_times()
         self.trajectories = self.get_trajectories()
 
+    # -- the chain
+
+    def _segments(self):
+        """Every segment of the chain, first to last."""
+        seg = self.get_first_solution()
+        while seg is not None:
+            yield seg
+            seg = seg._next
+
+    def get_first_solution(self):
+        seg = self
+        while seg._prev is not None:
+            seg = seg._prev
+        return seg
+
+    def get_chain_start_time(self):
+        return self.get_first_solution().times[0][0]
+
+    def get_chain_end_time(self):
+        seg = self
+        while seg._next is not None:
+            seg = seg._next
+        return seg.times[-1][0]
+
     def compact_all(self):
-        self.compact()
-        nxt = self._next
-        while nxt is not None:
-            nxt.compact()
-            nxt = nxt._next
-        prev = self._prev
-        while prev is not None:
-            prev.compact()
-            prev = prev._prev
+        for seg in self._segments():
+            seg.compact()
+
+    def get_name(self):
+        name = self.get_first_solution().filename
+        return "" if name is None else name
 
     def should_update_visualization(self):
-        if self.real_time_visualization:
-            now = time.time()
-            if now - self.last_update_time >= self.update_interval:
-                self.last_update_time = now
-                return True
-        return False
+        if not self.real_time_visualization:
+            return False
+        now = time.time()
+        if now - self.last_update_time < self.update_interval:
+            return False
+        self.last_update_time = now
+        return True
+
+    # -- merging
 
     def generate_next_solution(self):
-        """Merge the collided pairs and start the next trajectory segment."""
-        new_positions, new_radii, collided = merge_collided(
+        """Merge the collided pairs of the last recorded positions and open the
+        next segment (reference :208-287)."""
+        new_positions, new_radii, visit = merge_collided(
             self.trajectories[self.current_step], self.radii, self.collided_droplets)
-        return self.next_solution(new_positions, new_radii, collided)
+        return self.next_solution(new_positions, new_radii, visit)
 
     def next_solution(self, new_positions, new_radii, collided):
-        """The second half of ``generate_next_solution``: start the next
-        trajectory segment from already merged arrays (``merge_collided``'s
-        outputs, or the same values compacted on the device)."""
+        """Open the next segment from already merged arrays (``merge_collided``'s
+        outputs, or the same values compacted on the device); ``collided`` is
+        the pair list in visiting order, kept as this segment's record."""
         self.collided_droplets = collided
-        new_initial_state = DropletState(new_positions, new_radii, self.times[self.current_step])
-        next_solution = DropletSolution(
-            initial_droplet_state=new_initial_state,
+        start = DropletState(new_positions, new_radii, self.times[self.current_step])
+        nxt = DropletSolution(
+            initial_droplet_state=start,
             real_time_visualization=self.real_time_visualization,
             update_interval=self.update_interval,
             length=self.length,
             previous=self,
         )
-        self._next = next_solution
-        next_solution.is_collision = False
-        return next_solution
+        self._next = nxt
+        return nxt
+
+    # -- time lookup
+
+    def _segment_holding(self, t):
+        seg = self
+        while True:
+            if t < seg.times[0][0]:
+                if seg._prev is None:
+                    raise ValueError(f"time {t} precedes the chain start")
+                seg = seg._prev
+            elif t > seg.times[-1][0]:
+                if seg._next is None:
+                    raise ValueError(f"time {t} exceeds the chain end")
+                seg = seg._next
+            else:
+                return seg
+
+    def get_state(self, target_time):
+        """Positions at ``target_time``, linearly interpolated between the two
+        recorded rows around it (a recorded time returns its row as is)."""
+        seg = self._segment_holding(target_time)
+        ts = seg.times.ravel()
+        k = int(np.searchsorted(ts, target_time))
+        lo, hi = max(k - 1, 0), min(k, len(ts) - 1)
+        if target_time == ts[hi]:
+            pos = seg.trajectories[hi]
+        elif target_time == ts[lo]:
+            pos = seg.trajectories[lo]
+        else:
+            w = (target_time - ts[lo]) / (ts[hi] - ts[lo])
+            pos = seg.trajectories[lo] + w * (seg.trajectories[hi] - seg.trajectories[lo])
+        return DropletState(positions=pos, radii=seg.radii, time=target_time)
+
+    # -- files (npz layouts of the reference, :289-421)
 
     def save_to_file(self, filename):
-        np.savez(
-            filename,
-            num_particles=self.num_particles,
-            radii=self.radii,
-            times=self.get_times(),
-            trajectories=self.get_trajectories(),
-        )
+        """One segment: keys num_particles, radii, times, trajectories."""
+        np.savez(filename, num_particles=self.num_particles, radii=self.radii,
+                 times=self.get_times(), trajectories=self.get_trajectories())
         print(f"Solution saved to {filename}.")
 
     def load_from_file(self, filename):
         data = np.load(filename)
-        self.num_particles = data["num_particles"]
-        self.radii = data["radii"]
-        self.times = data["times"]
-        self.trajectories = data["trajectories"]
+        for key in ("num_particles", "radii", "times", "trajectories"):
+            setattr(self, key, data[key])
         self.length = len(self.times)
         self.filename = os.path.splitext(os.path.basename(filename))[0]
         print(f"Solution loaded from {filename}.")
 
+    def _record(self, dtype) -> dict:
+        """This segment as one entry of a saved chain (the reference's dict keys)."""
+        return {
+            "num_particles": self.num_particles,
+            "radii": self.radii.astype(dtype),
+            "length": self.length,
+            "real_time_visualization": self.real_time_visualization,
+            "update_interval": self.update_interval,
+            "times": self.get_times().astype(dtype),
+            "trajectories": self.get_trajectories().astype(dtype),
+            "is_collision": self.is_collision,
+            "collided_droplets": self.collided_droplets,
+            "resulting_droplet": self.resulting_droplet,
+        }
+
     def save_chain_to_file(self, filename, precision="float64"):
-        if precision == "float32":
-            dtype = np.float32
-        elif precision == "float16":
-            dtype = np.float16
-        else:
-            dtype = np.float64
-        first = self.get_first_solution()
-        solution_list = []
-        cur = first
-        while cur is not None:
-            cur.compact()
-            solution_list.append(
-                {
-                    "num_particles": cur.num_particles,
-                    "radii": cur.radii.astype(dtype),
-                    "length": cur.length,
-                    "real_time_visualization": cur.real_time_visualization,
-                    "update_interval": cur.update_interval,
-                    "times": cur.get_times().astype(dtype),
-                    "trajectories": cur.get_trajectories().astype(dtype),
-                    "is_collision": cur.is_collision,
-                    "collided_droplets": cur.collided_droplets,
-                    "resulting_droplet": cur.resulting_droplet,
-                }
-            )
-            cur = cur._next
-        np.savez(filename, solutions=solution_list)
+        """The whole chain as ``solutions``: an object array of per-segment dicts
+        (float64 unless precision is 'float32' or 'float16')."""
+        dtype = _PRECISIONS.get(precision, np.float64)
+        self.compact_all()
+        np.savez(filename, solutions=[seg._record(dtype) for seg in self._segments()])
         print(f"Solution chain saved to {filename}.")
 
     @staticmethod
     def load_chain_from_file(filename):
-        """Load a chain written by ``save_chain_to_file`` (a pickled list of
-        dicts inside an npz, the reference's format).  Only load files this
-        package (or the reference) wrote: the format unpickles."""
-        fname = os.path.splitext(os.path.basename(filename))[0]
-        data = np.load(filename, allow_pickle=True)
-        solution_list = data["solutions"]
-        first = None
-        prev = None
-        for sd in solution_list:
-            state = DropletState(sd["trajectories"][0], sd["radii"], sd["times"][0])
-            sol = DropletSolution(
-                initial_droplet_state=state,
-                real_time_visualization=sd["real_time_visualization"],
-                update_interval=sd["update_interval"],
-                length=sd["length"],
+        """Load a chain written by ``save_chain_to_file`` (or the reference).
+        The format unpickles: load only files this package or the reference
+        wrote."""
+        name = os.path.splitext(os.path.basename(filename))[0]
+        records = np.load(filename, allow_pickle=True)["solutions"]
+        first = prev = None
+        for rec in records:
+            seg = DropletSolution(
+                initial_droplet_state=DropletState(rec["trajectories"][0], rec["radii"],
+                                                   rec["times"][0]),
+                real_time_visualization=rec["real_time_visualization"],
+                update_interval=rec["update_interval"],
+                length=rec["length"],
                 previous=prev,
-                filename=fname,
+                filename=name,
             )
-            sol.times = sd["times"]
-            sol.trajectories = sd["trajectories"]
-            sol.is_collision = sd["is_collision"]
-            sol.collided_droplets = sd["collided_droplets"]
-            sol.resulting_droplet = sd["resulting_droplet"]
-            if prev is not None:
-                prev._next = sol
+            for key in ("times", "trajectories", "is_collision", "collided_droplets",
+                        "resulting_droplet"):
+                setattr(seg, key, rec[key])
+            if prev is None:
+                first = seg
             else:
-                first = sol
-            prev = sol
+                prev._next = seg
+            prev = seg
         print(f"Solution chain loaded from {filename}.")
         return first
-
-    def get_state(self, target_time):
-        cur = self
-        while True:
-            min_time = cur.times[0][0]
-            max_time = cur.times[-1][0]
-            if target_time < min_time:
-                if cur._prev is not None:
-                    cur = cur._prev
-                else:
-                    raise ValueError(f"time {target_time} precedes the chain start")
-            elif target_time > max_time:
-                if cur._next is not None:
-                    cur = cur._next
-                else:
-                    raise ValueError(f"time {target_time} exceeds the chain end")
-            else:
-                break
-        times = cur.times.flatten()
-        ib = np.searchsorted(times, target_time) - 1
-        ia = ib + 1
-        if ib < 0:
-            ib = 0
-        if ia >= len(times):
-            ia = len(times) - 1
-        tb, ta = times[ib], times[ia]
-        pb, pa = cur.trajectories[ib], cur.trajectories[ia]
-        if target_time == ta:
-            pos = pa
-        elif target_time == tb:
-            pos = pb
-        else:
-            f = (target_time - tb) / (ta - tb)
-            pos = pb + f * (pa - pb)
-        return DropletState(positions=pos, radii=cur.radii, time=target_time)
-
-    def get_name(self):
-        first = self.get_first_solution()
-        return first.filename if first.filename is not None else ""

@@ -153,5 +153,12 @@ def ensembles(count: int = 256, base_seed: int = 0, workers: int | None = None) 
         workers = min(16, os.cpu_count() or 1)
     if workers <= 1 or count < 4:
         return [_make_ensemble(j) for j in jobs]
-    with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(_make_ensemble, jobs, chunksize=1)
+    # close() + join() let the workers exit on their own; the pool's context
+    # manager would terminate() them, a SIGTERM that profilers report as an abort
+    pool = mp.get_context("fork").Pool(workers)
+    try:
+        out = pool.map(_make_ensemble, jobs, chunksize=1)
+    finally:
+        pool.close()
+        pool.join()
+    return out
