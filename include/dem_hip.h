@@ -174,6 +174,25 @@ int dem_hash_pairs(const double* pos, const double* radii, int n, double L,
                    int pair_capacity, int32_t* count_out, void* ws,
                    size_t ws_bytes, dem_stream_t stream);
 
+/* Bytes of the built-in scratch of dem_hash_pairs_noalloc (per GPU). */
+#define DEM_HASH_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* dem_hash_pairs with no workspace argument and no allocation: the scratch is
+ * a fixed array of DEM_HASH_NOALLOC_SCRATCH_BYTES in device memory, one per
+ * GPU, held until the process ends.  Same results as dem_hash_pairs.  Sizes
+ * whose workspace (dem_hash_workspace_bytes(n, table_size)) exceeds the
+ * scratch are rejected with DEM_ERR_CAPACITY (n = 65536 with table_size
+ * 131072 needs about 5 MB; n up to ~400000 fits with the reference's table
+ * size).  Calls that overlap on one GPU share the scratch and race: issue
+ * them on one stream or serialise them.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int dem_hash_pairs_noalloc(const double* pos, const double* radii, int n, double L,
+                           int periodic, int table_size, int32_t* pairs,
+                           int pair_capacity, int32_t* count_out,
+                           dem_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Explicit Euler step (float64, bit-exact with the reference's numpy)        */
 /* ------------------------------------------------------------------------ */

@@ -25,6 +25,8 @@
 // float64 arithmetic.
 #include <math.h>
 
+#include <mutex>
+
 #include "dem_common.h"
 #include "hash_core.h"
 #include "scan.h"
@@ -255,6 +257,53 @@ __global__ void k_fill(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+int check_hash_args(const double* pos, const double* radii, int n, int table_size,
+                    const int32_t* pairs, int pair_capacity, const int32_t* count_out) {
+  if (n < 0) return dem::set_error(DEM_ERR_INVALID, "n must be >= 0");
+  if (table_size < 1 || (table_size & (table_size - 1)))
+    return dem::set_error(DEM_ERR_INVALID, "table_size must be a power of two (got %d)", table_size);
+  if (!count_out) return dem::set_error(DEM_ERR_INVALID, "count_out is NULL");
+  if (pair_capacity < 0 || (pair_capacity > 0 && !pairs))
+    return dem::set_error(DEM_ERR_INVALID, "bad pair buffer");
+  if (n > 0 && (!pos || !radii)) return dem::set_error(DEM_ERR_INVALID, "null positions/radii");
+  return DEM_OK;
+}
+
+int run_hash(const Layout& l, const double* pos, const double* radii, int n, double L, int periodic,
+             int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out,
+             hipStream_t st);
+
+// Scratch of the allocation-free entry point: one array per GPU, held until
+// the process ends.  Host code never names the array in device work: its
+// address on the current device comes from hipGetSymbolAddress, once per
+// device, and is kept in g_scratch_base.  Calls that overlap on one GPU share
+// the array and race; every call rewrites what it reads (bucket counts are
+// cleared, cursors and pair counts are rewritten) before reading it.
+static_assert(DEM_HASH_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) unsigned char scratch[DEM_HASH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_base[kMaxDevices] = {};
+
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  DEM_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return dem::set_error(DEM_ERR_INVALID, "device %d outside the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_base[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return dem::set_error(DEM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                            hipGetErrorString(e));
+    g_scratch_base[dev] = p;
+  }
+  *out = g_scratch_base[dev];
+  return DEM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,17 +317,40 @@ size_t dem_hash_workspace_bytes(int n, int table_size) {
 int dem_hash_pairs(const double* pos, const double* radii, int n, double L, int periodic,
                    int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out, void* ws,
                    size_t ws_bytes, dem_stream_t stream) {
-  if (n < 0) return dem::set_error(DEM_ERR_INVALID, "n must be >= 0");
-  if (table_size < 1 || (table_size & (table_size - 1)))
-    return dem::set_error(DEM_ERR_INVALID, "table_size must be a power of two (got %d)", table_size);
-  if (!count_out) return dem::set_error(DEM_ERR_INVALID, "count_out is NULL");
-  if (pair_capacity < 0 || (pair_capacity > 0 && !pairs))
-    return dem::set_error(DEM_ERR_INVALID, "bad pair buffer");
-  if (n > 0 && (!pos || !radii)) return dem::set_error(DEM_ERR_INVALID, "null positions/radii");
-  hipStream_t st = dem::as_stream(stream);
+  int rc = check_hash_args(pos, radii, n, table_size, pairs, pair_capacity, count_out);
+  if (rc) return rc;
   Layout l = carve(ws, n, table_size);
   if (!ws || ws_bytes < l.bytes)
     return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
+  return run_hash(l, pos, radii, n, L, periodic, table_size, pairs, pair_capacity, count_out,
+                  dem::as_stream(stream));
+}
+
+int dem_hash_pairs_noalloc(const double* pos, const double* radii, int n, double L, int periodic,
+                           int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out,
+                           dem_stream_t stream) {
+  int rc = check_hash_args(pos, radii, n, table_size, pairs, pair_capacity, count_out);
+  if (rc) return rc;
+  const size_t need = carve(nullptr, n, table_size).bytes;
+  if (need > sizeof(scratch))
+    return dem::set_error(DEM_ERR_CAPACITY,
+                          "dem_hash_pairs_noalloc: n = %d with table_size = %d needs %zu bytes of "
+                          "scratch; the built-in scratch holds %zu (DEM_HASH_NOALLOC_SCRATCH_BYTES)",
+                          n, table_size, need, sizeof(scratch));
+  void* base = nullptr;
+  rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  return run_hash(carve(base, n, table_size), pos, radii, n, L, periodic, table_size, pairs,
+                  pair_capacity, count_out, dem::as_stream(stream));
+}
+
+}  // extern "C"
+
+namespace {
+
+int run_hash(const Layout& l, const double* pos, const double* radii, int n, double L, int periodic,
+             int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out,
+             hipStream_t st) {
   if (n == 0) {
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, count_out, 1, 0);
     DEM_CHECK_LAUNCH("k_fill");
@@ -323,4 +395,4 @@ int dem_hash_pairs(const double* pos, const double* radii, int n, double L, int 
   return DEM_OK;
 }
 
-}  // extern "C"
+}  // namespace

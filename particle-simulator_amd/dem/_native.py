@@ -125,6 +125,7 @@ SIGNATURES: dict[str, tuple] = {
     "dem_octree_set_exact_mac": (_I, [_I]),
     "dem_hash_workspace_bytes": (_S, [_I, _I]),
     "dem_hash_pairs": (_I, [_P, _P, _I, _D, _I, _I, _P, _I, _P, _P, _S, _P]),
+    "dem_hash_pairs_noalloc": (_I, [_P, _P, _I, _D, _I, _I, _P, _I, _P, _P]),
     "dem_euler_step": (_I, [_P, _P, _P, _P, _I, _D, _D, _D, _I, _P]),
     "dem_euler_step_total": (_I, [_P, _P, _I, _D, _D, _I, _P]),
     "dem_merge_compact": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, _P, _P]),
