@@ -15,21 +15,34 @@
 // the periodic miss across a narrow last cell when L/cell is not an integer.
 // This file reproduces exactly that set.
 //
-// MI355X design: counting sort of particles by bucket (atomic counts, device
-// scan, atomic scatter with a bucket-ordered copy of (x, y, z, r)), then two
-// passes over targets in bucket order: count pairs per i, scan, emit.  Each target visits each distinct bucket once, so pairs are
-// unique without a global sort; each target's pairs land in its own segment,
-// sorted by j in-segment, which makes the output lexicographic and
-// deterministic.  The contact predicate is evaluated in float64 with FMA
-// contraction disabled so that it rounds like the reference's element-wise
-// float64 arithmetic.
+// MI355X design, seven launches and no host synchronisation:
+//   prep     clear the bucket counters, partial maxima of the radii;
+//   assign   cell and bucket of every droplet, bucket counts (atomics);
+//   scan     per-tile exclusive scan of the bucket counts (4096-entry tiles,
+//            one launch; consumers add the prefix of the tile sums themselves);
+//   scatter  bucket offsets, droplets in bucket order with a bucket-ordered
+//            copy of (x, y, z, r);
+//   search   one wave per run of buckets: the droplets of a bucket that share
+//            a cell share their neighbour buckets, so the wave loads those
+//            buckets' candidates once (contiguous slots) and tests them
+//            against every target of the cell; each target's count and its
+//            first kStash partners (sorted by j) are kept;
+//   scan     per-tile scan of the per-droplet pair counts;
+//   emit     each droplet's pairs at its offset (its stash, or a rescan of its
+//            neighbourhood when it has more partners than the stash holds);
+//            the total goes to *count_out.
+// Each target visits each distinct neighbour bucket once, so pairs are unique
+// without a global sort; each target's pairs land in its own segment sorted by
+// j, which makes the output lexicographic and deterministic whatever order
+// the atomics give the slots.  The contact predicate is evaluated in float64
+// with FMA contraction disabled so that it rounds like the reference's
+// element-wise float64 arithmetic.
 #include <math.h>
 
 #include <mutex>
 
 #include "dem_common.h"
 #include "hash_core.h"
-#include "scan.h"
 
 namespace {
 
@@ -43,218 +56,496 @@ struct HashParams {
   int skip;     // 1: cell <= 0, no contacts
 };
 
+constexpr int kThreads = 256;
+constexpr int kTileItems = 16;
+constexpr int kTile = kThreads * kTileItems;  // entries per scan tile
+constexpr int kMaxTiles = 8192;               // so table_size and n are at most 2^25
+constexpr int kMaxRBlocks = 256;              // partial maxima of the radii
+constexpr int kStash = 8;                     // partners kept per target by the search
+constexpr int kBucketsPerWave = 8;
+
+inline int tiles(int n) { return (n + kTile - 1) / kTile; }
+
 struct Layout {
   HashParams* hp;
-  double* rmax_part;  // per-block max radius
-  int* bucket;        // [n]
-  int* counts;        // [T]
-  int* offsets;       // [T]
-  int* cursor;        // [T]
+  double* rmax_part;  // [kMaxRBlocks]
+  int* bucket;        // [n] bucket of each droplet
+  int* counts;        // [T] droplets per bucket
+  int* offsets;       // [T] first slot of each bucket
+  int* fill;          // [T] slots taken so far (scatter); with local[T] after it, reused
+  int* local;         // [T] as the search's stash: kStash partners per droplet
+  int* stash;         // (= fill; at least kStash n ints)
+  int* btile;         // [tiles(T)] tile sums of the bucket counts
   int* sorted;        // [n] droplet of each slot, bucket order
   double4* spos;      // [n] (x, y, z, r) of each slot
   int* pair_cnt;      // [n]
-  int* pair_off;      // [n]
-  int* scan_ws;
+  int* pscan;         // [n] per-tile exclusive scan of pair_cnt
+  int* ptile;         // [tiles(n)]
   size_t bytes;
 };
-
-constexpr int kMaxBlocks = 256;
-constexpr int kCountSplit = 4;  // lanes per droplet in the count pass
 
 Layout carve(void* ws, int n, int T) {
   dem::Carver c(ws);
   Layout l;
   const size_t nn = (size_t)(n > 0 ? n : 1);
-  const int big = T > n ? T : n;
+  const size_t tt = (size_t)T;
   l.hp = c.take<HashParams>(1);
-  l.rmax_part = c.take<double>(kMaxBlocks);
+  l.rmax_part = c.take<double>(kMaxRBlocks);
   l.bucket = c.take<int>(nn);
-  l.counts = c.take<int>((size_t)T);
-  l.offsets = c.take<int>((size_t)T);
-  l.cursor = c.take<int>((size_t)T);
+  l.counts = c.take<int>(tt);
+  l.offsets = c.take<int>(tt);
+  const size_t shared = 2 * tt > kStash * nn ? 2 * tt : kStash * nn;
+  l.fill = c.take<int>(shared);
+  l.local = l.fill + tt;
+  l.stash = l.fill;
+  l.btile = c.take<int>((size_t)tiles(T));
   l.sorted = c.take<int>(nn);
   l.spos = c.take<double4>(nn);
   l.pair_cnt = c.take<int>(nn);
-  l.pair_off = c.take<int>(nn);
-  l.scan_ws = c.take<int>(dem::scan_workspace_ints(big));
+  l.pscan = c.take<int>(nn);
+  l.ptile = c.take<int>((size_t)tiles((int)nn));
   l.bytes = dem::align_up(c.used, 256);
   return l;
 }
 
-__global__ __launch_bounds__(256) void k_rmax_partial(const double* __restrict__ r, int n,
-                                                      double* __restrict__ part) {
-  __shared__ double sh[256];
-  double m = -INFINITY;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, r[i]);
-  sh[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
+// ---------------------------------------------------------------- block helpers
+
+__device__ __forceinline__ int wave_inclusive(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  return v;
 }
 
-// One wave: the largest radius over the partial maxima (fmax is exact and
-// order-free), then the cell size and grid.
-__global__ void k_hash_setup(const double* __restrict__ part, int nblk, double L,
-                             HashParams* __restrict__ hp) {
+// Exclusive scan of one value per thread over the block (kThreads threads).
+__device__ __forceinline__ int block_exclusive(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int inc = wave_inclusive(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kThreads / 64; ++k) {
+    const int s = wsum[k];
+    before += k < w ? s : 0;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+// pre[k] = sum of tile_sums[0..k) for k < nt (LDS), and the grand total.
+__device__ __forceinline__ int tile_prefix(const int* __restrict__ tile_sums, int nt, int* pre) {
+  __shared__ int wsum[kThreads / 64];
+  const int per = (nt + kThreads - 1) / kThreads;
+  const int b = threadIdx.x * per;
+  int sum = 0;
+  for (int k = 0; k < per; ++k)
+    if (b + k < nt) sum += tile_sums[b + k];
+  int total;
+  int run = block_exclusive(sum, wsum, &total);
+  for (int k = 0; k < per; ++k)
+    if (b + k < nt) {
+      const int v = tile_sums[b + k];
+      pre[b + k] = run;
+      run += v;
+    }
+  __syncthreads();
+  return total;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Clears the bucket counters and slot cursors; blocks b < nrb also write the
+// largest radius of droplets b*256 + k*nrb*256 (fmax: exact, order-free).
+__global__ __launch_bounds__(kThreads) void k_hash_prep(const double* __restrict__ r, int n, int nrb,
+                                                        int* __restrict__ counts,
+                                                        int* __restrict__ fill, int T,
+                                                        double* __restrict__ rmax_part) {
+  for (int h = blockIdx.x * kThreads + threadIdx.x; h < T; h += gridDim.x * kThreads) {
+    counts[h] = 0;
+    fill[h] = 0;
+  }
+  if ((int)blockIdx.x >= nrb) return;
   double m = -INFINITY;
-  for (int b = threadIdx.x; b < nblk; b += 64) m = fmax(m, part[b]);
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += nrb * kThreads) m = fmax(m, r[i]);
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-  if (threadIdx.x != 0) return;
-  HashParams p;
-  p.cell = 2.0 * m;
-  p.skip = !(p.cell > 0.0);
-  double q = p.skip ? 1.0 : ceil(L / p.cell);
-  if (!(q >= 1.0)) q = 1.0;
-  if (q > 1073741824.0) q = 1073741824.0;
-  p.g = (int)q;
-  *hp = p;
+  __shared__ double wm[kThreads / 64];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, wm[w]);
+    rmax_part[blockIdx.x] = fmax(m, wm[0]);
+  }
 }
 
-__global__ __launch_bounds__(256) void k_assign(const double* __restrict__ pos, int n, int periodic,
-                                                int mask, const HashParams* __restrict__ hpp,
-                                                int* __restrict__ bucket, int* __restrict__ counts) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const HashParams hp = *hpp;
-  if (hp.skip) return;
-  int ix = cell_of(pos[3 * (size_t)i], hp.cell, hp.g, periodic);
-  int iy = cell_of(pos[3 * (size_t)i + 1], hp.cell, hp.g, periodic);
-  int iz = cell_of(pos[3 * (size_t)i + 2], hp.cell, hp.g, periodic);
-  int h = hash_cell(ix, iy, iz, mask);
+// Cell size and grid from the largest radius (every block derives them; block
+// 0 stores them), then the bucket of every droplet and the bucket counts.
+__global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restrict__ pos, int n,
+                                                          int periodic, int mask, double L,
+                                                          const double* __restrict__ rmax_part,
+                                                          int nrb, HashParams* __restrict__ hpo,
+                                                          int* __restrict__ bucket,
+                                                          int* __restrict__ counts) {
+  __shared__ HashParams shp;
+  if (threadIdx.x < 64) {
+    double m = -INFINITY;
+    for (int b = threadIdx.x; b < nrb; b += 64) m = fmax(m, rmax_part[b]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) {
+      HashParams p;
+      p.cell = 2.0 * m;
+      p.skip = !(p.cell > 0.0);
+      double q = p.skip ? 1.0 : ceil(L / p.cell);
+      if (!(q >= 1.0)) q = 1.0;
+      if (q > 1073741824.0) q = 1073741824.0;
+      p.g = (int)q;
+      shp = p;
+      if (blockIdx.x == 0) *hpo = p;
+    }
+  }
+  __syncthreads();
+  const HashParams hp = shp;
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n || hp.skip) return;
+  const int ix = cell_of(pos[3 * (size_t)i], hp.cell, hp.g, periodic);
+  const int iy = cell_of(pos[3 * (size_t)i + 1], hp.cell, hp.g, periodic);
+  const int iz = cell_of(pos[3 * (size_t)i + 2], hp.cell, hp.g, periodic);
+  const int h = hash_cell(ix, iy, iz, mask);
   bucket[i] = h;
   atomicAdd(&counts[h], 1);
 }
 
-__global__ __launch_bounds__(256) void k_scatter(int n, const HashParams* __restrict__ hpp,
-                                                 const int* __restrict__ bucket,
-                                                 const double* __restrict__ pos,
-                                                 const double* __restrict__ r,
-                                                 int* __restrict__ cursor, int* __restrict__ sorted,
-                                                 double4* __restrict__ spos) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n || hpp->skip) return;
-  int slot = atomicAdd(&cursor[bucket[i]], 1);
-  sorted[slot] = i;
-  spos[slot] = make_double4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2],
-                            r[i]);
+// Exclusive scan of in[0, n) within tiles of kTile entries (out) and the tile
+// sums; the global prefix of entry k is out[k] + (sum of tile sums before k's tile).
+__global__ __launch_bounds__(kThreads) void k_tile_scan(const int* __restrict__ in,
+                                                        int* __restrict__ out,
+                                                        int* __restrict__ tile_sums, int n) {
+  // padded LDS: entry e at e + e / 32 (a thread's 16 consecutive entries
+  // then fall in distinct banks for neighbouring threads)
+  __shared__ int s[kTile + kTile / 32];
+  __shared__ int wsum[kThreads / 64];
+  const int base = blockIdx.x * kTile, t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int e = k * kThreads + t;
+    s[e + (e >> 5)] = base + e < n ? in[base + e] : 0;
+  }
+  __syncthreads();
+  int v[kTileItems];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int e = t * kTileItems + k;
+    v[k] = s[e + (e >> 5)];
+    sum += v[k];
+  }
+  int total;
+  int run = block_exclusive(sum, wsum, &total);
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int e = t * kTileItems + k;
+    s[e + (e >> 5)] = run;
+    run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int e = k * kThreads + t;
+    if (base + e < n) out[base + e] = s[e + (e >> 5)];
+  }
+  if (t == 0) tile_sums[blockIdx.x] = total;
 }
 
-// EMIT = false: count pairs of each target; EMIT = true: write them.  The
-// droplet in bucket slot t is searched by SPLIT consecutive lanes (neighbouring
-// slots then search the same buckets), lane `sub` taking the distinct
-// neighbour buckets k = sub, sub + SPLIT, ...; their counts are summed across
-// the lanes.  The buckets' ranges are loaded up front and candidates come from
-// the bucket-ordered (x, y, z, r) copy; the droplet index of a slot is read
-// only for a contact.  The emit pass (SPLIT = 1) skips droplets the count
-// pass found without pairs.
-template <bool EMIT, int SPLIT>
-__global__ __launch_bounds__(256) void k_pairs(int n, double L, int periodic, int mask,
-                                               const HashParams* __restrict__ hpp,
-                                               const int* __restrict__ counts,
-                                               const int* __restrict__ offsets,
-                                               const int* __restrict__ sorted,
-                                               const double4* __restrict__ spos,
-                                               int* __restrict__ pair_cnt,
-                                               const int* __restrict__ pair_off,
-                                               int* __restrict__ pairs, int cap) {
-  static_assert(!EMIT || SPLIT == 1, "the emit pass writes a target's pairs from one lane");
-  static_assert(SPLIT == 1 || SPLIT == 2 || SPLIT == 4, "lanes per target");
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int t = gid / SPLIT, sub = gid % SPLIT;
-  if (t >= n) return;  // the SPLIT lanes of a target leave together
-  const HashParams hp = *hpp;
-  if (hp.skip) {  // nothing was bucketed: slot t stands for droplet t
-    if (!EMIT && sub == 0) pair_cnt[t] = 0;
-    return;
-  }
-  const int g = hp.g;
-  const int i = sorted[t];
-  if (EMIT && pair_cnt[i] == 0) return;
-  const double4 me = spos[t];
-  const int ix = cell_of(me.x, hp.cell, g, periodic);
-  const int iy = cell_of(me.y, hp.cell, g, periodic);
-  const int iz = cell_of(me.z, hp.cell, g, periodic);
-  int nb[27];
-  int nseen = 0;
-  int cx[3], cy[3], cz[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    cx[d] = dem::nb_coord(ix, d - 1, g, periodic);
-    cy[d] = dem::nb_coord(iy, d - 1, g, periodic);
-    cz[d] = dem::nb_coord(iz, d - 1, g, periodic);
-  }
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    const int nx = cx[k / 9], ny = cy[(k / 3) % 3], nz = cz[k % 3];
-    const bool valid = nx >= 0 && ny >= 0 && nz >= 0;
-    if (valid) {
-      const int nh = hash_cell(nx, ny, nz, mask);
-      bool dup = false;
-#pragma unroll
-      for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
-      if (!dup) nb[nseen++] = nh;
+// Bucket offsets (all T buckets) and the droplets in bucket order.  A cleared
+// grid (no buckets) leaves every droplet without pairs.
+__global__ __launch_bounds__(kThreads) void k_hash_scatter(
+    int n, int T, const HashParams* __restrict__ hpp, const int* __restrict__ bucket,
+    const double* __restrict__ pos, const double* __restrict__ r, const int* __restrict__ local,
+    const int* __restrict__ btile, int ntile, int* __restrict__ offsets, int* __restrict__ fill,
+    int* __restrict__ sorted, double4* __restrict__ spos, int* __restrict__ pair_cnt) {
+  extern __shared__ int pre[];
+  tile_prefix(btile, ntile, pre);
+  const int stride = gridDim.x * kThreads;
+  const int g0 = blockIdx.x * kThreads + threadIdx.x;
+  for (int h = g0; h < T; h += stride) offsets[h] = local[h] + pre[h / kTile];
+  const bool skip = hpp->skip;
+  for (int i = g0; i < n; i += stride) {
+    if (skip) {
+      pair_cnt[i] = 0;
+      continue;
     }
+    const int h = bucket[i];
+    const int slot = local[h] + pre[h / kTile] + atomicAdd(&fill[h], 1);
+    sorted[slot] = i;
+    spos[slot] = make_double4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2],
+                              r[i]);
   }
-  constexpr int KS = (27 + SPLIT - 1) / SPLIT;
-  int s0[KS], s1[KS];
-#pragma unroll
-  for (int u = 0; u < KS; ++u) {
-    const int k = sub + SPLIT * u;
-    const bool ok = k < nseen;
-    const int h = ok ? nb[k < 27 ? k : 0] : 0;
-    s0[u] = offsets[h];
-    s1[u] = ok ? s0[u] + counts[h] : s0[u];
+}
+
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Distinct neighbour buckets of a cell: lane k < 27 gets neighbour k's bucket,
+// or -1 when it is outside an open box or repeats the bucket of a lower lane.
+__device__ __forceinline__ int neighbour_bucket(int ix, int iy, int iz, int g, int periodic,
+                                                int mask) {
+  const int lane = threadIdx.x & 63;
+  int nh = -1;
+  if (lane < 27) {
+    const int nx = dem::nb_coord(ix, lane / 9 - 1, g, periodic);
+    const int ny = dem::nb_coord(iy, (lane / 3) % 3 - 1, g, periodic);
+    const int nz = dem::nb_coord(iz, lane % 3 - 1, g, periodic);
+    if (nx >= 0 && ny >= 0 && nz >= 0) nh = hash_cell(nx, ny, nz, mask);
   }
-  int cnt = 0;
-  const int base = EMIT ? pair_off[i] : 0;
+  bool dup = false;
 #pragma unroll
-  for (int u = 0; u < KS; ++u) {
-    for (int slot = s0[u]; slot < s1[u]; ++slot) {
-      const double4 q = spos[slot];
-      if (contact_pair(me.x, me.y, me.z, me.w, q.x, q.y, q.z, q.w, L, periodic)) {
-        const int j = sorted[slot];
-        if (i < j) {
-          if (EMIT) {
-            const int at = base + cnt;
-            if (at < cap) {
-              pairs[2 * (size_t)at] = i;
-              pairs[2 * (size_t)at + 1] = j;
+  for (int q = 0; q < 26; ++q) {
+    const int v = readlane(nh, q);
+    dup |= (q < lane && v >= 0 && v == nh);
+  }
+  return dup ? -1 : nh;
+}
+
+// Orders one wave's LDS accesses across its lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct SearchLds {
+  double x[64], y[64], z[64], r[64];  // the batch's targets
+  int id[64];                         // their droplet indices
+  int cnt[64];                        // partners found (i < j)
+  int list[64][kStash];               // the first kStash of them
+  int start[27];                      // neighbour bucket k: first slot
+  int before[27];                     // candidates in neighbours < k
+};
+
+// One wave per kBucketsPerWave consecutive buckets (their counts and offsets
+// probed by one load per lane).  Targets of a bucket are taken 64 at a time
+// and grouped by cell (aliased cells of one bucket have different
+// neighbourhoods); a group's candidates are the slots of its distinct
+// neighbour buckets, spread over the lanes and tested against every target
+// of the group.  Cross-lane values move by readlane (scalar) where the source
+// lane is uniform.
+__global__ __launch_bounds__(kThreads) void k_hash_search(
+    int T, double L, int periodic, int mask, const HashParams* __restrict__ hpp,
+    const int* __restrict__ counts, const int* __restrict__ offsets,
+    const int* __restrict__ sorted, const double4* __restrict__ spos, int* __restrict__ pair_cnt,
+    int* __restrict__ stash) {
+  __shared__ SearchLds lds_all[kThreads / 64];
+  SearchLds& S = lds_all[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const HashParams hp = *hpp;
+  if (hp.skip) return;
+  const int g = hp.g;
+  const int b0 = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * kBucketsPerWave;
+  int my_c = 0, my_o = 0;
+  if (lane < kBucketsPerWave && b0 + lane < T) {
+    my_c = counts[b0 + lane];
+    my_o = offsets[b0 + lane];
+  }
+  unsigned long long busy = __ballot(my_c > 0);
+  while (busy) {
+    const int kb = __ffsll((long long)busy) - 1;
+    busy &= busy - 1;
+    const int c = readlane(my_c, kb), o = readlane(my_o, kb);
+    for (int t0 = 0; t0 < c; t0 += 64) {
+      const int cb = min(64, c - t0);
+      int kx = 0, ky = 0, kz = 0;
+      if (lane < cb) {
+        const double4 q = spos[o + t0 + lane];
+        S.x[lane] = q.x;
+        S.y[lane] = q.y;
+        S.z[lane] = q.z;
+        S.r[lane] = q.w;
+        S.id[lane] = sorted[o + t0 + lane];
+        S.cnt[lane] = 0;
+        kx = cell_of(q.x, hp.cell, g, periodic);
+        ky = cell_of(q.y, hp.cell, g, periodic);
+        kz = cell_of(q.z, hp.cell, g, periodic);
+      }
+      unsigned long long remaining = __ballot(lane < cb);
+      wave_sync();
+      while (remaining) {
+        const int lead = __ffsll((long long)remaining) - 1;
+        const int gx = readlane(kx, lead), gy = readlane(ky, lead), gz = readlane(kz, lead);
+        const bool mine = ((remaining >> lane) & 1ull) && kx == gx && ky == gy && kz == gz;
+        const unsigned long long members = __ballot(mine);
+        remaining &= ~members;
+        // candidates: the slots of the group's distinct neighbour buckets
+        const int nh = neighbour_bucket(gx, gy, gz, g, periodic, mask);
+        const int len = nh >= 0 ? counts[nh] : 0;
+        const int first = nh >= 0 ? offsets[nh] : 0;
+        int before = 0, M = 0;
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+          const int lq = readlane(len, q);
+          before += q < lane ? lq : 0;
+          M += lq;
+        }
+        if (lane < 27) {
+          S.start[lane] = first;
+          S.before[lane] = before;
+        }
+        const int inc = before + len;  // candidates in neighbours <= lane
+        wave_sync();
+        // a uniform loop: every lane is active at the readlanes (a lane read
+        // while inactive in a divergent region holds no defined value)
+        // two candidates per lane and pass, both loads in flight before the
+        // tests; a uniform loop: every lane is active at the readlanes (a lane
+        // read while inactive in a divergent region holds no defined value)
+        for (int m0 = 0; m0 < M; m0 += 128) {
+          int slot[2];
+          double4 cand[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int m = m0 + 64 * u + lane;
+            int k = 0;  // the neighbour holding candidate m: neighbours whose range ends by m
+#pragma unroll
+            for (int q = 0; q < 26; ++q) k += readlane(inc, q) <= m ? 1 : 0;
+            slot[u] = m < M ? S.start[k] + (m - S.before[k]) : -1;
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            if (slot[u] >= 0) cand[u] = spos[slot[u]];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            if (slot[u] < 0) continue;
+            const double4 q = cand[u];
+            int j = -1;
+            unsigned long long mm = members;
+            while (mm) {
+              const int t = __ffsll((long long)mm) - 1;
+              mm &= mm - 1;
+              if (contact_pair(S.x[t], S.y[t], S.z[t], S.r[t], q.x, q.y, q.z, q.w, L, periodic)) {
+                if (j < 0) j = sorted[slot[u]];
+                if (S.id[t] < j) {
+                  const int p = atomicAdd(&S.cnt[t], 1);
+                  if (p < kStash) S.list[t][p] = j;
+                }
+              }
             }
           }
-          ++cnt;
+        }
+        wave_sync();
+      }
+      if (lane < cb) {
+        const int i = S.id[lane];
+        const int cnt = S.cnt[lane];
+        pair_cnt[i] = cnt;
+        const int kept = min(cnt, kStash);
+        int* lst = S.list[lane];
+        for (int a = 1; a < kept; ++a) {  // insertion sort by j
+          const int v = lst[a];
+          int b = a - 1;
+          while (b >= 0 && lst[b] > v) {
+            lst[b + 1] = lst[b];
+            --b;
+          }
+          lst[b + 1] = v;
+        }
+        for (int a = 0; a < kept; ++a) stash[(size_t)kStash * i + a] = lst[a];
+      }
+      wave_sync();
+    }
+  }
+}
+
+// Pairs of droplet i with every candidate j > i of its neighbourhood, written
+// to pairs[off ..] (within cap) and sorted by j: the search for a droplet
+// with more partners than the stash holds.
+__device__ void rescan_droplet(int i, int off, int cnt, const double* __restrict__ pos,
+                               const double* __restrict__ r, double L, int periodic, int mask,
+                               const HashParams& hp, const int* __restrict__ counts,
+                               const int* __restrict__ offsets, const int* __restrict__ sorted,
+                               const double4* __restrict__ spos, int* __restrict__ pairs,
+                               int cap) {
+  const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+  const double ri = r[i];
+  const int g = hp.g;
+  const int ix = cell_of(xi, hp.cell, g, periodic);
+  const int iy = cell_of(yi, hp.cell, g, periodic);
+  const int iz = cell_of(zi, hp.cell, g, periodic);
+  int nb[27];
+  int nseen = 0;
+  for (int k = 0; k < 27; ++k) {
+    const int nx = dem::nb_coord(ix, k / 9 - 1, g, periodic);
+    const int ny = dem::nb_coord(iy, (k / 3) % 3 - 1, g, periodic);
+    const int nz = dem::nb_coord(iz, k % 3 - 1, g, periodic);
+    if (nx < 0 || ny < 0 || nz < 0) continue;
+    const int nh = hash_cell(nx, ny, nz, mask);
+    bool dup = false;
+    for (int q = 0; q < nseen; ++q) dup |= nb[q] == nh;
+    if (!dup) nb[nseen++] = nh;
+  }
+  int w = 0;
+  for (int k = 0; k < nseen; ++k) {
+    const int s1 = offsets[nb[k]] + counts[nb[k]];
+    for (int slot = offsets[nb[k]]; slot < s1; ++slot) {
+      const double4 q = spos[slot];
+      if (!contact_pair(xi, yi, zi, ri, q.x, q.y, q.z, q.w, L, periodic)) continue;
+      const int j = sorted[slot];
+      if (j <= i) continue;
+      const int at = off + w++;
+      if (at < cap) {
+        pairs[2 * (size_t)at] = i;
+        pairs[2 * (size_t)at + 1] = j;
+      }
+    }
+  }
+  // in-segment insertion sort by j
+  const int lo = off, hi = min(off + cnt, cap);
+  for (int a = lo + 1; a < hi; ++a) {
+    const int v = pairs[2 * (size_t)a + 1];
+    int b = a - 1;
+    while (b >= lo && pairs[2 * (size_t)b + 1] > v) {
+      pairs[2 * (size_t)(b + 1) + 1] = pairs[2 * (size_t)b + 1];
+      --b;
+    }
+    pairs[2 * (size_t)(b + 1) + 1] = v;
+  }
+}
+
+// Every droplet's pairs at its offset in the lexicographic list; the total to
+// *count_out (block 0).
+__global__ __launch_bounds__(kThreads) void k_hash_emit(
+    int n, const int* __restrict__ pair_cnt, const int* __restrict__ pscan,
+    const int* __restrict__ ptile, int ntile, const int* __restrict__ stash,
+    const double* __restrict__ pos, const double* __restrict__ r, double L, int periodic,
+    int mask, const HashParams* __restrict__ hpp, const int* __restrict__ counts,
+    const int* __restrict__ offsets, const int* __restrict__ sorted,
+    const double4* __restrict__ spos, int* __restrict__ pairs, int cap,
+    int* __restrict__ count_out) {
+  extern __shared__ int pre[];
+  const int total = tile_prefix(ptile, ntile, pre);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count_out = total;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int cnt = pair_cnt[i];
+    if (cnt == 0) continue;
+    const int off = pscan[i] + pre[i / kTile];
+    if (cnt <= kStash) {
+      for (int a = 0; a < cnt; ++a) {
+        const int at = off + a;
+        if (at < cap) {
+          pairs[2 * (size_t)at] = i;
+          pairs[2 * (size_t)at + 1] = stash[(size_t)kStash * i + a];
         }
       }
-    }
-  }
-  if (!EMIT) {
-    for (int o = 1; o < SPLIT; o <<= 1) cnt += __shfl_xor(cnt, o);
-    if (sub == 0) pair_cnt[i] = cnt;
-  } else if (cnt > 1) {
-    // In-segment insertion sort by j (segments hold a handful of pairs).
-    const int lo = base, hi = min(base + cnt, cap);
-    for (int a = lo + 1; a < hi; ++a) {
-      int v = pairs[2 * (size_t)a + 1];
-      int b = a - 1;
-      while (b >= lo && pairs[2 * (size_t)b + 1] > v) {
-        pairs[2 * (size_t)(b + 1) + 1] = pairs[2 * (size_t)b + 1];
-        --b;
-      }
-      pairs[2 * (size_t)(b + 1) + 1] = v;
+    } else {
+      rescan_droplet(i, off, cnt, pos, r, L, periodic, mask, *hpp, counts, offsets, sorted, spos,
+                     pairs, cap);
     }
   }
 }
 
-__global__ void k_copy(const int* __restrict__ a, int* __restrict__ b, int n) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) b[i] = a[i];
-}
-
-__global__ void k_fill(int* p, int n, int v) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) p[i] = v;
+__global__ void k_zero_count(int* p) {
+  if (threadIdx.x == 0) *p = 0;
 }
 
 int check_hash_args(const double* pos, const double* radii, int n, int table_size,
@@ -262,6 +553,9 @@ int check_hash_args(const double* pos, const double* radii, int n, int table_siz
   if (n < 0) return dem::set_error(DEM_ERR_INVALID, "n must be >= 0");
   if (table_size < 1 || (table_size & (table_size - 1)))
     return dem::set_error(DEM_ERR_INVALID, "table_size must be a power of two (got %d)", table_size);
+  if (table_size > kMaxTiles * kTile || n > kMaxTiles * kTile)
+    return dem::set_error(DEM_ERR_INVALID, "n and table_size must be at most %d (got %d, %d)",
+                          kMaxTiles * kTile, n, table_size);
   if (!count_out) return dem::set_error(DEM_ERR_INVALID, "count_out is NULL");
   if (pair_capacity < 0 || (pair_capacity > 0 && !pairs))
     return dem::set_error(DEM_ERR_INVALID, "bad pair buffer");
@@ -271,14 +565,63 @@ int check_hash_args(const double* pos, const double* radii, int n, int table_siz
 
 int run_hash(const Layout& l, const double* pos, const double* radii, int n, double L, int periodic,
              int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out,
-             hipStream_t st);
+             hipStream_t st) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, st, count_out);
+    DEM_CHECK_LAUNCH("k_zero_count");
+    return DEM_OK;
+  }
+  const int T = table_size, mask = T - 1;
+  const int per = periodic ? 1 : 0;
+  const int nb = (n + kThreads - 1) / kThreads;
+  const int nrb = nb < kMaxRBlocks ? nb : kMaxRBlocks;
+  int prep_blocks = (T + kThreads - 1) / kThreads;
+  if (prep_blocks > 2048) prep_blocks = 2048;
+  if (prep_blocks < nrb) prep_blocks = nrb;
+  hipLaunchKernelGGL(k_hash_prep, dim3(prep_blocks), dim3(kThreads), 0, st, radii, n, nrb,
+                     l.counts, l.fill, T, l.rmax_part);
+  DEM_CHECK_LAUNCH("k_hash_prep");
+  hipLaunchKernelGGL(k_hash_assign, dim3(nb), dim3(kThreads), 0, st, pos, n, per, mask, L,
+                     (const double*)l.rmax_part, nrb, l.hp, l.bucket, l.counts);
+  DEM_CHECK_LAUNCH("k_hash_assign");
+  const int ntT = tiles(T);
+  hipLaunchKernelGGL(k_tile_scan, dim3(ntT), dim3(kThreads), 0, st, (const int*)l.counts, l.local,
+                     l.btile, T);
+  DEM_CHECK_LAUNCH("k_tile_scan<buckets>");
+  int sc_blocks = ((n > T ? n : T) + kThreads - 1) / kThreads;
+  if (sc_blocks > 4096) sc_blocks = 4096;
+  hipLaunchKernelGGL(k_hash_scatter, dim3(sc_blocks), dim3(kThreads), ntT * sizeof(int), st, n, T,
+                     (const HashParams*)l.hp, (const int*)l.bucket, pos, radii,
+                     (const int*)l.local, (const int*)l.btile, ntT, l.offsets, l.fill, l.sorted,
+                     l.spos, l.pair_cnt);
+  DEM_CHECK_LAUNCH("k_hash_scatter");
+  const int waves = (T + kBucketsPerWave - 1) / kBucketsPerWave;
+  const int se_blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
+  hipLaunchKernelGGL(k_hash_search, dim3(se_blocks), dim3(kThreads), 0, st, T, L, per, mask,
+                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
+                     (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt, l.stash);
+  DEM_CHECK_LAUNCH("k_hash_search");
+  const int ntn = tiles(n);
+  hipLaunchKernelGGL(k_tile_scan, dim3(ntn), dim3(kThreads), 0, st, (const int*)l.pair_cnt,
+                     l.pscan, l.ptile, n);
+  DEM_CHECK_LAUNCH("k_tile_scan<pairs>");
+  int em_blocks = nb < 2048 ? nb : 2048;
+  hipLaunchKernelGGL(k_hash_emit, dim3(em_blocks), dim3(kThreads), ntn * sizeof(int), st, n,
+                     (const int*)l.pair_cnt, (const int*)l.pscan, (const int*)l.ptile, ntn,
+                     (const int*)l.stash, pos, radii, L, per, mask, (const HashParams*)l.hp,
+                     (const int*)l.counts, (const int*)l.offsets, (const int*)l.sorted,
+                     (const double4*)l.spos, pairs, pair_capacity, count_out);
+  DEM_CHECK_LAUNCH("k_hash_emit");
+  return DEM_OK;
+}
 
 // Scratch of the allocation-free entry point: one array per GPU, held until
 // the process ends.  Host code never names the array in device work: its
 // address on the current device comes from hipGetSymbolAddress, once per
 // device, and is kept in g_scratch_base.  Calls that overlap on one GPU share
-// the array and race; every call rewrites what it reads (bucket counts are
-// cleared, cursors and pair counts are rewritten) before reading it.
+// the array and race; every call rewrites what it reads (bucket counters are
+// cleared, offsets, slots, pair counts and stashes are rewritten) before
+// reading it.
 static_assert(DEM_HASH_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
 static __device__ __align__(256) unsigned char scratch[DEM_HASH_NOALLOC_SCRATCH_BYTES];
 
@@ -345,54 +688,3 @@ int dem_hash_pairs_noalloc(const double* pos, const double* radii, int n, double
 }
 
 }  // extern "C"
-
-namespace {
-
-int run_hash(const Layout& l, const double* pos, const double* radii, int n, double L, int periodic,
-             int table_size, int32_t* pairs, int pair_capacity, int32_t* count_out,
-             hipStream_t st) {
-  if (n == 0) {
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, count_out, 1, 0);
-    DEM_CHECK_LAUNCH("k_fill");
-    return DEM_OK;
-  }
-  const int mask = table_size - 1;
-  const int T = table_size;
-  int nblk = (n + 255) / 256;
-  if (nblk > kMaxBlocks) nblk = kMaxBlocks;
-  hipLaunchKernelGGL(k_rmax_partial, dim3(nblk), dim3(256), 0, st, radii, n, l.rmax_part);
-  DEM_CHECK_LAUNCH("k_rmax_partial");
-  hipLaunchKernelGGL(k_hash_setup, dim3(1), dim3(64), 0, st, (const double*)l.rmax_part, nblk, L,
-                     l.hp);
-  DEM_CHECK_LAUNCH("k_hash_setup");
-  hipLaunchKernelGGL(k_fill, dim3((T + 255) / 256), dim3(256), 0, st, l.counts, T, 0);
-  DEM_CHECK_LAUNCH("k_fill");
-  const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_assign, dim3(nb), dim3(256), 0, st, pos, n, periodic ? 1 : 0, mask,
-                     (const HashParams*)l.hp, l.bucket, l.counts);
-  DEM_CHECK_LAUNCH("k_assign");
-  int rc = dem::exclusive_scan(l.counts, l.offsets, T, l.scan_ws, nullptr, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_copy, dim3((T + 255) / 256), dim3(256), 0, st, (const int*)l.offsets,
-                     l.cursor, T);
-  DEM_CHECK_LAUNCH("k_copy");
-  hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, st, n, (const HashParams*)l.hp,
-                     (const int*)l.bucket, pos, radii, l.cursor, l.sorted, l.spos);
-  DEM_CHECK_LAUNCH("k_scatter");
-  hipLaunchKernelGGL((k_pairs<false, kCountSplit>), dim3((int)(((long long)n * kCountSplit + 255) / 256)),
-                     dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
-                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
-                     (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
-                     (const int*)l.pair_off, pairs, pair_capacity);
-  DEM_CHECK_LAUNCH("k_pairs<count>");
-  rc = dem::exclusive_scan(l.pair_cnt, l.pair_off, n, l.scan_ws, count_out, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL((k_pairs<true, 1>), dim3(nb), dim3(256), 0, st, n, L, periodic ? 1 : 0, mask,
-                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
-                     (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt,
-                     (const int*)l.pair_off, pairs, pair_capacity);
-  DEM_CHECK_LAUNCH("k_pairs<emit>");
-  return DEM_OK;
-}
-
-}  // namespace

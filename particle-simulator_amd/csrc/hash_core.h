@@ -55,8 +55,13 @@ __device__ __forceinline__ bool contact_pair(double xi, double yi, double zi, do
   double d2 = dx * dx;
   d2 = d2 + dy * dy;
   d2 = d2 + dz * dz;
-  const double dist = sqrt(d2);
   const double rs = ri + rj;
+  // Far pairs skip the square root: with t = fl(rs * rs) (1 + 2^-40) >
+  // rs^2 (1 + 2^-41), d2 > t makes the exact sqrt(d2) exceed rs (1 + 2^-43),
+  // so its correctly rounded value (within 2^-53 relative) is above rs too:
+  // the same answer as the test below.
+  if (d2 > rs * rs * (1.0 + 0x1p-40)) return false;
+  const double dist = sqrt(d2);
   return dist <= rs;
 }
 

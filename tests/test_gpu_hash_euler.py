@@ -80,6 +80,38 @@ def test_dense_random_sets_bit_exact(gpu, periodic, seed):
     assert len(ref) > 50
 
 
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("capacity", [16, 200])
+def test_aliased_buckets_bit_exact(gpu, periodic, capacity):
+    """A table far smaller than the cell grid (32 or 512 buckets for ~8000
+    cells): every bucket holds droplets of many cells, whose neighbourhoods
+    differ, and neighbour cells alias onto shared buckets."""
+    rng = np.random.default_rng(40 + capacity)
+    n, L = 3000, 1e-3
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(5e-6, 25e-6, size=n)
+    det = SpatialHashCollisionDetector(num_particles=capacity, L=L,
+                                       boundary_mode="periodic" if periodic else "open")
+    is_coll, pairs = det.detect(pos, r)
+    ok, ref = oracle.hash_pairs(pos, r, L, periodic, _next_power_of_2(2 * capacity))
+    assert is_coll is ok and np.array_equal(pairs, ref) and len(ref) > 50
+
+
+def test_crowded_droplets_beyond_the_stash_bit_exact(gpu):
+    """Droplets with many partners (more than the search keeps per droplet)
+    are rescanned by the emit step: same sorted segments."""
+    rng = np.random.default_rng(8)
+    n, L = 800, 1.2e-4
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(6e-6, 14e-6, size=n)
+    det = SpatialHashCollisionDetector(num_particles=n, L=L, boundary_mode="periodic")
+    _, pairs = det.detect(pos, r)
+    _, ref = oracle.hash_pairs(pos, r, L, True, _next_power_of_2(2 * n))
+    per_target = np.bincount(ref[:, 0], minlength=n)
+    assert per_target.max() > 16 and (per_target > 16).sum() > 10
+    assert np.array_equal(pairs, ref)
+
+
 def test_periodic_miss_reproduced(gpu):
     r = np.full(2, 1 / 9)
     pos = np.array([[0.01, 0.5, 0.5], [0.87, 0.5, 0.5]])
