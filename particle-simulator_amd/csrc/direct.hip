@@ -269,14 +269,16 @@ __global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict
   dem::force_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
+// Four lanes per target (the reversed partials are most of the bytes).
 __global__ __launch_bounds__(256) void k_force_sym_reduce(
     const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
     const double* __restrict__ radii, const DevScales* __restrict__ scp, double m_const,
     double* __restrict__ F, RecC* __restrict__ recc) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
+  if (i >= n) return;  // the four lanes of a target leave together
   double sum[3] = {0.0, 0.0, 0.0};
-  dem::sym_sum(part, S, n, part_j, B, i, sum);
+  dem::sym_sum4(part, S, n, part_j, B, i, sum);
+  if (threadIdx.x & 3) return;
   const DevScales sc = *scp;
   double Fi[3];
   float fr[3];
@@ -358,11 +360,12 @@ __global__ __launch_bounds__(256) void k_conv_sym_reduce(
     const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
     const DevScales* __restrict__ scp, double eta_const, const double* __restrict__ F,
     double* __restrict__ V) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
+  if (i >= n) return;  // the four lanes of a target leave together
   double m[3] = {0.0, 0.0, 0.0};
   const double c[3] = {0.0, 0.0, 0.0};
-  dem::sym_sum(part, S, n, part_j, B, i, m);
+  dem::sym_sum4(part, S, n, part_j, B, i, m);
+  if (threadIdx.x & 3) return;
   const DevScales sc = *scp;
   const dem::CorrScalars cs{};
   double Vi[3];
@@ -580,7 +583,7 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
                                        (const Rec*)l.rec, n, B, ec, choose_tile_force(), l.part,
                                        l.part_j));
     DEM_CHECK_LAUNCH("k_force_sym");
-    hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const double*)l.part_j, B, radii,
                        (const DevScales*)l.sc, ph->m_const, F, l.recc);
     DEM_CHECK_LAUNCH("k_force_sym_reduce");
@@ -619,7 +622,7 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
                                        (const RecC*)l.recc, n, B, ec, choose_tile_conv(),
                                        l.part, l.part_j));
     DEM_CHECK_LAUNCH("k_conv_sym");
-    hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const double*)l.part_j, B,
                        (const DevScales*)l.sc, ph->eta_const, F, V);
     DEM_CHECK_LAUNCH("k_conv_sym_reduce");

@@ -329,24 +329,54 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
   }
 }
 
-// Sum of a target's pair-once partials: its own chunks, then the reversed
-// pairs from the blocks b = bi - e (mod B), e = 1, 2, ..., that have block bi
-// as their e-th partner (e <= P_b).
+// Reversed-pair sums of target i from the blocks b = bi - e (mod B) that
+// have block bi as their e-th partner (e <= P_b), over e = k + 1, k + 5, ...
+// (one of four interleaved streams, k = 0..3), in ascending e.
+__device__ __forceinline__ void sym_reversed(const double* __restrict__ part_j, int B, int i, int k,
+                                             double* r) {
+  const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
+  r[0] = r[1] = r[2] = 0.0;
+#pragma unroll 2
+  for (int e = k + 1; e <= Pmax; e += 4) {
+    const int b = bi >= e ? bi - e : bi - e + B;
+    if (e > sym_partners(b, B)) continue;
+    const double* q = part_j + ((size_t)b * Pmax + (e - 1)) * 3 * kSymB + il;
+    r[0] += q[0];
+    r[1] += q[kSymB];
+    r[2] += q[2 * kSymB];
+  }
+}
+
+// Sum of a target's pair-once partials: its own S chunks in order, plus the
+// reversed pairs as ((r_0 + r_1) + (r_2 + r_3)) of the four streams of
+// sym_reversed.  The four-lane reduce kernels form the same sums in the same
+// order, so every caller gets identical bits.
 __device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, int n,
                                         const double* __restrict__ part_j, int B, int i,
                                         double* sum) {
   for (int s = 0; s < S; ++s)
     for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
-  const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
-#pragma unroll 4
-  for (int e = 1; e <= Pmax; ++e) {
-    const int b = bi >= e ? bi - e : bi - e + B;
-    if (e > sym_partners(b, B)) continue;
-    const double* q = part_j + ((size_t)b * Pmax + (e - 1)) * 3 * kSymB + il;
-    sum[0] += q[0];
-    sum[1] += q[kSymB];
-    sum[2] += q[2 * kSymB];
+  double r[4][3];
+  for (int k = 0; k < 4; ++k) sym_reversed(part_j, B, i, k, r[k]);
+  for (int a = 0; a < 3; ++a) sum[a] += (r[0][a] + r[1][a]) + (r[2][a] + r[3][a]);
+}
+
+// sym_sum with the four streams on four consecutive lanes (lane k = lane & 3);
+// the result is valid on lane k = 0.
+__device__ __forceinline__ void sym_sum4(const double* __restrict__ part, int S, int n,
+                                         const double* __restrict__ part_j, int B, int i,
+                                         double* sum) {
+  const int k = threadIdx.x & 3;
+  double r[3];
+  sym_reversed(part_j, B, i, k, r);
+  for (int a = 0; a < 3; ++a) {
+    r[a] += __shfl_xor(r[a], 1);  // lanes 0, 1: r_0 + r_1; lanes 2, 3: r_2 + r_3
+    r[a] += __shfl_xor(r[a], 2);  // lane 0: (r_0 + r_1) + (r_2 + r_3)
   }
+  if (k != 0) return;
+  for (int s = 0; s < S; ++s)
+    for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
+  for (int a = 0; a < 3; ++a) sum[a] += r[a];
 }
 
 }  // namespace dem
