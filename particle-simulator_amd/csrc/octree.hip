@@ -37,6 +37,7 @@
 #include "profile.h"
 #include "scan.h"
 #include "sort.h"
+#include "tile_scan.h"
 
 namespace {
 
@@ -174,6 +175,17 @@ struct Scales {
   int periodic, pad;
 };
 
+// State of the level-by-level split kept on the device, so that levels run
+// back to back without a host round trip per level.
+struct LevelRec {
+  int node_count, parent_count;  // after the levels split so far
+  int ls, le;                    // node range of the next level to examine
+  int stop;                      // 1: a level split nothing; later levels do nothing
+  int num_levels, max_depth;
+  int pad;
+  int level_offsets[DEM_TREE_MAX_LEVELS];
+};
+
 // Workspace layout.
 struct TreeLayout {
   Scales* sc;
@@ -200,6 +212,7 @@ struct TreeLayout {
   NodeC* nc;                     // [M]
   NodeF* nf;                     // [M]
   int* counter;                  // [16]
+  struct LevelRec* lrec;         // the level loop's device-side record, two slots
   // particles (N = capacity)
   unsigned long long* keys;      // [N]
   unsigned long long* keys_alt;  // [N]
@@ -210,6 +223,7 @@ struct TreeLayout {
   double* srad;                  // [N]
   RecF* srec;                    // [N + 1]
   double* pforces;               // [3N] forces of the last force walk (original order)
+  double4* packed;               // [N] (x, y, z, r) in original order (build input copy)
   int* scan_ws;
   int* sort_ws;
   size_t bytes;
@@ -243,6 +257,7 @@ TreeLayout carve(void* ws, int ncap, int M) {
   l.nc = c.take<NodeC>(m);
   l.nf = c.take<NodeF>(m);
   l.counter = c.take<int>(16);
+  l.lrec = c.take<LevelRec>(2);
   l.keys = c.take<unsigned long long>(N);
   l.keys_alt = c.take<unsigned long long>(N);
   l.perm = c.take<int>(N);
@@ -252,6 +267,7 @@ TreeLayout carve(void* ws, int ncap, int M) {
   l.srad = c.take<double>(N);
   l.srec = c.take<RecF>(N + 1);  // + one spare record (leaf loops read members in twos)
   l.pforces = c.take<double>(3 * N);
+  l.packed = c.take<double4>(N);
   const size_t big = N > m ? N : m;
   l.scan_ws = c.take<int>(dem::scan_workspace_ints((int)big));
   l.sort_ws = c.take<int>(dem::radix_sort_workspace_ints((int)N));
@@ -280,13 +296,16 @@ __global__ void k_fill_d(double* p, size_t n, double v) {
 
 // Octant path of every particle: replays the reference's bound recursion
 // from the root [0, L]^3 (flat_tree.py:250-259, :338-349).
-__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ pos, int n, double L,
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ pos,
+                                              const double* __restrict__ radii, int n, double L,
                                               unsigned long long* __restrict__ keys,
-                                              int* __restrict__ perm) {
+                                              int* __restrict__ perm, double4* __restrict__ packed) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const double x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1], z = pos[3 * (size_t)i + 2];
+  // one aligned 32-byte record per droplet: the sorted gather reads one line
+  packed[i] = make_double4(x, y, z, radii[i]);
   double mnx = 0.0, mny = 0.0, mnz = 0.0, mxx = L, mxy = L, mxz = L;
   unsigned long long key = 0ull;
   for (int l = 0; l < kKeyLevels; ++l) {
@@ -306,9 +325,16 @@ __device__ __forceinline__ int digit_at(unsigned long long key, int level) {
   return (int)((key >> (3 * (kKeyLevels - level))) & 7ull);
 }
 
-__global__ void k_root(double L, int n, double* bmin, double* bmax, int* first_child, int* next,
-                       int* leaf_start, int* count, int* mstart) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The root node [0, L]^3 holding all n particles, and the level loop's start.
+__device__ void root_init(double L, int n, double* bmin, double* bmax, int* first_child, int* next,
+                          int* leaf_start, int* count, int* mstart, int* counter, LevelRec* rec) {
+  counter[1] = 0;  // leaf slots (atomic total)
+  counter[2] = 0;  // split candidates of level 0 (atomic total; levels alternate 2, 3)
+  LevelRec r{};
+  r.node_count = 1;
+  r.ls = 0;
+  r.le = 1;
+  *rec = r;
   bmin[0] = bmin[1] = bmin[2] = 0.0;
   bmax[0] = bmax[1] = bmax[2] = L;
   first_child[0] = -1;
@@ -319,74 +345,145 @@ __global__ void k_root(double L, int n, double* bmin, double* bmax, int* first_c
 }
 
 // Split candidates of one level: leaves with count > mpl (depth-limited by the key).
-__global__ void k_level_flags(int ls, int le, int mpl, int level, const int* __restrict__ count,
-                              int* __restrict__ flag) {
-  int k = blockIdx.x * 256 + threadIdx.x;
-  int node = ls + k;
-  if (node >= le) return;
-  flag[k] = (level < kKeyLevels && count[node] > mpl) ? 1 : 0;
+__device__ __forceinline__ bool split_candidate(int level, int mpl, int count) {
+  return level < kKeyLevels && count > mpl;
 }
 
-// Split the first `budget` candidates (in node order) of the level.
-__global__ void k_level_split(int ls, int le, int level, int budget, int node_base, int pc_base,
-                              const int* __restrict__ flag, const int* __restrict__ rank,
-                              const unsigned long long* __restrict__ keys, double* bmin,
-                              double* bmax, int* first_child, int* next, int* leaf_start,
-                              int* count, int* mstart, int* parents, int* pfc) {
+// Tile scan of the level's candidate flags: local ranks, tile sums and the
+// candidate total (*cand, cleared beforehand).  The level's node range comes
+// from its device record; a stopped loop scans nothing.
+__global__ __launch_bounds__(dem::kScanThreads) void k_level_scan(
+    int mpl, int level, const LevelRec* __restrict__ rec, const int* __restrict__ count,
+    int* __restrict__ rank, int* __restrict__ tile_sums, int* __restrict__ cand) {
+  if (rec->stop) return;
+  const int ls = rec->ls, cnt = rec->le - rec->ls;
+  if ((int)blockIdx.x >= dem::scan_tiles(cnt)) return;
+  dem::ts_tile_scan([=](int k) { return split_candidate(level, mpl, count[ls + k]) ? 1 : 0; },
+                    cnt, rank, tile_sums, cand);
+}
+
+// Split the first k candidates (in node order) of the level, k the capacity
+// clamp of the candidates (node_count + 8 must stay within max_nodes,
+// flat_tree.py:325-327); eight lanes per node: lane `oct` finds where its
+// octant starts in the node's sorted range (a binary search on the next
+// octant digit) and writes that child; its end is the next lane's start.  The
+// candidate rank is the tile-local rank plus the prefix of the tile sums.
+// Every block derives k and the next level's record from this level's record
+// (read only); block 0 writes the next record (the other slot) and clears the
+// next level's candidate total.  A level that splits nothing stops the loop,
+// as the reference's does.
+__global__ __launch_bounds__(256) void k_level_split(
+    int level, int mpl, int max_nodes, const LevelRec* __restrict__ rec,
+    LevelRec* __restrict__ rec_next, const int* __restrict__ cand, int* __restrict__ cand_next,
+    const int* __restrict__ rank_local, const int* __restrict__ tile_sums,
+    const unsigned long long* __restrict__ keys, double* bmin, double* bmax, int* first_child,
+    int* next, int* leaf_start, int* count, int* mstart, int* parents, int* pfc) {
 #pragma clang fp contract(off)
-  int k = blockIdx.x * 256 + threadIdx.x;
-  int node = ls + k;
-  if (node >= le || !flag[k] || rank[k] >= budget) return;
-  const int q = rank[k];
+  extern __shared__ int pre[];
+  const LevelRec r = *rec;
+  int budget = 0;
+  if (!r.stop) {
+    const int c = *cand;
+    const int room = (max_nodes - r.node_count) / 8;
+    budget = c < room ? c : (room > 0 ? room : 0);
+    if (r.le <= r.ls || r.num_levels >= DEM_TREE_MAX_LEVELS) budget = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    LevelRec nx = r;
+    if (r.stop || budget == 0) {
+      nx.stop = 1;
+    } else {
+      nx.level_offsets[nx.num_levels++] = r.parent_count;
+      nx.parent_count = r.parent_count + budget;
+      nx.ls = r.le;
+      nx.le = r.node_count + 8 * budget;
+      nx.node_count = nx.le;
+      nx.max_depth = level + 1;
+    }
+    *rec_next = nx;
+    *cand_next = 0;
+  }
+  if (budget == 0) return;  // uniform
+  const int ls = r.ls, le = r.le;
+  const int node_base = r.node_count, pc_base = r.parent_count;
+  const int cnt = le - ls;
+  if ((int)blockIdx.x * 32 >= cnt) return;  // 32 nodes per block; uniform per block
+  dem::ts_tile_prefix(tile_sums, dem::scan_tiles(cnt), pre);
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int kk = gid >> 3, oct = gid & 7;
+  const int node = ls + kk;
+  bool work = node < le && split_candidate(level, mpl, count[node]);
+  int q = 0;
+  if (work) {
+    q = rank_local[kk] + pre[kk / dem::kScanTile];
+    work = q < budget;
+  }
+  int start = 0, s1 = 0;
+  if (work) {
+    const int s0 = mstart[node];
+    s1 = s0 + count[node];
+    // first position in [s0, s1) whose digit is >= oct
+    int a = s0, b = s1;
+    if (oct > 0)
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (digit_at(keys[mid], level + 1) < oct) a = mid + 1; else b = mid;
+      }
+    start = a;
+  }
+  // the lanes of a node are eight consecutive lanes of one wave
+  const int nxt = __shfl_down(start, 1, 8);
+  if (!work) return;
+  const int end = oct == 7 ? s1 : nxt;
   const int base = node_base + 8 * q;
-  first_child[node] = base;
-  parents[pc_base + q] = node;
-  pfc[pc_base + q] = base;
+  if (oct == 0) {
+    first_child[node] = base;
+    parents[pc_base + q] = node;
+    pfc[pc_base + q] = base;
+  }
   const double mnx = bmin[3 * node], mny = bmin[3 * node + 1], mnz = bmin[3 * node + 2];
   const double mxx = bmax[3 * node], mxy = bmax[3 * node + 1], mxz = bmax[3 * node + 2];
   const double cx = (mnx + mxx) * 0.5, cy = (mny + mxy) * 0.5, cz = (mnz + mxz) * 0.5;
-  const int s0 = mstart[node], s1 = s0 + count[node];
-  int lo = s0;
-  for (int oct = 0; oct < 8; ++oct) {
-    // first position in [lo, s1) whose digit exceeds `oct`
-    int a = lo, b = s1;
-    while (a < b) {
-      int mid = (a + b) >> 1;
-      if (digit_at(keys[mid], level + 1) <= oct) a = mid + 1; else b = mid;
-    }
-    const int child = base + oct;
-    bmin[3 * child] = (oct & 1) ? cx : mnx;
-    bmin[3 * child + 1] = (oct & 2) ? cy : mny;
-    bmin[3 * child + 2] = (oct & 4) ? cz : mnz;
-    bmax[3 * child] = (oct & 1) ? mxx : cx;
-    bmax[3 * child + 1] = (oct & 2) ? mxy : cy;
-    bmax[3 * child + 2] = (oct & 4) ? mxz : cz;
-    first_child[child] = -1;
-    next[child] = -1;
-    leaf_start[child] = -1;
-    count[child] = a - lo;
-    mstart[child] = lo;
-    lo = a;
-  }
+  const int child = base + oct;
+  bmin[3 * child] = (oct & 1) ? cx : mnx;
+  bmin[3 * child + 1] = (oct & 2) ? cy : mny;
+  bmin[3 * child + 2] = (oct & 4) ? cz : mnz;
+  bmax[3 * child] = (oct & 1) ? mxx : cx;
+  bmax[3 * child + 1] = (oct & 2) ? mxy : cy;
+  bmax[3 * child + 2] = (oct & 4) ? mxz : cz;
+  first_child[child] = -1;
+  next[child] = -1;
+  leaf_start[child] = -1;
+  count[child] = end - start;
+  mstart[child] = start;
 }
 
-__global__ void k_leaf_counts(int nn, const int* __restrict__ first_child,
-                              const int* __restrict__ count, int* __restrict__ leafcnt) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nn) return;
-  leafcnt[i] = (first_child[i] < 0 && count[i] > 0) ? count[i] : 0;
+__device__ __forceinline__ int leaf_slots(int first_child, int count) {
+  return (first_child < 0 && count > 0) ? count : 0;
 }
 
-// leaf_start from the exclusive scan of leaf counts (flat_tree.py:370-385),
+// Tile scan of the leaves' particle counts; the total (counter[1], cleared by
+// k_root) is the number of leaf slots.
+__global__ __launch_bounds__(dem::kScanThreads) void k_leaf_scan(
+    int nn, const int* __restrict__ first_child, const int* __restrict__ count,
+    int* __restrict__ off_local, int* __restrict__ tile_sums, int* __restrict__ counter) {
+  dem::ts_tile_scan([=](int i) { return leaf_slots(first_child[i], count[i]); }, nn, off_local,
+                    tile_sums, counter + 1);
+}
+
+// leaf_start from the scan of leaf counts (flat_tree.py:370-385),
 // leaf_indices filled in sorted order (:387-397).
-__global__ void k_leaf_fill(int nn, const int* __restrict__ first_child,
-                            const int* __restrict__ count, const int* __restrict__ mstart,
-                            const int* __restrict__ off, const int* __restrict__ perm,
-                            int* __restrict__ leaf_start, int* __restrict__ leaf_indices) {
+__global__ __launch_bounds__(256) void k_leaf_fill(
+    int nn, const int* __restrict__ first_child, const int* __restrict__ count,
+    const int* __restrict__ mstart, const int* __restrict__ off_local,
+    const int* __restrict__ tile_sums, int nt, const int* __restrict__ perm,
+    int* __restrict__ leaf_start, int* __restrict__ leaf_indices) {
+  extern __shared__ int pre[];
+  dem::ts_tile_prefix(tile_sums, nt, pre);
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
   if (first_child[i] < 0 && count[i] > 0) {
-    const int ls = off[i];
+    const int ls = off_local[i] + pre[i / dem::kScanTile];
     leaf_start[i] = ls;
     const int m0 = mstart[i];
     for (int k = 0; k < count[i]; ++k) leaf_indices[ls + k] = perm[m0 + k];
@@ -395,44 +492,59 @@ __global__ void k_leaf_fill(int nn, const int* __restrict__ first_child,
   }
 }
 
-// Sibling chain of the children of one parent level (pass 1, :399-415).
-__global__ void k_next_pass1(int p0, int p1, const int* __restrict__ parents,
-                             const int* __restrict__ pfc, int* next) {
-  int p = p0 + blockIdx.x * 256 + threadIdx.x;
-  if (p >= p1) return;
-  const int base = pfc[p];
-  const int pn = next[parents[p]];
-  for (int oct = 0; oct < 8; ++oct) next[base + oct] = (oct == 7) ? pn : base + oct + 1;
+// Next pointers and first children (flat_tree.py:399-463) in one launch.  The
+// reference sets them in two top-down sweeps: pass 1 chains each parent's
+// eight children (child 7 takes the parent's next), pass 2 relinks the
+// occupied children (each to the next occupied sibling, the last to the
+// parent's next) and points the parent at its first occupied child.  Both
+// have closed forms, evaluated here per node by walking up the tree: children
+// of the P-th parent (global parent index) are nodes 1 + 8P .. 8 + 8P.
+//   pass-1 next of x: x + 1 if x is not its parent's last child, else the
+//     pass-1 next of the parent (root: -1);
+//   final next of an occupied x: its next occupied sibling, else the final
+//     next of its (occupied) parent (root: -1); an empty node keeps its
+//     pass-1 next;
+//   first child of a split node: its first occupied child (an empty split
+//     node, possible only with mpl < 1: its pass-1 next).
+__device__ __forceinline__ int next_pass1(int x, const int* __restrict__ parents) {
+  while (x != 0) {
+    const int P = (x - 1) >> 3;
+    if (((x - 1) & 7) < 7) return x + 1;
+    x = parents[P];
+  }
+  return -1;
 }
 
-// Unlink empty children (pass 2, :417-446).
-__global__ void k_next_pass2(int p0, int p1, const int* __restrict__ parents,
-                             const int* __restrict__ pfc, const int* __restrict__ count, int* next,
-                             int* first_child) {
-  int p = p0 + blockIdx.x * 256 + threadIdx.x;
-  if (p >= p1) return;
-  const int parent = parents[p];
-  const int base = pfc[p];
-  if (count[parent] == 0) {
-    first_child[parent] = next[parent];
+__device__ __forceinline__ int next_occupied(int x, const int* __restrict__ parents,
+                                             const int* __restrict__ count) {
+  while (x != 0) {
+    const int P = (x - 1) >> 3;
+    const int base = 1 + 8 * P;
+    for (int s = ((x - 1) & 7) + 1; s < 8; ++s)
+      if (count[base + s] > 0) return base + s;
+    x = parents[P];
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(256) void k_next_links(int nn, const int* __restrict__ parents,
+                                                    const int* __restrict__ count,
+                                                    int* __restrict__ first_child,
+                                                    int* __restrict__ next) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nn) return;
+  if (c != 0) next[c] = count[c] > 0 ? next_occupied(c, parents, count) : next_pass1(c, parents);
+  const int base = first_child[c];
+  if (base < 0) return;  // a leaf
+  if (count[c] == 0) {
+    first_child[c] = c == 0 ? -1 : next_pass1(c, parents);
     return;
   }
-  int first = -1, last = -1;
-  for (int j = 0; j < 8; ++j) {
-    const int c = base + j;
-    if (count[c] > 0) {
-      if (first < 0) {
-        first = c;
-      } else {
-        next[last] = c;
-      }
-      last = c;
+  for (int j = 0; j < 8; ++j)
+    if (count[base + j] > 0) {
+      first_child[c] = base + j;
+      return;
     }
-  }
-  if (last >= 0) {
-    next[last] = next[parent];
-    first_child[parent] = first;
-  }
 }
 
 // Leaf R^3 aggregates, zeroing every node first (:469-511).
@@ -566,9 +678,13 @@ __global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ pos, in
 }
 
 // Scales of the fixed-point / work units (same scheme as the direct kernel).
+// Scales of the fixed-point records, and the root node (root_init).
 __global__ void k_scales(const double* __restrict__ bbox, int nblk, double L, int periodic,
-                         double m_const, Scales* __restrict__ sc) {
+                         double m_const, Scales* __restrict__ sc, int n, double* bmin,
+                         double* bmax, int* first_child, int* next, int* leaf_start, int* count,
+                         int* mstart, int* counter, LevelRec* rec) {
   if (threadIdx.x != 0) return;
+  root_init(L, n, bmin, bmax, first_child, next, leaf_start, count, mstart, counter, rec);
   Scales s;
   s.periodic = periodic;
   s.pad = 0;
@@ -611,21 +727,21 @@ __device__ __forceinline__ uint32_t quantize(double x, int a, const Scales& sc) 
 
 // Sorted particle data: float64 positions / radii and fixed-point records.
 __global__ __launch_bounds__(256) void k_sorted_particles(
-    const double* __restrict__ pos, const double* __restrict__ radii, int n,
-    const int* __restrict__ perm, const Scales* __restrict__ scp, double* __restrict__ spos,
-    double* __restrict__ srad, RecF* __restrict__ srec) {
+    const double4* __restrict__ packed, int n, const int* __restrict__ perm,
+    const Scales* __restrict__ scp, double* __restrict__ spos, double* __restrict__ srad,
+    RecF* __restrict__ srec) {
   int p = blockIdx.x * 256 + threadIdx.x;
   if (p == 0) srec[n] = RecF{};  // the spare record after the last
   if (p >= n) return;
   const Scales sc = *scp;
-  const int o = perm[p];
+  const double4 d = packed[perm[p]];
+  const double xs[3] = {d.x, d.y, d.z};
   uint32_t q[3];
   for (int a = 0; a < 3; ++a) {
-    const double x = pos[3 * (size_t)o + a];
-    spos[3 * (size_t)p + a] = x;
-    q[a] = quantize(x, a, sc);
+    spos[3 * (size_t)p + a] = xs[a];
+    q[a] = quantize(xs[a], a, sc);
   }
-  const double r = radii[o];
+  const double r = d.w;
   srad[p] = r;
   const double rl = r / sc.lambda;
   RecF rr;
@@ -1256,80 +1372,107 @@ int dem_octree_build(const double* pos, const double* radii, int n, double L, in
     DEM_CHECK_LAUNCH("k_bbox");
   }
   hipLaunchKernelGGL(k_scales, dim3(1), dim3(64), 0, st, (const double*)l.bbox, nblk, L,
-                     periodic ? 1 : 0, m_const, l.sc);
+                     periodic ? 1 : 0, m_const, l.sc, n, l.bmin, l.bmax, l.first_child, l.next,
+                     l.leaf_start, l.count, l.mstart, l.counter, l.lrec);
   DEM_CHECK_LAUNCH("k_scales");
+  // Keys sorted first by their top kPartialSortBits bits: that orders every
+  // node of levels <= kPartialLevels as the full key does (a node's range,
+  // and its children's by the next octant digit).  Only a tree that splits
+  // deeper takes the remaining passes (a full stable sort of the partially
+  // sorted keys, which keeps those ranges).
+  constexpr int kPartialSortBits = 32;  // four 8-bit digits: bits 32..63
+  // the key has 3 kKeyLevels = 63 bits, so bits 32..62 hold levels 1..10
+  constexpr int kPartialLevels = (3 * kKeyLevels - (64 - kPartialSortBits)) / 3;
+  static_assert(kPartialLevels == 10, "levels ordered by the first sort");
+  bool full_sort = false;
+  unsigned long long* skeys = l.keys;  // the buffer holding the sorted keys
   if (n > 0) {
-    hipLaunchKernelGGL(k_keys, dim3(nb256(n)), dim3(256), 0, st, pos, n, L, l.keys, l.perm);
+    hipLaunchKernelGGL(k_keys, dim3(nb256(n)), dim3(256), 0, st, pos, radii, n, L, l.keys, l.perm,
+                       l.packed);
     DEM_CHECK_LAUNCH("k_keys");
-    int rc = dem::radix_sort_u64(l.keys, l.perm, l.keys_alt, l.perm_alt, n, 3 * kKeyLevels,
-                                 l.sort_ws, st);
+    int rc = dem::radix_sort_u64(l.keys, l.perm, l.keys_alt, l.perm_alt, n,
+                                 64 - kPartialSortBits, 64, l.sort_ws, st, &skeys);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sorted_particles, dim3(nb256(n)), dim3(256), 0, st, pos, radii, n,
-                       (const int*)l.perm, (const Scales*)l.sc, l.spos, l.srad, l.srec);
+  }
+
+  // level-by-level split (flat_tree.py:209-236).  Each level is two launches
+  // — candidate scan, split — working from a device record of the loop state
+  // (two slots: level `level` reads slot level & 1, its split writes the
+  // other); the host launches them for an upper bound of the level's size
+  // (8 x the previous bound) and reads the record only every kLevelsPerSync
+  // levels to stop (a tree of N = 2^20 with leaves of 16 is 7 levels deep).
+  constexpr int kLevelsPerSync = 8;
+  int* tile_sums = l.flag;  // the flags' storage holds the scan-tile sums
+  LevelRec rec{};
+  int launched = 0;         // levels launched
+  bool rec_read = false;    // rec holds the record the launched levels left
+  long long bound = 1;      // nodes the level can hold at most
+  for (int level = 0; level < kMaxLevels; ++level) {
+    if (level > 0 && (level % kLevelsPerSync == 0 || level == kPartialLevels)) {
+      DEM_CHECK_HIP(hipMemcpyAsync(&rec, l.lrec + (level & 1), sizeof(LevelRec),
+                                   hipMemcpyDeviceToHost, st));
+      DEM_CHECK_HIP(hipStreamSynchronize(st));
+      rec_read = true;
+      if (rec.stop) break;
+    }
+    rec_read = false;
+    if (level == kPartialLevels && !full_sort && n > 1) {
+      // the next split reads octant digits below the sorted bits
+      int rc = dem::radix_sort_u64(skeys, l.perm, skeys == l.keys ? l.keys_alt : l.keys,
+                                   l.perm_alt, n, 0, 64, l.sort_ws, st, &skeys);
+      if (rc) return rc;
+      full_sort = true;
+    }
+    const int cb = (int)(bound < M ? bound : M);
+    const int nt = dem::scan_tiles(cb);
+    const int cur = level & 1, nxt = cur ^ 1;
+    hipLaunchKernelGGL(k_level_scan, dim3(nt), dim3(dem::kScanThreads), 0, st, mpl, level,
+                       (const LevelRec*)(l.lrec + cur), (const int*)l.count, l.rank, tile_sums,
+                       l.counter + 2 + cur);
+    DEM_CHECK_LAUNCH("k_level_scan");
+    hipLaunchKernelGGL(k_level_split, dim3((cb + 31) / 32), dim3(256), nt * sizeof(int), st, level,
+                       mpl, M, (const LevelRec*)(l.lrec + cur), l.lrec + nxt,
+                       (const int*)(l.counter + 2 + cur), l.counter + 2 + nxt,
+                       (const int*)l.rank, (const int*)tile_sums,
+                       (const unsigned long long*)skeys, l.bmin, l.bmax, l.first_child, l.next,
+                       l.leaf_start, l.count, l.mstart, l.parents, l.pfc);
+    DEM_CHECK_LAUNCH("k_level_split");
+    launched = level + 1;
+    bound = bound * 8 < M ? bound * 8 : M;
+  }
+  if (!rec_read) {
+    DEM_CHECK_HIP(hipMemcpyAsync(&rec, l.lrec + (launched & 1), sizeof(LevelRec),
+                                 hipMemcpyDeviceToHost, st));
+    DEM_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  info->node_count = rec.node_count;
+  info->parent_count = rec.parent_count;
+  info->num_levels = rec.num_levels;
+  info->max_depth = rec.max_depth;
+  for (int k = 0; k < rec.num_levels; ++k) info->level_offsets[k] = rec.level_offsets[k];
+  const int nn = info->node_count;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_sorted_particles, dim3(nb256(n)), dim3(256), 0, st,
+                       (const double4*)l.packed, n, (const int*)l.perm, (const Scales*)l.sc,
+                       l.spos, l.srad, l.srec);
     DEM_CHECK_LAUNCH("k_sorted_particles");
   }
-  hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, L, n, l.bmin, l.bmax, l.first_child,
-                     l.next, l.leaf_start, l.count, l.mstart);
-  DEM_CHECK_LAUNCH("k_root");
-
-  // level-by-level split (flat_tree.py:209-236)
-  int ls = 0, le = 1;
-  for (int level = 0; level < kMaxLevels; ++level) {
-    const int cnt = le - ls;
-    if (cnt <= 0) break;
-    hipLaunchKernelGGL(k_level_flags, dim3(nb256(cnt)), dim3(256), 0, st, ls, le, mpl, level,
-                       (const int*)l.count, l.flag);
-    DEM_CHECK_LAUNCH("k_level_flags");
-    int rc = dem::exclusive_scan(l.flag, l.rank, cnt, l.scan_ws, l.counter, st);
-    if (rc) return rc;
-    int cand = 0;
-    DEM_CHECK_HIP(hipMemcpyAsync(&cand, l.counter, sizeof(int), hipMemcpyDeviceToHost, st));
-    DEM_CHECK_HIP(hipStreamSynchronize(st));
-    const int room = (M - info->node_count) / 8;
-    const int k = cand < room ? cand : (room > 0 ? room : 0);
-    if (k <= 0) break;  // no split at this level: the reference's overflow flag stays 0
-    hipLaunchKernelGGL(k_level_split, dim3(nb256(cnt)), dim3(256), 0, st, ls, le, level, k,
-                       info->node_count, info->parent_count, (const int*)l.flag,
-                       (const int*)l.rank, (const unsigned long long*)l.keys, l.bmin, l.bmax,
-                       l.first_child, l.next, l.leaf_start, l.count, l.mstart, l.parents, l.pfc);
-    DEM_CHECK_LAUNCH("k_level_split");
-    if (info->num_levels >= DEM_TREE_MAX_LEVELS)
-      return dem::set_error(DEM_ERR_CAPACITY, "too many tree levels");
-    info->level_offsets[info->num_levels++] = info->parent_count;
-    info->parent_count += k;
-    ls = le;
-    le = info->node_count + 8 * k;
-    info->node_count = le;
-    info->max_depth = level + 1;
-  }
-  const int nn = info->node_count;
 
   // leaf slots (flat_tree.py:370-397)
-  hipLaunchKernelGGL(k_leaf_counts, dim3(nb256(nn)), dim3(256), 0, st, nn,
-                     (const int*)l.first_child, (const int*)l.count, l.leafcnt);
-  DEM_CHECK_LAUNCH("k_leaf_counts");
-  int rc = dem::exclusive_scan(l.leafcnt, l.rank, nn, l.scan_ws, l.counter + 1, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_leaf_fill, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.first_child,
-                     (const int*)l.count, (const int*)l.mstart, (const int*)l.rank,
-                     (const int*)l.perm, l.leaf_start, l.leaf_indices);
+  const int ntl = dem::scan_tiles(nn);
+  hipLaunchKernelGGL(k_leaf_scan, dim3(ntl), dim3(dem::kScanThreads), 0, st, nn,
+                     (const int*)l.first_child, (const int*)l.count, l.rank, tile_sums, l.counter);
+  DEM_CHECK_LAUNCH("k_leaf_scan");
+  hipLaunchKernelGGL(k_leaf_fill, dim3(nb256(nn)), dim3(256), ntl * sizeof(int), st, nn,
+                     (const int*)l.first_child, (const int*)l.count, (const int*)l.mstart,
+                     (const int*)l.rank, (const int*)tile_sums, ntl, (const int*)l.perm,
+                     l.leaf_start, l.leaf_indices);
   DEM_CHECK_LAUNCH("k_leaf_fill");
 
-  // next pointers, two top-down passes (flat_tree.py:448-463)
-  for (int pass = 0; pass < 2; ++pass)
-    for (int lev = 0; lev < info->num_levels; ++lev) {
-      const int p0 = info->level_offsets[lev];
-      const int p1 = lev + 1 < info->num_levels ? info->level_offsets[lev + 1] : info->parent_count;
-      if (p1 <= p0) continue;
-      if (pass == 0)
-        hipLaunchKernelGGL(k_next_pass1, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
-                           (const int*)l.parents, (const int*)l.pfc, l.next);
-      else
-        hipLaunchKernelGGL(k_next_pass2, dim3(nb256(p1 - p0)), dim3(256), 0, st, p0, p1,
-                           (const int*)l.parents, (const int*)l.pfc, (const int*)l.count, l.next,
-                           l.first_child);
-      DEM_CHECK_LAUNCH("k_next_pass");
-    }
+  // next pointers and first children (flat_tree.py:399-463)
+  hipLaunchKernelGGL(k_next_links, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.parents,
+                     (const int*)l.count, l.first_child, l.next);
+  DEM_CHECK_LAUNCH("k_next_links");
 
   // R^3 aggregates (flat_tree.py:544-554)
   hipLaunchKernelGGL(k_leaf_r3, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.first_child,

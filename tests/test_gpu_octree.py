@@ -185,6 +185,34 @@ def test_structure_identical_to_oracle(gpu, mpl, periodic, max_nodes, seed):
         assert set(li[s: s + c]) == set(ref.arr["leaf_indices"][s: s + c])
 
 
+def test_deep_tree_structure_identical_to_oracle(gpu):
+    """Clusters much smaller than the box: the tree splits past level 10, so
+    the build sorts the keys' low bits too (the first sort orders only the
+    top 32 bits)."""
+    rng = np.random.default_rng(17)
+    L = 1e-3
+    c1 = rng.uniform(0, L, 3)
+    pos = np.vstack([rng.uniform(0, L, size=(1500, 3)),
+                     c1 + rng.normal(scale=L * 2.0**-16, size=(800, 3)),
+                     rng.uniform(0.3 * L, 0.3 * L + L * 2.0**-13, size=(700, 3))]) % L
+    r = rng.uniform(2e-9, 8e-9, size=len(pos))
+    for mpl, periodic in ((1, True), (4, False)):
+        tree = FlatOctree(theta=0.5, mpl=mpl, num_particles=len(pos))
+        tree.build(pos, r, L, periodic)
+        ref = oracle.OracleTree(pos, r, L, periodic, mpl=mpl, max_nodes=tree.max_nodes)
+        assert len(ref.level_offsets) > 11, "the case should split past level 10"
+        g = _structure(tree)
+        nc = ref.node_count
+        assert int(tree.node_count[None]) == nc
+        assert tree._parent_level_offsets == ref.level_offsets
+        for k in ("count", "first_child", "next", "leaf_start"):
+            assert np.array_equal(g[k], ref.arr[k][:nc]), k
+        li = tree.leaf_indices.to_numpy()
+        for i in np.nonzero((g["first_child"] < 0) & (g["count"] > 0))[0]:
+            s0, c = g["leaf_start"][i], g["count"][i]
+            assert set(li[s0: s0 + c]) == set(ref.arr["leaf_indices"][s0: s0 + c])
+
+
 # --- walk parity with the oracle
 
 
