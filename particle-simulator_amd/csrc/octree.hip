@@ -547,17 +547,30 @@ __global__ __launch_bounds__(256) void k_next_links(int nn, const int* __restric
     }
 }
 
-// Leaf R^3 aggregates, zeroing every node first (:469-511).
-__global__ void k_leaf_r3(int nn, const int* __restrict__ first_child,
-                          const int* __restrict__ count, const int* __restrict__ mstart,
-                          const double* __restrict__ spos, const double* __restrict__ srad,
-                          double* r3sum, double* r3c, double* maxr) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nn) return;
+// Sum of a value over four consecutive lanes in a fixed order,
+// ((v0 + v1) + (v2 + v3)), valid on the first of the four.
+__device__ __forceinline__ double quad_sum(double v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  return v;
+}
+
+// Leaf R^3 aggregates, zeroing every node first (:469-511).  Four lanes per
+// node: lane k takes the leaf's particles k, k + 4, ...; the lanes' sums are
+// combined in a fixed order.
+__global__ __launch_bounds__(256) void k_leaf_r3(int nn, const int* __restrict__ first_child,
+                                                 const int* __restrict__ count,
+                                                 const int* __restrict__ mstart,
+                                                 const double* __restrict__ spos,
+                                                 const double* __restrict__ srad, double* r3sum,
+                                                 double* r3c, double* maxr) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int i = g >> 2, k0 = g & 3;
+  if (i >= nn) return;  // the four lanes of a node leave together
   double s = 0.0, wx = 0.0, wy = 0.0, wz = 0.0, mr = 0.0;
   if (first_child[i] < 0 && count[i] > 0) {
     const int m0 = mstart[i];
-    for (int k = 0; k < count[i]; ++k) {
+    for (int k = k0; k < count[i]; k += 4) {
       const int p = m0 + k;
       const double r = srad[p];
       const double r3 = r * r * r;
@@ -568,6 +581,13 @@ __global__ void k_leaf_r3(int nn, const int* __restrict__ first_child,
       if (r > mr) mr = r;
     }
   }
+  s = quad_sum(s);
+  wx = quad_sum(wx);
+  wy = quad_sum(wy);
+  wz = quad_sum(wz);
+  mr = fmax(mr, __shfl_xor(mr, 1));
+  mr = fmax(mr, __shfl_xor(mr, 2));
+  if (k0) return;
   r3sum[i] = s;
   r3c[3 * i] = wx;
   r3c[3 * i + 1] = wy;
@@ -600,17 +620,22 @@ __global__ void k_sweep_r3(int p0, int p1, const int* __restrict__ parents,
   maxr[parent] = mr;
 }
 
-// Leaf force sums and raw moments P_ab = sum x_a F_b (:556-626, :696-766).
-__global__ void k_leaf_forces(int nn, const int* __restrict__ first_child,
-                              const int* __restrict__ count, const int* __restrict__ mstart,
-                              const int* __restrict__ perm, const double* __restrict__ spos,
-                              const double* __restrict__ F, double* fsum, double* fmom) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nn) return;
+// Leaf force sums and raw moments P_ab = sum x_a F_b (:556-626, :696-766);
+// four lanes per node as in k_leaf_r3.
+__global__ __launch_bounds__(256) void k_leaf_forces(int nn, const int* __restrict__ first_child,
+                                                     const int* __restrict__ count,
+                                                     const int* __restrict__ mstart,
+                                                     const int* __restrict__ perm,
+                                                     const double* __restrict__ spos,
+                                                     const double* __restrict__ F, double* fsum,
+                                                     double* fmom) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int i = g >> 2, k0 = g & 3;
+  if (i >= nn) return;  // the four lanes of a node leave together
   double f[3] = {0.0, 0.0, 0.0}, P[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (first_child[i] < 0 && count[i] > 0) {
     const int m0 = mstart[i];
-    for (int k = 0; k < count[i]; ++k) {
+    for (int k = k0; k < count[i]; k += 4) {
       const int p = m0 + k;
       const int o = perm[p];
       const double Fx = F[3 * (size_t)o], Fy = F[3 * (size_t)o + 1], Fz = F[3 * (size_t)o + 2];
@@ -623,6 +648,9 @@ __global__ void k_leaf_forces(int nn, const int* __restrict__ first_child,
       P[6] += rz * Fx; P[7] += rz * Fy; P[8] += rz * Fz;
     }
   }
+  for (int a = 0; a < 3; ++a) f[a] = quad_sum(f[a]);
+  for (int a = 0; a < 9; ++a) P[a] = quad_sum(P[a]);
+  if (k0) return;
   for (int a = 0; a < 3; ++a) fsum[3 * (size_t)i + a] = f[a];
   for (int a = 0; a < 9; ++a) fmom[9 * (size_t)i + a] = P[a];
 }
@@ -1251,7 +1279,7 @@ int check_info(const dem_tree_info* info, const void* ws, size_t ws_bytes, TreeL
 int propagate_forces(const dem_tree_info* info, const TreeLayout& l, const double* F,
                      hipStream_t st) {
   const int nn = info->node_count;
-  hipLaunchKernelGGL(k_leaf_forces, dim3(nb256(nn)), dim3(256), 0, st, nn,
+  hipLaunchKernelGGL(k_leaf_forces, dim3(nb256(4 * nn)), dim3(256), 0, st, nn,
                      (const int*)l.first_child, (const int*)l.count, (const int*)l.mstart,
                      (const int*)l.perm, (const double*)l.spos, F, l.fsum, l.fmom);
   DEM_CHECK_LAUNCH("k_leaf_forces");
@@ -1475,7 +1503,7 @@ int dem_octree_build(const double* pos, const double* radii, int n, double L, in
   DEM_CHECK_LAUNCH("k_next_links");
 
   // R^3 aggregates (flat_tree.py:544-554)
-  hipLaunchKernelGGL(k_leaf_r3, dim3(nb256(nn)), dim3(256), 0, st, nn, (const int*)l.first_child,
+  hipLaunchKernelGGL(k_leaf_r3, dim3(nb256(4 * nn)), dim3(256), 0, st, nn, (const int*)l.first_child,
                      (const int*)l.count, (const int*)l.mstart, (const double*)l.spos,
                      (const double*)l.srad, l.r3sum, l.r3c, l.maxr);
   DEM_CHECK_LAUNCH("k_leaf_r3");
