@@ -426,6 +426,17 @@ int dem_batch_merge_async(dem_batch_t b, int count, const int32_t* systems,
                           const int32_t* merged_off, const double* merged,
                           dem_stream_t stream);
 
+/* on = 1 declares that every later dem_batch_merge / dem_batch_merge_async
+ * applies, for each listed system, the greedy merge of the complete pair list
+ * of that system's last check (as dem_merge_plan computes it; the reference's
+ * merge, droplet_solution.py:208-262).  A re-check of such a system may then
+ * search only around the merged droplets: no pair of two survivors is left,
+ * so when the hash cell is unchanged the re-check's pairs are exactly those
+ * with a merged droplet, found without searching the survivors (same pair
+ * lists as a full re-check).  0 (the default) makes every re-check a full one.
+ * No reference counterpart (scheduling). */
+int dem_batch_set_incremental(dem_batch_t b, int on);
+
 /* Makes `stream` wait until the last dem_batch_advance has decided which
  * systems it moves; from then on the systems it leaves alone (those found in
  * contact) may be merged and re-checked on `stream` while it runs.  No
@@ -460,8 +471,9 @@ int dem_batch_capture(dem_batch_t b);
 /* `steps` x (check, advance) without merging: systems with a contact stay. */
 int dem_batch_run(dem_batch_t b, int steps, dem_stream_t stream);
 
-/* Device address and size of "active", "stepped", "pair_count", "steps_done"
- * (int32 per system), "pair_base" (int32, n_ens + 1), "time" (float64 per
+/* Device address and size of "active", "stepped", "pair_count", "steps_done",
+ * "incremental" (int32 per system: 1 when the last re-check searched only
+ * around the merged droplets), "pair_base" (int32, n_ens + 1), "time" (float64 per
  * system: sum of dt over advanced steps), "pairs" (int32 (i, j), local
  * indices), "pair_values" (float64 (x, y, z, r) x 2 per pair). */
 int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes);

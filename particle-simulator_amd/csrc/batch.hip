@@ -66,6 +66,8 @@ constexpr int kRow = 256;       // droplets per row
 constexpr int kSearchSplit = 4;  // lanes per droplet in the contact count pass
 constexpr int kScanWG = 1024;   // threads of a per-system scan workgroup
 constexpr int kZeroBlocks = 1024;
+constexpr int kIncCap = 1024;   // pairs an incremental re-check holds per system
+constexpr int kIncFresh = 64;   // merged droplets an incremental re-check takes per system
 
 inline int split_max(int n) {  // same cap as the single-system kernels
   long long cap = (1ll << 21) / (n > 0 ? n : 1);
@@ -124,6 +126,9 @@ struct BEns {
   int n, cap, S;        // S: source splits, or chunks of a block's partner list
   int jchunk;           // sources per split, or partner entries per chunk
   int steps, T;
+  int fresh;  // droplets appended by the caller's last merge (a complete greedy
+              // plan of the last check's pairs), 0 when unknown
+  int pad_;
   double L, dt, sf, m_const, eta_const;
 };
 
@@ -133,6 +138,7 @@ struct BDev {
   double cell;  // 2 max r
   int g;        // cells per axis
   int skip;     // cell <= 0: no contacts
+  double cell_checked;  // cell of the last check
 };
 
 struct BHead {
@@ -165,6 +171,10 @@ struct BLayout {
   int2* pairs;   // [pair_cap] packed system after system
   double4* pvals;// [2 pair_cap] (x, y, z, r) of i and of j
   int* pair_base;// [n_ens + 1] exclusive scan of pair_count
+  int* inc;      // [n_ens] 1: this re-check searches only around the merged droplets
+  int* oob;      // [n_ens] 1: a droplet lies outside [0, L]^3
+  int* merged;   // [n_ens] 1: merged on the device since the last check
+  int2* inc_pairs;  // [n_ens kIncCap] sorted pairs of the incremental re-checks
   int* stage_i;  // merge lists uploaded by dem_batch_merge
   double* stage_d;
   BEns* ens_stage;  // [n_ens] updated descriptors of the systems just merged
@@ -249,6 +259,10 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   l.pairs = c.take<int2>(at_least1(p.pair_total));
   l.pvals = c.take<double4>(2 * at_least1(p.pair_total));
   l.pair_base = c.take<int>(ne + 1);
+  l.inc = c.take<int>(ne);
+  l.oob = c.take<int>(ne);
+  l.merged = c.take<int>(ne);
+  l.inc_pairs = c.take<int2>(ne * kIncCap);
   l.stage_i = c.take<int>(at_least1(p.cap_total) + 3 * ne + 3);
   l.stage_d = c.take<double>(4 * at_least1(p.cap_total));
   l.ens_stage = c.take<BEns>(ne);
@@ -267,10 +281,16 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
 __global__ void k_b_init(const BEns* __restrict__ ens, BDev* __restrict__ dev, int n_ens,
                          int* __restrict__ active, int* __restrict__ stepped,
                          int* __restrict__ go, int* __restrict__ pair_count,
-                         int* __restrict__ steps_done, double* __restrict__ time) {
+                         int* __restrict__ steps_done, double* __restrict__ time,
+                         int* __restrict__ inc, int* __restrict__ oob,
+                         int* __restrict__ merged) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_ens) return;
   dev[e].sc = dem::periodic_scales(ens[e].L, ens[e].m_const);
+  dev[e].cell_checked = -1.0;
+  inc[e] = 0;
+  oob[e] = 0;
+  merged[e] = 0;
   active[e] = 0;
   stepped[e] = 0;
   go[e] = 0;
@@ -417,10 +437,11 @@ __global__ __launch_bounds__(256) void k_b_compact_gather(
 __global__ __launch_bounds__(256) void k_b_compact_scatter(
     const BEns* __restrict__ ens, int c, int rows_max, const int* __restrict__ stage_i,
     const double* __restrict__ stage_d, const double4* __restrict__ scratch,
-    double* __restrict__ pos, double* __restrict__ radii) {
+    double* __restrict__ pos, double* __restrict__ radii, int* __restrict__ merged) {
   const int k = (int)blockIdx.x / rows_max, r = (int)blockIdx.x % rows_max;
   if (k >= c) return;
   const int e = stage_i[k];
+  if (r == 0 && threadIdx.x == 0) merged[e] = 1;
   const int* roff = stage_i + c;
   const int* moff = stage_i + 2 * c + 1;
   const int m = roff[k + 1] - roff[k];
@@ -456,16 +477,45 @@ __device__ __forceinline__ bool eligible(const BEns& E, const int* steps_done,
 // a global step) clears "stepped" and checks every system with steps left; a
 // re-check only covers the systems the previous check found in contact (the
 // caller has merged them since); the others keep pair_count = 0.
-__global__ void k_b_begin(const BEns* __restrict__ ens, const BHead* __restrict__ head,
-                          int new_step, const int* __restrict__ steps_done,
-                          int* __restrict__ stepped, int* __restrict__ active,
-                          int* __restrict__ pair_count) {
+//
+// A re-check may be incremental (inc): after a greedy merge of the complete
+// pair list every pair of that list lost a droplet (a pair whose droplets
+// were both free when it was visited merged them), and survivors keep their
+// positions, radii and relative order.  If the hash cell is unchanged too
+// (so are g and every bucket), the survivors' mutual pairs are those of the
+// last check without a removed droplet: none.  The re-check's pairs then all
+// involve a merged droplet, which the caller appended (the last `fresh`
+// indices), and k_b_inc_search finds exactly those.  (`merged`, set by the
+// compaction and cleared here, says that the merge came after the last check.)  Otherwise, and when the
+// last cell is too narrow for k_b_inc_search's neighbourhood bound, the
+// re-check is a full one.
+__global__ void k_b_begin(const BEns* __restrict__ ens, BDev* __restrict__ dev,
+                          const BHead* __restrict__ head, int new_step,
+                          const int* __restrict__ steps_done, int* __restrict__ stepped,
+                          int* __restrict__ active, int* __restrict__ pair_count,
+                          int* __restrict__ inc, int* __restrict__ oob,
+                          int* __restrict__ merged) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= head->n_ens) return;
+  const bool was_merged = merged[e] != 0;
+  merged[e] = 0;
   if (new_step) stepped[e] = 0;
+  const BEns& E = ens[e];
   const bool want = new_step ? true : pair_count[e] > 0;
-  active[e] = (want && eligible(ens[e], steps_done, stepped, e)) ? 1 : 0;
+  const bool act = want && eligible(E, steps_done, stepped, e);
+  active[e] = act ? 1 : 0;
   pair_count[e] = 0;
+  oob[e] = 0;
+  bool incremental = false;
+  if (act) {
+    BDev& D = dev[e];
+    const double w = E.L / D.cell - (double)(D.g - 1);  // width of the last cell, in cells
+    incremental = !new_step && was_merged && E.fresh > 0 && E.fresh <= kIncFresh &&
+                  E.fresh <= E.n &&
+                  !D.skip && D.cell == D.cell_checked && (D.g <= 5 || w >= 1e-3);
+    D.cell_checked = D.cell;
+  }
+  inc[e] = incremental ? 1 : 0;
 }
 
 __global__ void k_b_zero(int* __restrict__ p, long long count) {
@@ -477,6 +527,12 @@ __global__ void k_b_zero(int* __restrict__ p, long long count) {
 __device__ __forceinline__ bool hashed(const BHead* head, const int* active, const BDev* dev,
                                        int e) {
   return head->contacts && active[e] && !dev[e].skip;
+}
+
+// Systems the full search covers: hashed and not re-checked incrementally.
+__device__ __forceinline__ bool searched(const BHead* head, const int* active, const BDev* dev,
+                                         const int* inc, int e) {
+  return hashed(head, active, dev, e) && !inc[e];
 }
 
 __device__ __forceinline__ int bucket_of(const double* p, const BDev& D, int mask) {
@@ -492,7 +548,7 @@ __global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
                                                    const int* __restrict__ active,
                                                    const double* __restrict__ pos,
                                                    int* __restrict__ cnt, int* __restrict__ bucket,
-                                                   int* __restrict__ rank) {
+                                                   int* __restrict__ rank, int* __restrict__ oob) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
   if (!hashed(head, active, dev, row.x)) return;
@@ -500,7 +556,11 @@ __global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
   const int i = row.y + threadIdx.x;
   if (i >= E.n) return;
   const size_t g = (size_t)E.off + i;
-  const int h = bucket_of(pos + 3 * g, dev[row.x], E.T - 1);
+  const double* p = pos + 3 * g;
+  // (NaN fails the test too)
+  if (!(p[0] >= 0.0 && p[0] <= E.L && p[1] >= 0.0 && p[1] <= E.L && p[2] >= 0.0 && p[2] <= E.L))
+    oob[row.x] = 1;
+  const int h = bucket_of(p, dev[row.x], E.T - 1);
   bucket[g] = h;
   rank[g] = atomicAdd(&cnt[E.cnt_off + h], 1);
 }
@@ -510,8 +570,10 @@ __global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
 // thread t takes values 4t .. 4t + 3 of a tile (consecutive threads read
 // consecutive 16-byte runs), a wave scan and a scan of the wave totals give
 // its offset, and the tile total carries into the next tile.
-__device__ void block_scan(const int* __restrict__ in, int* __restrict__ out, int len,
-                           bool total) {
+// With `clear`, every input value is set to 0 once read (in and clear are
+// then the same array, hence no __restrict__ on either).
+__device__ void block_scan(const int* in, int* __restrict__ out, int len, bool total,
+                           int* clear = nullptr) {
   constexpr int kW = kScanWG / 64;
   __shared__ int s_w[kW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -521,6 +583,11 @@ __device__ void block_scan(const int* __restrict__ in, int* __restrict__ out, in
     int x[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) x[k] = b + k < len ? in[b + k] : 0;
+    if (clear) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (b + k < len) clear[b + k] = 0;
+    }
     const int local = x[0] + x[1] + x[2] + x[3];
     int v = local;
 #pragma unroll
@@ -556,12 +623,12 @@ __global__ __launch_bounds__(kScanWG) void k_b_bucket_scan(const BEns* __restric
                                                            const BDev* __restrict__ dev,
                                                            const BHead* __restrict__ head,
                                                            const int* __restrict__ active,
-                                                           const int* __restrict__ cnt,
-                                                           int* __restrict__ start) {
+                                                           int* cnt, int* __restrict__ start) {
   const int e = blockIdx.x;
   if (e >= head->n_ens || !hashed(head, active, dev, e)) return;
   const BEns& E = ens[e];
-  block_scan(cnt + E.cnt_off, start + E.start_off, E.T, true);  // T + 1 slots per system
+  // T + 1 slots per system; the counts are left at 0 for the next check's atomics
+  block_scan(cnt + E.cnt_off, start + E.start_off, E.T, true, cnt + E.cnt_off);
 }
 
 __global__ __launch_bounds__(kRow) void k_b_scatter(const BEns* __restrict__ ens,
@@ -726,6 +793,7 @@ __global__ __launch_bounds__(kRow) void k_b_search(const BEns* __restrict__ ens,
                                                    const int2* __restrict__ rows,
                                                    const BHead* __restrict__ head,
                                                    const int* __restrict__ active,
+                                                   const int* __restrict__ inc,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ sorted,
                                                    const double4* __restrict__ spos,
@@ -735,7 +803,7 @@ __global__ __launch_bounds__(kRow) void k_b_search(const BEns* __restrict__ ens,
   const int rb = (int)blockIdx.x / kSearchSplit, part = (int)blockIdx.x % kSearchSplit;
   if (rb >= head->n_rows) return;
   const int2 row = rows[rb];
-  if (!hashed(head, active, dev, row.x)) return;  // uniform over the block
+  if (!searched(head, active, dev, inc, row.x)) return;  // uniform over the block
   const BEns& E = ens[row.x];
   const int sub = threadIdx.x % kSearchSplit;
   const int slot = row.y + part * (kRow / kSearchSplit) + (int)threadIdx.x / kSearchSplit;
@@ -762,12 +830,200 @@ __global__ __launch_bounds__(kRow) void k_b_search(const BEns* __restrict__ ens,
 __global__ __launch_bounds__(kScanWG) void k_b_pair_scan(const BEns* __restrict__ ens,
                                                          const BHead* __restrict__ head,
                                                          const int* __restrict__ pair_count,
+                                                         const int* __restrict__ inc,
                                                          const int* __restrict__ pc,
                                                          int* __restrict__ poff) {
   const int e = blockIdx.x;
-  if (e >= head->n_ens || pair_count[e] == 0) return;
+  if (e >= head->n_ens || pair_count[e] == 0 || inc[e]) return;
   const BEns& E = ens[e];
   block_scan(pc + E.off, poff + E.off, E.n, false);
+}
+
+// ---------------------------------------------------------------- incremental re-check
+
+__device__ __forceinline__ int b_wave_inclusive(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Whether bucket h is one of the buckets of the 27 cells around cell
+// (cx, cy, cz) (periodic): h in B(i) for a droplet i of that cell.
+__device__ __forceinline__ bool in_neighbourhood(int cx, int cy, int cz, int g, int mask, int h) {
+  bool hit = false;
+  for (int k = 0; k < 27; ++k) {
+    const int nx = dem::nb_coord(cx, k / 9 - 1, g, 1);
+    const int ny = dem::nb_coord(cy, (k / 3) % 3 - 1, g, 1);
+    const int nz = dem::nb_coord(cz, k % 3 - 1, g, 1);
+    hit |= dem::hash_cell(nx, ny, nz, mask) == h;
+  }
+  return hit;
+}
+
+struct IncLds {
+  unsigned long long key[kIncCap];  // (i << 32) | j of each pair found
+  int nb[4][128];                   // per wave: buckets of the 125 cells around m
+  int pre[4][128];                  // per wave: candidates in those buckets before each
+  int first[4][128];                // per wave: first slot of each bucket
+  int count;
+};
+
+// Pairs of an incremental re-check (one workgroup per system; k_b_begin has
+// set inc).  The reference's pair set is {(i, j): i < j, contact(i, j),
+// bucket(j) in B(i)}; with no pair among the n_s = n - fresh survivors left,
+// it is
+//   (a) (a, m), a < n_s <= m: survivor a, merged m, contact and bucket(m) in
+//       B(a).  Contact needs |d| <= r_a + r_m <= cell per axis (up to
+//       rounding), so along each axis a's cell lies within two cells of m's on
+//       the periodic ring (floor(fl(x/cell)) of the two positions differ by at
+//       most 2 directly, and across the seam by g - 2 .. g + 1 when the last
+//       cell is at least 1e-3 cells wide, k_b_begin's condition; with g <= 5
+//       the five offsets cover the ring).  The buckets of those 125 cells,
+//       each once, hold every such a.
+//   (b) (m1, m2), both merged: every one of the fresh^2 / 2 pairs is tested.
+// contact_pair is symmetric in its two droplets (negated differences, the same
+// squares and sum of radii), so a pair is tested as in the full search.  The
+// pairs are sorted lexicographically into inc_pairs[e].  More than kIncCap of
+// them, or a droplet outside [0, L]^3 (the ring bound assumes wrapped
+// positions), clears inc: the full search then runs for the system.
+__global__ __launch_bounds__(256) void k_b_inc_search(
+    const BEns* __restrict__ ens, const BDev* __restrict__ dev, const BHead* __restrict__ head,
+    const int* __restrict__ active, int* __restrict__ inc, const int* __restrict__ oob,
+    const double* __restrict__ pos, const double* __restrict__ radii,
+    const int* __restrict__ start, const int* __restrict__ sorted,
+    const double4* __restrict__ spos, int* __restrict__ pair_count,
+    int2* __restrict__ inc_pairs) {
+  __shared__ IncLds S;
+  const int e = blockIdx.x;
+  if (e >= head->n_ens || !inc[e]) return;  // uniform over the block
+  if (!hashed(head, active, dev, e) || oob[e]) {
+    if (threadIdx.x == 0) inc[e] = 0;
+    return;
+  }
+  const BEns& E = ens[e];
+  const BDev D = dev[e];
+  const int n = E.n, k = E.fresh, ns = n - k, g = D.g, mask = E.T - 1;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int* st = start + E.start_off;
+  const int* so = sorted + E.off;
+  const double4* sp = spos + E.off;
+  const double* P = pos + 3 * (size_t)E.off;
+  const double* R = radii + E.off;
+  if (threadIdx.x == 0) S.count = 0;
+  __syncthreads();
+  // (a): wave w takes merged droplets ns + w, ns + w + 4, ... (a uniform loop)
+  for (int m = ns + w; m < n; m += 4) {
+    const double xm = P[3 * (size_t)m], ym = P[3 * (size_t)m + 1], zm = P[3 * (size_t)m + 2];
+    const double rm = R[m];
+    const int mx = dem::cell_of(xm, D.cell, g, 1);
+    const int my = dem::cell_of(ym, D.cell, g, 1);
+    const int mz = dem::cell_of(zm, D.cell, g, 1);
+    const int bm = dem::hash_cell(mx, my, mz, mask);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int o = lane + 64 * p;
+      int h = -1;
+      if (o < 125)
+        h = dem::hash_cell(dem::pymod(mx + o / 25 - 2, g), dem::pymod(my + (o / 5) % 5 - 2, g),
+                           dem::pymod(mz + o % 5 - 2, g), mask);
+      S.nb[w][o] = h;
+    }
+    dem::wave_sync();
+    int len[2], fst[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int o = lane + 64 * p;
+      const int h = S.nb[w][o];
+      bool dup = h < 0;
+      for (int q = 0; q < o; ++q) dup |= S.nb[w][q] == h;
+      fst[p] = dup ? 0 : st[h];
+      len[p] = dup ? 0 : st[h + 1] - fst[p];
+    }
+    const int inc0 = b_wave_inclusive(len[0]);
+    const int inc1 = b_wave_inclusive(len[1]) + __shfl(inc0, 63, 64);
+    const int M = __shfl(inc1, 63, 64);
+    S.pre[w][lane] = inc0 - len[0];
+    S.pre[w][lane + 64] = inc1 - len[1];
+    S.first[w][lane] = fst[0];
+    S.first[w][lane + 64] = fst[1];
+    dem::wave_sync();
+    for (int c0 = 0; c0 < M; c0 += 64) {
+      const int c = c0 + lane;
+      if (c >= M) continue;
+      int lo = 0, hi = 127;  // the last bucket whose candidates start at or before c
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.pre[w][mid] <= c) lo = mid;
+        else hi = mid - 1;
+      }
+      const int slot = S.first[w][lo] + (c - S.pre[w][lo]);
+      const int a = so[slot];
+      if (a >= ns) continue;
+      const double4 q = sp[slot];
+      if (!dem::contact_pair(q.x, q.y, q.z, q.w, xm, ym, zm, rm, E.L, 1)) continue;
+      if (!in_neighbourhood(dem::cell_of(q.x, D.cell, g, 1), dem::cell_of(q.y, D.cell, g, 1),
+                            dem::cell_of(q.z, D.cell, g, 1), g, mask, bm))
+        continue;
+      const int at = atomicAdd(&S.count, 1);
+      if (at < kIncCap) S.key[at] = ((unsigned long long)a << 32) | (unsigned)m;
+    }
+    dem::wave_sync();  // nb, pre and first are rewritten for the next droplet
+  }
+  // (b)
+  for (int idx = threadIdx.x; idx < k * k; idx += 256) {
+    const int m1 = ns + idx / k, m2 = ns + idx % k;
+    if (m1 >= m2) continue;
+    const double x1 = P[3 * (size_t)m1], y1 = P[3 * (size_t)m1 + 1], z1 = P[3 * (size_t)m1 + 2];
+    const double x2 = P[3 * (size_t)m2], y2 = P[3 * (size_t)m2 + 1], z2 = P[3 * (size_t)m2 + 2];
+    if (!dem::contact_pair(x1, y1, z1, R[m1], x2, y2, z2, R[m2], E.L, 1)) continue;
+    const int h2 = dem::hash_cell(dem::cell_of(x2, D.cell, g, 1), dem::cell_of(y2, D.cell, g, 1),
+                                  dem::cell_of(z2, D.cell, g, 1), mask);
+    if (!in_neighbourhood(dem::cell_of(x1, D.cell, g, 1), dem::cell_of(y1, D.cell, g, 1),
+                          dem::cell_of(z1, D.cell, g, 1), g, mask, h2))
+      continue;
+    const int at = atomicAdd(&S.count, 1);
+    if (at < kIncCap) S.key[at] = ((unsigned long long)m1 << 32) | (unsigned)m2;
+  }
+  __syncthreads();
+  const int cnt = S.count;
+  if (cnt > kIncCap) {
+    if (threadIdx.x == 0) inc[e] = 0;
+    return;
+  }
+  // rank sort (the keys are distinct: every pair is found once)
+  for (int t = threadIdx.x; t < cnt; t += 256) {
+    const unsigned long long v = S.key[t];
+    int r = 0;
+    for (int u = 0; u < cnt; ++u) r += S.key[u] < v ? 1 : 0;
+    inc_pairs[(size_t)e * kIncCap + r] = make_int2((int)(v >> 32), (int)(v & 0xffffffffull));
+  }
+  if (threadIdx.x == 0) pair_count[e] = cnt;
+}
+
+// The incremental re-checks' pairs at their systems' places in the packed
+// buffer, with the droplets' values (as k_b_emit writes them).
+__global__ __launch_bounds__(256) void k_b_inc_emit(
+    const BEns* __restrict__ ens, const BHead* __restrict__ head, const int* __restrict__ inc,
+    const int* __restrict__ pair_count, const int* __restrict__ pair_base,
+    const int2* __restrict__ inc_pairs, const double* __restrict__ pos,
+    const double* __restrict__ radii, int2* __restrict__ pairs, double4* __restrict__ pvals) {
+  const int e = blockIdx.x;
+  if (e >= head->n_ens || !inc[e]) return;
+  const BEns& E = ens[e];
+  const int cnt = pair_count[e];
+  for (int t = threadIdx.x; t < cnt; t += 256) {
+    const long long at = (long long)pair_base[e] + t;
+    if (at >= head->pair_cap) break;
+    const int2 p = inc_pairs[(size_t)e * kIncCap + t];
+    pairs[at] = p;
+    const size_t gi = (size_t)E.off + p.x, gj = (size_t)E.off + p.y;
+    pvals[2 * at] = make_double4(pos[3 * gi], pos[3 * gi + 1], pos[3 * gi + 2], radii[gi]);
+    pvals[2 * at + 1] = make_double4(pos[3 * gj], pos[3 * gj + 1], pos[3 * gj + 2], radii[gj]);
+  }
 }
 
 // Where each system's pairs start in the packed pair buffer.
@@ -782,6 +1038,7 @@ __global__ __launch_bounds__(kRow) void k_b_emit(const BEns* __restrict__ ens,
                                                  const int2* __restrict__ rows,
                                                  const BHead* __restrict__ head,
                                                  const int* __restrict__ pair_count,
+                                                 const int* __restrict__ inc,
                                                  const double* __restrict__ pos,
                                                  const double* __restrict__ radii,
                                                  const int* __restrict__ start,
@@ -794,7 +1051,7 @@ __global__ __launch_bounds__(kRow) void k_b_emit(const BEns* __restrict__ ens,
                                                  double4* __restrict__ pvals) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (pair_count[row.x] == 0) return;
+  if (pair_count[row.x] == 0 || inc[row.x]) return;
   const BEns& E = ens[row.x];
   const int i = row.y + threadIdx.x;
   if (i >= E.n) return;
@@ -1018,6 +1275,8 @@ struct dem_batch {
   std::vector<int> stage_h;
   std::vector<BEns> ens_stage_h;
   int cap_max = 0;            // largest capacity (rows of k_b_pack_list)
+  std::vector<int> fresh;     // droplets appended by each system's last merge (BEns::fresh)
+  bool incremental = false;   // merges are complete greedy plans: re-checks may be incremental
   bool tables_dirty = false;  // merged systems' items / rows not rebuilt yet
   hipEvent_t go_done = nullptr;  // recorded after the "go" kernel of an advance
   hipStream_t cap_stream = nullptr;
@@ -1087,6 +1346,7 @@ BEns ens_entry(const dem_batch* b, int e) {
   }
   E.steps = d.steps;
   E.T = next_pow2(2 * (d.capacity > 0 ? d.capacity : 1));
+  E.fresh = b->fresh[e];
   E.L = d.L;
   E.dt = d.dt;
   E.sf = d.stokes_factor;
@@ -1159,19 +1419,19 @@ int enqueue_check(dem_batch* b, hipStream_t st, int new_step) {
   const BLayout& l = b->l;
   const unsigned gr = (unsigned)std::max<long long>(b->plan.rows_cap, 1);
   hipLaunchKernelGGL(k_b_begin, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
-                     (const BEns*)l.ens, (const BHead*)l.head, new_step,
-                     (const int*)l.steps_done, l.stepped, l.active, l.pair_count);
+                     (const BEns*)l.ens, l.dev, (const BHead*)l.head, new_step,
+                     (const int*)l.steps_done, l.stepped, l.active, l.pair_count, l.inc, l.oob,
+                     l.merged);
   DEM_CHECK_LAUNCH("k_b_begin");
   if (!b->tune.contacts) return DEM_OK;
-  hipLaunchKernelGGL(k_b_zero, dim3(kZeroBlocks), dim3(256), 0, st, l.cnt, b->plan.cnt_total);
-  DEM_CHECK_LAUNCH("k_b_zero");
+  // the bucket counts are 0 here: cleared at creation and by every bucket scan
   hipLaunchKernelGGL(k_b_assign, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.active, (const double*)b->pos, l.cnt, l.bucket, l.rank);
+                     (const int*)l.active, (const double*)b->pos, l.cnt, l.bucket, l.rank, l.oob);
   DEM_CHECK_LAUNCH("k_b_assign");
   hipLaunchKernelGGL(k_b_bucket_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
-                     (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active,
-                     (const int*)l.cnt, l.start);
+                     (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active, l.cnt,
+                     l.start);
   DEM_CHECK_LAUNCH("k_b_bucket_scan");
   hipLaunchKernelGGL(k_b_scatter, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
@@ -1179,24 +1439,41 @@ int enqueue_check(dem_batch* b, hipStream_t st, int new_step) {
                      (const int*)l.rank, (const double*)b->pos, (const double*)b->radii, l.sorted,
                      l.spos);
   DEM_CHECK_LAUNCH("k_b_scatter");
+  if (!new_step) {  // a first check is never incremental
+    hipLaunchKernelGGL(k_b_inc_search, dim3(b->n_ens), dim3(256), 0, st, (const BEns*)l.ens,
+                       (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active, l.inc,
+                       (const int*)l.oob, (const double*)b->pos, (const double*)b->radii,
+                       (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos,
+                       l.pair_count, l.inc_pairs);
+    DEM_CHECK_LAUNCH("k_b_inc_search");
+  }
   hipLaunchKernelGGL(k_b_search, dim3(gr * kSearchSplit), dim3(kRow), 0, st,
                      (const BEns*)l.ens, (const BDev*)l.dev, (const int2*)l.rows,
-                     (const BHead*)l.head, (const int*)l.active, (const int*)l.start,
-                     (const int*)l.sorted, (const double4*)l.spos, l.pc, l.pair_count);
+                     (const BHead*)l.head, (const int*)l.active, (const int*)l.inc,
+                     (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos, l.pc,
+                     l.pair_count);
   DEM_CHECK_LAUNCH("k_b_search");
   hipLaunchKernelGGL(k_b_pair_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
-                     (const BHead*)l.head, (const int*)l.pair_count, (const int*)l.pc, l.poff);
+                     (const BHead*)l.head, (const int*)l.pair_count, (const int*)l.inc,
+                     (const int*)l.pc, l.poff);
   DEM_CHECK_LAUNCH("k_b_pair_scan");
   hipLaunchKernelGGL(k_b_pair_base, dim3(1), dim3(kScanWG), 0, st, (const BHead*)l.head,
                      (const int*)l.pair_count, l.pair_base);
   DEM_CHECK_LAUNCH("k_b_pair_base");
   hipLaunchKernelGGL(k_b_emit, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.pair_count, (const double*)b->pos, (const double*)b->radii,
-                     (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos,
-                     (const int*)l.pc, (const int*)l.poff, (const int*)l.pair_base, l.pairs,
-                     l.pvals);
+                     (const int*)l.pair_count, (const int*)l.inc, (const double*)b->pos,
+                     (const double*)b->radii, (const int*)l.start, (const int*)l.sorted,
+                     (const double4*)l.spos, (const int*)l.pc, (const int*)l.poff,
+                     (const int*)l.pair_base, l.pairs, l.pvals);
   DEM_CHECK_LAUNCH("k_b_emit");
+  if (!new_step) {
+    hipLaunchKernelGGL(k_b_inc_emit, dim3(b->n_ens), dim3(256), 0, st, (const BEns*)l.ens,
+                       (const BHead*)l.head, (const int*)l.inc, (const int*)l.pair_count,
+                       (const int*)l.pair_base, (const int2*)l.inc_pairs, (const double*)b->pos,
+                       (const double*)b->radii, l.pairs, l.pvals);
+    DEM_CHECK_LAUNCH("k_b_inc_emit");
+  }
   return DEM_OK;
 }
 
@@ -1444,6 +1721,7 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   b->n_ens = n_ens;
   b->tune = *tuning;
   b->desc.assign(ens, ens + n_ens);
+  b->fresh.assign(n_ens, 0);
   b->order.resize(n_ens);
   std::iota(b->order.begin(), b->order.end(), 0);
   std::stable_sort(b->order.begin(), b->order.end(),
@@ -1460,9 +1738,14 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   if (!rc) {
     hipLaunchKernelGGL(k_b_init, dim3((n_ens + 255) / 256), dim3(256), 0, st,
                        (const BEns*)l.ens, l.dev, n_ens, l.active, l.stepped, l.go,
-                       l.pair_count, l.steps_done, l.time);
+                       l.pair_count, l.steps_done, l.time, l.inc, l.oob, l.merged);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = dem::set_error(DEM_ERR_HIP, "k_b_init: %s", hipGetErrorString(e));
+  }
+  if (!rc) {  // bucket counts start at 0; each bucket scan leaves them at 0
+    hipLaunchKernelGGL(k_b_zero, dim3(kZeroBlocks), dim3(256), 0, st, l.cnt, p.cnt_total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = dem::set_error(DEM_ERR_HIP, "k_b_zero: %s", hipGetErrorString(e));
   }
   if (!rc) rc = refresh(b, st);
   if (rc) {
@@ -1487,7 +1770,10 @@ int dem_batch_resize(dem_batch_t b, int count, const int32_t* systems, const int
   }
   hipStream_t st = dem::as_stream(stream);
   DEM_CHECK_HIP(hipStreamSynchronize(st));
-  for (int k = 0; k < count; ++k) b->desc[systems[k]].n = n[k];
+  for (int k = 0; k < count; ++k) {
+    b->desc[systems[k]].n = n[k];
+    b->fresh[systems[k]] = 0;
+  }
   int rc = upload_tables(b, st);
   if (!rc) b->tables_dirty = false;
   if (rc) return rc;
@@ -1547,9 +1833,13 @@ static int merge_systems(dem_batch_t b, int count, const int32_t* systems,
   DEM_CHECK_LAUNCH("k_b_compact_gather");
   hipLaunchKernelGGL(k_b_compact_scatter, dim3((unsigned)blocks), dim3(256), 0, st,
                      (const BEns*)b->l.ens, count, rows_max, (const int*)b->l.stage_i,
-                     (const double*)b->l.stage_d, (const double4*)b->l.spos, b->pos, b->radii);
+                     (const double*)b->l.stage_d, (const double4*)b->l.spos, b->pos, b->radii,
+                     b->l.merged);
   DEM_CHECK_LAUNCH("k_b_compact_scatter");
-  for (int k = 0; k < count; ++k) b->desc[systems[k]].n = new_n[k];
+  for (int k = 0; k < count; ++k) {
+    b->desc[systems[k]].n = new_n[k];
+    b->fresh[systems[k]] = b->incremental ? merged_off[k + 1] - merged_off[k] : 0;
+  }
   if (!defer) {
     int rc = upload_tables(b, st);  // synchronises: the host lists may be freed after return
     if (rc) return rc;
@@ -1593,6 +1883,14 @@ int dem_batch_merge_async(dem_batch_t b, int count, const int32_t* systems,
                           const int32_t* removed_off, const int32_t* removed,
                           const int32_t* merged_off, const double* merged, dem_stream_t stream) {
   return merge_systems(b, count, systems, removed_off, removed, merged_off, merged, stream, true);
+}
+
+int dem_batch_set_incremental(dem_batch_t b, int on) {
+  if (!b) return dem::set_error(DEM_ERR_INVALID, "null handle");
+  if (on != 0 && on != 1) return dem::set_error(DEM_ERR_INVALID, "on must be 0 or 1");
+  b->incremental = on != 0;
+  if (!on) std::fill(b->fresh.begin(), b->fresh.end(), 0);
+  return DEM_OK;
 }
 
 int dem_batch_wait_go(dem_batch_t b, dem_stream_t stream) {
@@ -1646,6 +1944,7 @@ int dem_batch_field(dem_batch_t b, const char* name, void** ptr, size_t* bytes) 
                 {"stepped", b->l.stepped, ne * sizeof(int)},
                 {"pair_count", b->l.pair_count, ne * sizeof(int)},
                 {"pair_base", b->l.pair_base, (ne + 1) * sizeof(int)},
+                {"incremental", b->l.inc, ne * sizeof(int)},
                 {"steps_done", b->l.steps_done, ne * sizeof(int)},
                 {"time", b->l.time, ne * sizeof(double)},
                 {"pairs", b->l.pairs, pc * sizeof(int2)},

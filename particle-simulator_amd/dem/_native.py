@@ -148,6 +148,7 @@ SIGNATURES: dict[str, tuple] = {
     "dem_batch_merge": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "dem_batch_merge_async": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "dem_batch_wait_go": (_I, [_P, _P]),
+    "dem_batch_set_incremental": (_I, [_P, _I]),
     "dem_merge_plan": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "dem_batch_capture": (_I, [_P]),
     "dem_batch_run": (_I, [_P, _I, _P]),

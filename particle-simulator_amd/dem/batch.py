@@ -131,6 +131,7 @@ class BatchedEnsembles:
         self.pair_count = self._field("pair_count", t.int32)
         self.pair_base = self._field("pair_base", t.int32)
         self.steps_done = self._field("steps_done", t.int32)
+        self.incremental_rechecks = self._field("incremental", t.int32)
         self.time = self._field("time", t.float64)
         self.pairs = self._field("pairs", t.int32).view(-1, 2)
         self.pair_values = self._field("pair_values", t.float64).view(-1, 8)
@@ -141,6 +142,11 @@ class BatchedEnsembles:
         self._h_pairs = t.empty((k, 2), dtype=t.int32, pin_memory=True)
         self._h_vals = t.empty((k, 8), dtype=t.float64, pin_memory=True)
         self._staged = t.cuda.Event()
+        # pairs staged per round; grows to twice the largest total seen
+        self._stage_k = min(k, 2048)
+        # every merge below is the greedy plan of a check's complete pair list
+        self.incremental = True
+        _native.call("dem_batch_set_incremental", self._h, 1)
         self.captured = False
         if capture:
             _native.call("dem_batch_capture", self._h)
@@ -172,11 +178,13 @@ class BatchedEnsembles:
         _native.call("dem_batch_advance", self._h, _device.stream())
 
     def _stage(self):
-        """Queue the copies of the last check's results to pinned memory."""
-        k = self._h_pairs.shape[0]
+        """Queue the copies of the last check's results to pinned memory: the
+        counts and the first ``self._stage_k`` pairs with their values (a
+        larger total is read from the device by ``_merge_staged``)."""
+        k = self._stage_k
         self._h_counts.copy_(self.pair_count, non_blocking=True)
-        self._h_pairs.copy_(self.pairs[:k], non_blocking=True)
-        self._h_vals.copy_(self.pair_values[:k], non_blocking=True)
+        self._h_pairs[:k].copy_(self.pairs[:k], non_blocking=True)
+        self._h_vals[:k].copy_(self.pair_values[:k], non_blocking=True)
         self._staged.record()
 
     def _tick(self, name, t0):
@@ -321,13 +329,14 @@ class BatchedEnsembles:
         if len(hit) == 0:
             return 0
         base = np.concatenate([[0], np.cumsum(counts)])
-        k = self._h_pairs.shape[0]
+        k = self._stage_k
         total = int(min(base[-1], self.pair_capacity))
         pairs_h = self._h_pairs.numpy()
         vals_h = self._h_vals.numpy()
-        if total > k:  # rare: more pairs than the pinned staging holds
+        if total > k:  # more pairs than this round staged
             pairs_h = self.pairs[:total].cpu().numpy()
             vals_h = self.pair_values[:total].cpu().numpy()
+            self._stage_k = min(self._h_pairs.shape[0], 1 << int(2 * total - 1).bit_length())
         vp = ctypes.c_void_p
         if base[-1] <= self.pair_capacity:
             ro, rm, mo, mg = self._plan_native(hit, counts, base, pairs_h, vals_h)

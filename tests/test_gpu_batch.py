@@ -306,3 +306,76 @@ def test_autotune_returns_a_timed_candidate(gpu, setup):
     assert key in times and len(times) == 5 and ("pair_once", 1024) in times
     assert times[key] == min(times.values())
     assert all(t > 0 for t in times.values())
+
+
+def _recheck_rounds(b, systems, check_oracle=True):
+    """Merge rounds of one global step, comparing every check's pair lists
+    with the oracle's restatement of the reference hash on the systems'
+    current droplets.  Returns (rounds, incremental re-checks, full re-checks)."""
+    caps = [int(c) for c in b.capacity]
+    rounds = inc = full = 0
+    b.check(new_step=True)
+    while True:
+        b._stage()
+        b._staged.synchronize()
+        cnt = b.pair_count.cpu().numpy()
+        if rounds:
+            act = b.active.cpu().numpy()
+            flag = b.incremental_rechecks.cpu().numpy()
+            inc += int(((act == 1) & (flag == 1)).sum())
+            full += int(((act == 1) & (flag == 0)).sum())
+        if check_oracle:
+            for e, sy in enumerate(systems):
+                T = 1 << int(np.ceil(np.log2(2 * caps[e])))
+                _, ref = oracle.hash_pairs(b.positions(e), b.radii_of(e), sy.L, True, T)
+                got = b.contact_pairs(e)
+                assert np.array_equal(got, ref.astype(np.int32)), f"round {rounds}, system {e}"
+        if b._merge_staged(defer_tables=True) == 0:
+            return rounds, inc, full
+        rounds += 1
+        b.check(new_step=False)
+
+
+def test_incremental_rechecks_equal_the_oracle_hash(gpu):
+    """Re-checks after complete greedy merges search only around the merged
+    droplets; every round's pair lists still equal the oracle hash of the
+    current droplets.  Dense systems give chains of rounds; a system whose
+    largest droplet merges (the cell grows) and one with g <= 5 take the full
+    path or the wrapped neighbourhood."""
+    from dem.workloads import System
+
+    systems = [_overlapping(n, 70 + k, phi) for k, (n, phi) in
+               enumerate([(900, 0.12), (2500, 0.06), (400, 0.2), (1500, 0.08)])]
+    rng = np.random.default_rng(5)
+    # few large droplets in a small box: g <= 5, the largest ones touch
+    r = np.concatenate([np.full(6, 12e-6), np.full(60, 4e-6)])
+    L = 1.1e-4
+    systems.append(System(positions=rng.uniform(0, L, size=(len(r), 3)), radii=r, L=L))
+    b = BatchedEnsembles(systems, [1e-3] * len(systems), [1] * len(systems), Tuning(128, 512))
+    rounds, inc, full = _recheck_rounds(b, systems)
+    assert rounds >= 3, rounds
+    assert inc > 0 and full > 0, (inc, full)
+    b.close()
+
+
+def test_incremental_rechecks_equal_full_rechecks_bit_for_bit(gpu, setup):
+    """The same systems stepped with incremental re-checks and with full ones
+    end in identical states (and the dense setup does take incremental ones)."""
+    systems, dts = setup
+    dense = [_overlapping(n, 90 + k, phi) for k, (n, phi) in enumerate([(800, 0.1), (1600, 0.07)])]
+    sys_all = list(systems) + dense
+    dts_all = list(dts) + _dts(dense)
+    steps = STEPS + [6, 6]
+    runs = []
+    for on in (1, 0):
+        b = BatchedEnsembles(sys_all, dts_all, steps, Tuning(256, 512, pair_once=True))
+        _native.call("dem_batch_set_incremental", b._h, on)
+        b.run_until_done(60)
+        runs.append(b)
+    a, c = runs
+    assert a.merges == c.merges and a.merges > 0 and a.merge_rounds == c.merge_rounds
+    for e in range(len(sys_all)):
+        assert np.array_equal(a.radii_of(e), c.radii_of(e)), e
+        assert np.array_equal(a.positions(e), c.positions(e)), e
+    for b in runs:
+        b.close()
