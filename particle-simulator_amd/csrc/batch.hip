@@ -68,6 +68,7 @@ constexpr int kScanWG = 1024;   // threads of a per-system scan workgroup
 constexpr int kZeroBlocks = 1024;
 constexpr int kIncCap = 1024;   // pairs an incremental re-check holds per system
 constexpr int kIncFresh = 64;   // merged droplets an incremental re-check takes per system
+constexpr int kIncOld = 256;    // survivors merged since the last bucket table, per system
 
 inline int split_max(int n) {  // same cap as the single-system kernels
   long long cap = (1ll << 21) / (n > 0 ? n : 1);
@@ -174,6 +175,9 @@ struct BLayout {
   int* inc;      // [n_ens] 1: this re-check searches only around the merged droplets
   int* oob;      // [n_ens] 1: a droplet lies outside [0, L]^3
   int* merged;   // [n_ens] 1: merged on the device since the last check
+  int* t0n;      // [n_ens] droplets in the last bucket table (built by a full check)
+  int* t0alive;  // [n_ens] of which still present (their indices are 0 .. t0alive - 1)
+  double4* cscr; // [sum cap] survivors of a merge (compaction scratch)
   int2* inc_pairs;  // [n_ens kIncCap] sorted pairs of the incremental re-checks
   int* stage_i;  // merge lists uploaded by dem_batch_merge
   double* stage_d;
@@ -262,6 +266,9 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   l.inc = c.take<int>(ne);
   l.oob = c.take<int>(ne);
   l.merged = c.take<int>(ne);
+  l.t0n = c.take<int>(ne);
+  l.t0alive = c.take<int>(ne);
+  l.cscr = c.take<double4>(at_least1(p.n_total));
   l.inc_pairs = c.take<int2>(ne * kIncCap);
   l.stage_i = c.take<int>(at_least1(p.cap_total) + 3 * ne + 3);
   l.stage_d = c.take<double>(4 * at_least1(p.cap_total));
@@ -283,7 +290,8 @@ __global__ void k_b_init(const BEns* __restrict__ ens, BDev* __restrict__ dev, i
                          int* __restrict__ go, int* __restrict__ pair_count,
                          int* __restrict__ steps_done, double* __restrict__ time,
                          int* __restrict__ inc, int* __restrict__ oob,
-                         int* __restrict__ merged) {
+                         int* __restrict__ merged, int* __restrict__ t0n,
+                         int* __restrict__ t0alive) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_ens) return;
   dev[e].sc = dem::periodic_scales(ens[e].L, ens[e].m_const);
@@ -291,6 +299,8 @@ __global__ void k_b_init(const BEns* __restrict__ ens, BDev* __restrict__ dev, i
   inc[e] = 0;
   oob[e] = 0;
   merged[e] = 0;
+  t0n[e] = 0;
+  t0alive[e] = 0;
   active[e] = 0;
   stepped[e] = 0;
   go[e] = 0;
@@ -410,10 +420,26 @@ __global__ void k_b_ens_scatter(int count, const int* __restrict__ systems,
 // the survivors are gathered into `scratch` (a survivor's new index is i minus
 // the removed indices below it), then copied back and the merged droplets
 // appended.
+// Removed indices below i in the ascending list R[0, m).
+__device__ __forceinline__ int removed_below(const int* __restrict__ R, int m, int i) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (R[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The gather also carries the system's last bucket table through the merge:
+// each table slot's droplet index becomes its index after the merge, or -1
+// when it leaves, and t0alive counts the table's droplets still present
+// (survivors keep their order, so they are the first t0alive).
 __global__ __launch_bounds__(256) void k_b_compact_gather(
     const BEns* __restrict__ ens, int c, int rows_max, const int* __restrict__ stage_i,
     const double* __restrict__ pos, const double* __restrict__ radii,
-    double4* __restrict__ scratch) {
+    double4* __restrict__ scratch, int* __restrict__ sorted, const int* __restrict__ t0n,
+    int* __restrict__ t0alive) {
   const int k = (int)blockIdx.x / rows_max, r = (int)blockIdx.x % rows_max;
   if (k >= c) return;
   const int e = stage_i[k];
@@ -422,13 +448,17 @@ __global__ __launch_bounds__(256) void k_b_compact_gather(
   const int m = roff[k + 1] - roff[k];
   const BEns& E = ens[e];
   const int i = r * 256 + threadIdx.x;
-  if (i >= E.n) return;
-  int lo = 0, hi = m;  // removed indices below i
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (R[mid] < i) lo = mid + 1;
-    else hi = mid;
+  if (r == 0 && threadIdx.x == 0) t0alive[e] -= removed_below(R, m, t0alive[e]);
+  if (i < t0n[e]) {
+    int* so = sorted + E.off;
+    const int a = so[i];
+    if (a >= 0) {
+      const int lo = removed_below(R, m, a);
+      so[i] = (lo < m && R[lo] == a) ? -1 : a - lo;
+    }
   }
+  if (i >= E.n) return;
+  const int lo = removed_below(R, m, i);
   if (lo < m && R[lo] == i) return;
   const size_t off = (size_t)E.off, g = off + i;
   scratch[off + (i - lo)] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
@@ -494,7 +524,8 @@ __global__ void k_b_begin(const BEns* __restrict__ ens, BDev* __restrict__ dev,
                           const int* __restrict__ steps_done, int* __restrict__ stepped,
                           int* __restrict__ active, int* __restrict__ pair_count,
                           int* __restrict__ inc, int* __restrict__ oob,
-                          int* __restrict__ merged) {
+                          int* __restrict__ merged, int* __restrict__ t0n,
+                          int* __restrict__ t0alive) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= head->n_ens) return;
   const bool was_merged = merged[e] != 0;
@@ -505,15 +536,19 @@ __global__ void k_b_begin(const BEns* __restrict__ ens, BDev* __restrict__ dev,
   const bool act = want && eligible(E, steps_done, stepped, e);
   active[e] = act ? 1 : 0;
   pair_count[e] = 0;
-  oob[e] = 0;
   bool incremental = false;
   if (act) {
     BDev& D = dev[e];
     const double w = E.L / D.cell - (double)(D.g - 1);  // width of the last cell, in cells
     incremental = !new_step && was_merged && E.fresh > 0 && E.fresh <= kIncFresh &&
-                  E.fresh <= E.n &&
+                  E.fresh <= E.n && !oob[e] && t0n[e] > 0 &&
                   !D.skip && D.cell == D.cell_checked && (D.g <= 5 || w >= 1e-3);
     D.cell_checked = D.cell;
+    if (!incremental) {  // this check builds a new bucket table
+      oob[e] = 0;
+      t0n[e] = E.n;
+      t0alive[e] = E.n;
+    }
   }
   inc[e] = incremental ? 1 : 0;
 }
@@ -546,12 +581,13 @@ __global__ __launch_bounds__(kRow) void k_b_assign(const BEns* __restrict__ ens,
                                                    const int2* __restrict__ rows,
                                                    const BHead* __restrict__ head,
                                                    const int* __restrict__ active,
+                                                   const int* __restrict__ inc,
                                                    const double* __restrict__ pos,
                                                    int* __restrict__ cnt, int* __restrict__ bucket,
                                                    int* __restrict__ rank, int* __restrict__ oob) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (!hashed(head, active, dev, row.x)) return;
+  if (!searched(head, active, dev, inc, row.x)) return;
   const BEns& E = ens[row.x];
   const int i = row.y + threadIdx.x;
   if (i >= E.n) return;
@@ -623,9 +659,10 @@ __global__ __launch_bounds__(kScanWG) void k_b_bucket_scan(const BEns* __restric
                                                            const BDev* __restrict__ dev,
                                                            const BHead* __restrict__ head,
                                                            const int* __restrict__ active,
+                                                           const int* __restrict__ inc,
                                                            int* cnt, int* __restrict__ start) {
   const int e = blockIdx.x;
-  if (e >= head->n_ens || !hashed(head, active, dev, e)) return;
+  if (e >= head->n_ens || !searched(head, active, dev, inc, e)) return;
   const BEns& E = ens[e];
   // T + 1 slots per system; the counts are left at 0 for the next check's atomics
   block_scan(cnt + E.cnt_off, start + E.start_off, E.T, true, cnt + E.cnt_off);
@@ -636,6 +673,7 @@ __global__ __launch_bounds__(kRow) void k_b_scatter(const BEns* __restrict__ ens
                                                     const int2* __restrict__ rows,
                                                     const BHead* __restrict__ head,
                                                     const int* __restrict__ active,
+                                                    const int* __restrict__ inc,
                                                     const int* __restrict__ start,
                                                     const int* __restrict__ bucket,
                                                     const int* __restrict__ rank,
@@ -645,7 +683,7 @@ __global__ __launch_bounds__(kRow) void k_b_scatter(const BEns* __restrict__ ens
                                                     double4* __restrict__ spos) {
   if ((int)blockIdx.x >= head->n_rows) return;
   const int2 row = rows[blockIdx.x];
-  if (!hashed(head, active, dev, row.x)) return;
+  if (!searched(head, active, dev, inc, row.x)) return;
   const BEns& E = ens[row.x];
   const int i = row.y + threadIdx.x;
   if (i >= E.n) return;
@@ -655,7 +693,37 @@ __global__ __launch_bounds__(kRow) void k_b_scatter(const BEns* __restrict__ ens
   spos[slot] = make_double4(pos[3 * g], pos[3 * g + 1], pos[3 * g + 2], radii[g]);
 }
 
-// Pairs (i, j), i < j, of target i over its distinct neighbour buckets
+// The 27 neighbour cells of cell (ix, iy, iz) (periodic), neighbour k =
+// 9 a + 3 b + c at offsets (a - 1, b - 1, c - 1).  The reference visits the
+// bucket of every neighbour, so a bucket that two neighbours share (a hash
+// alias, or the same cell when g <= 2) yields duplicate pairs, which its
+// np.unique removes: its pair set is {(i, j): i < j, contact, bucket(j) in
+// B(i)}.  Here a candidate is counted in neighbour k only when no neighbour
+// q < k has the same bucket; that test runs only for candidates in contact,
+// so the neighbour buckets need no deduplication up front.
+struct Nbhd {
+  int cx[3], cy[3], cz[3];
+  int mask;
+  __device__ __forceinline__ Nbhd(int ix, int iy, int iz, int g, int m) : mask(m) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      cx[d] = dem::nb_coord(ix, d - 1, g, 1);
+      cy[d] = dem::nb_coord(iy, d - 1, g, 1);
+      cz[d] = dem::nb_coord(iz, d - 1, g, 1);
+    }
+  }
+  __device__ __forceinline__ int bucket(int k) const {
+    return dem::hash_cell(cx[k / 9], cy[(k / 3) % 3], cz[k % 3], mask);
+  }
+  // whether neighbour k is the first with bucket h
+  __device__ __forceinline__ bool first(int k, int h) const {
+    for (int q = 0; q < k; ++q)
+      if (bucket(q) == h) return false;
+    return true;
+  }
+};
+
+// Pairs (i, j), i < j, of target i over its neighbour buckets
 // (spatial_hash...py:134-229); EMIT writes them to out[0, room) sorted by j.
 // The 27 bucket ranges are loaded up front and candidates are read from the
 // bucket-ordered copy (sorted index and (x, y, z, r) of the same slot), so a
@@ -666,35 +734,17 @@ __device__ int search_target(const BEns& E, const BDev& D, const double* __restr
                              const int* __restrict__ sorted, const double4* __restrict__ spos,
                              int i, int2* __restrict__ out, double4* __restrict__ vals,
                              int room) {
-  const int mask = E.T - 1, g = D.g;
+  const int g = D.g;
   const double xi = p[3 * (size_t)i], yi = p[3 * (size_t)i + 1], zi = p[3 * (size_t)i + 2];
   const double ri = r[i];
-  const int ix = dem::cell_of(xi, D.cell, g, 1);
-  const int iy = dem::cell_of(yi, D.cell, g, 1);
-  const int iz = dem::cell_of(zi, D.cell, g, 1);
-  int cx[3], cy[3], cz[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    cx[d] = dem::nb_coord(ix, d - 1, g, 1);
-    cy[d] = dem::nb_coord(iy, d - 1, g, 1);
-    cz[d] = dem::nb_coord(iz, d - 1, g, 1);
-  }
-  int nb[27];
-  int nseen = 0;
+  const Nbhd nb(dem::cell_of(xi, D.cell, g, 1), dem::cell_of(yi, D.cell, g, 1),
+                dem::cell_of(zi, D.cell, g, 1), g, E.T - 1);
+  int hk[27], s0[27], s1[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) {
-    const int nh = dem::hash_cell(cx[k / 9], cy[(k / 3) % 3], cz[k % 3], mask);
-    bool dup = false;
-#pragma unroll
-    for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
-    if (!dup) nb[nseen++] = nh;
-  }
-  int s0[27], s1[27];
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    const int h = k < nseen ? nb[k] : 0;
-    s0[k] = start[h];
-    s1[k] = k < nseen ? start[h + 1] : s0[k];
+    hk[k] = nb.bucket(k);
+    s0[k] = start[hk[k]];
+    s1[k] = start[hk[k] + 1];
   }
   int cnt = 0;
 #pragma unroll
@@ -702,7 +752,8 @@ __device__ int search_target(const BEns& E, const BDev& D, const double* __restr
     for (int slot = s0[k]; slot < s1[k]; ++slot) {
       const int j = sorted[slot];
       const double4 q = spos[slot];
-      if (i < j && dem::contact_pair(xi, yi, zi, ri, q.x, q.y, q.z, q.w, E.L, 1)) {
+      if (i < j && dem::contact_pair(xi, yi, zi, ri, q.x, q.y, q.z, q.w, E.L, 1) &&
+          nb.first(k, hk[k])) {
         if (EMIT && cnt < room) {
           out[cnt] = make_int2(i, j);
           vals[2 * cnt] = make_double4(xi, yi, zi, ri);
@@ -731,46 +782,28 @@ __device__ int search_target(const BEns& E, const BDev& D, const double* __restr
 }
 
 // Count pass of the droplet in bucket slot `slot` (neighbouring slots share
-// neighbour buckets): lane `sub` of SPLIT takes the distinct neighbour buckets
-// k = sub, sub + SPLIT, ...; the candidate's droplet index is read only for a
-// contact.  Returns this lane's share of the droplet's pairs (i, j), i < j.
+// neighbour buckets): lane `sub` of SPLIT takes the neighbours k = sub,
+// sub + SPLIT, ...; the candidate's droplet index is read only for a contact.
+// Returns this lane's share of the droplet's pairs (i, j), i < j.
 template <int SPLIT>
 __device__ int count_slot(const BEns& E, const BDev& D, const int* __restrict__ start,
                           const int* __restrict__ sorted, const double4* __restrict__ spos,
                           int slot, int sub, int* i_out) {
-  const int mask = E.T - 1, g = D.g;
+  const int g = D.g;
   const double4 me = spos[slot];
   const int i = sorted[slot];
   *i_out = i;
-  const int ix = dem::cell_of(me.x, D.cell, g, 1);
-  const int iy = dem::cell_of(me.y, D.cell, g, 1);
-  const int iz = dem::cell_of(me.z, D.cell, g, 1);
-  int cx[3], cy[3], cz[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    cx[d] = dem::nb_coord(ix, d - 1, g, 1);
-    cy[d] = dem::nb_coord(iy, d - 1, g, 1);
-    cz[d] = dem::nb_coord(iz, d - 1, g, 1);
-  }
-  int nb[27];
-  int nseen = 0;
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    const int nh = dem::hash_cell(cx[k / 9], cy[(k / 3) % 3], cz[k % 3], mask);
-    bool dup = false;
-#pragma unroll
-    for (int q = 0; q < k; ++q) dup |= (q < nseen && nb[q] == nh);
-    if (!dup) nb[nseen++] = nh;
-  }
+  const Nbhd nb(dem::cell_of(me.x, D.cell, g, 1), dem::cell_of(me.y, D.cell, g, 1),
+                dem::cell_of(me.z, D.cell, g, 1), g, E.T - 1);
   constexpr int KS = (27 + SPLIT - 1) / SPLIT;
-  int s0[KS], s1[KS];
+  int hk[KS], s0[KS], s1[KS];
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
     const int k = sub + SPLIT * u;
-    const bool ok = k < nseen;
-    const int h = ok ? nb[k < 27 ? k : 0] : 0;
-    s0[u] = start[h];
-    s1[u] = ok ? start[h + 1] : s0[u];
+    const bool ok = k < 27;
+    hk[u] = nb.bucket(ok ? k : 0);
+    s0[u] = start[hk[u]];
+    s1[u] = ok ? start[hk[u] + 1] : s0[u];
   }
   int cnt = 0;
 #pragma unroll
@@ -778,7 +811,7 @@ __device__ int count_slot(const BEns& E, const BDev& D, const int* __restrict__ 
     for (int q = s0[u]; q < s1[u]; ++q) {
       const double4 c = spos[q];
       if (dem::contact_pair(me.x, me.y, me.z, me.w, c.x, c.y, c.z, c.w, E.L, 1) &&
-          i < sorted[q])
+          i < sorted[q] && nb.first(sub + SPLIT * u, hk[u]))
         ++cnt;
     }
   }
@@ -882,40 +915,59 @@ struct IncLds {
 //       the periodic ring (floor(fl(x/cell)) of the two positions differ by at
 //       most 2 directly, and across the seam by g - 2 .. g + 1 when the last
 //       cell is at least 1e-3 cells wide, k_b_begin's condition; with g <= 5
-//       the five offsets cover the ring).  The buckets of those 125 cells,
-//       each once, hold every such a.
+//       the five offsets cover the ring).  Survivors from the system's last
+//       bucket table (built by its last full check, carried through the merges
+//       since by k_b_compact_gather: positions unchanged, indices remapped,
+//       -1 once removed) are found in the buckets of those 125 cells, each
+//       bucket once; the few survivors merged after that table was built
+//       (indices t0alive .. n_s - 1) are tested one by one.
 //   (b) (m1, m2), both merged: every one of the fresh^2 / 2 pairs is tested.
 // contact_pair is symmetric in its two droplets (negated differences, the same
 // squares and sum of radii), so a pair is tested as in the full search.  The
 // pairs are sorted lexicographically into inc_pairs[e].  More than kIncCap of
-// them, or a droplet outside [0, L]^3 (the ring bound assumes wrapped
-// positions), clears inc: the full search then runs for the system.
+// them, more than kIncOld later survivors, or a merged droplet outside
+// [0, L]^3 (the ring bound assumes wrapped positions) clears inc: the full
+// check (new table, full search) then runs for the system.
 __global__ __launch_bounds__(256) void k_b_inc_search(
     const BEns* __restrict__ ens, const BDev* __restrict__ dev, const BHead* __restrict__ head,
-    const int* __restrict__ active, int* __restrict__ inc, const int* __restrict__ oob,
-    const double* __restrict__ pos, const double* __restrict__ radii,
-    const int* __restrict__ start, const int* __restrict__ sorted,
-    const double4* __restrict__ spos, int* __restrict__ pair_count,
-    int2* __restrict__ inc_pairs) {
+    const int* __restrict__ active, int* __restrict__ inc, int* __restrict__ oob,
+    int* __restrict__ t0n, int* __restrict__ t0alive, const double* __restrict__ pos,
+    const double* __restrict__ radii, const int* __restrict__ start,
+    const int* __restrict__ sorted, const double4* __restrict__ spos,
+    int* __restrict__ pair_count, int2* __restrict__ inc_pairs) {
   __shared__ IncLds S;
   const int e = blockIdx.x;
   if (e >= head->n_ens || !inc[e]) return;  // uniform over the block
-  if (!hashed(head, active, dev, e) || oob[e]) {
-    if (threadIdx.x == 0) inc[e] = 0;
-    return;
-  }
   const BEns& E = ens[e];
   const BDev D = dev[e];
   const int n = E.n, k = E.fresh, ns = n - k, g = D.g, mask = E.T - 1;
+  const int old0 = t0alive[e];
+  const double* P = pos + 3 * (size_t)E.off;
+  const double* R = radii + E.off;
+  bool bad = false;
+  for (int m = ns + (int)threadIdx.x; m < n; m += 256) {
+    const double* x = P + 3 * (size_t)m;
+    bad |= !(x[0] >= 0.0 && x[0] <= E.L && x[1] >= 0.0 && x[1] <= E.L && x[2] >= 0.0 &&
+             x[2] <= E.L);
+  }
+  bad = __syncthreads_or(bad) != 0;
+  if (!hashed(head, active, dev, e) || bad || old0 > ns || ns - old0 > kIncOld) {
+    if (threadIdx.x == 0) {  // a full check: new table for the current droplets
+      inc[e] = 0;
+      oob[e] = 0;
+      t0n[e] = n;
+      t0alive[e] = n;
+    }
+    return;
+  }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int* st = start + E.start_off;
   const int* so = sorted + E.off;
   const double4* sp = spos + E.off;
-  const double* P = pos + 3 * (size_t)E.off;
-  const double* R = radii + E.off;
   if (threadIdx.x == 0) S.count = 0;
   __syncthreads();
-  // (a): wave w takes merged droplets ns + w, ns + w + 4, ... (a uniform loop)
+  // (a), table survivors: wave w takes merged droplets ns + w, ns + w + 4, ...
+  // (a uniform loop)
   for (int m = ns + w; m < n; m += 4) {
     const double xm = P[3 * (size_t)m], ym = P[3 * (size_t)m + 1], zm = P[3 * (size_t)m + 2];
     const double rm = R[m];
@@ -962,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_b_inc_search(
       }
       const int slot = S.first[w][lo] + (c - S.pre[w][lo]);
       const int a = so[slot];
-      if (a >= ns) continue;
+      if (a < 0 || a >= ns) continue;
       const double4 q = sp[slot];
       if (!dem::contact_pair(q.x, q.y, q.z, q.w, xm, ym, zm, rm, E.L, 1)) continue;
       if (!in_neighbourhood(dem::cell_of(q.x, D.cell, g, 1), dem::cell_of(q.y, D.cell, g, 1),
@@ -973,25 +1025,31 @@ __global__ __launch_bounds__(256) void k_b_inc_search(
     }
     dem::wave_sync();  // nb, pre and first are rewritten for the next droplet
   }
-  // (b)
-  for (int idx = threadIdx.x; idx < k * k; idx += 256) {
-    const int m1 = ns + idx / k, m2 = ns + idx % k;
-    if (m1 >= m2) continue;
-    const double x1 = P[3 * (size_t)m1], y1 = P[3 * (size_t)m1 + 1], z1 = P[3 * (size_t)m1 + 2];
-    const double x2 = P[3 * (size_t)m2], y2 = P[3 * (size_t)m2 + 1], z2 = P[3 * (size_t)m2 + 2];
-    if (!dem::contact_pair(x1, y1, z1, R[m1], x2, y2, z2, R[m2], E.L, 1)) continue;
-    const int h2 = dem::hash_cell(dem::cell_of(x2, D.cell, g, 1), dem::cell_of(y2, D.cell, g, 1),
-                                  dem::cell_of(z2, D.cell, g, 1), mask);
-    if (!in_neighbourhood(dem::cell_of(x1, D.cell, g, 1), dem::cell_of(y1, D.cell, g, 1),
-                          dem::cell_of(z1, D.cell, g, 1), g, mask, h2))
+  // (a), later survivors, and (b): target t < m, merged m
+  const int nt = n - old0;  // targets old0 .. n - 1
+  for (int idx = threadIdx.x; idx < nt * k; idx += 256) {
+    const int t = old0 + idx / k, m = ns + idx % k;
+    if (t >= m) continue;
+    const double xt = P[3 * (size_t)t], yt = P[3 * (size_t)t + 1], zt = P[3 * (size_t)t + 2];
+    const double xm = P[3 * (size_t)m], ym = P[3 * (size_t)m + 1], zm = P[3 * (size_t)m + 2];
+    if (!dem::contact_pair(xt, yt, zt, R[t], xm, ym, zm, R[m], E.L, 1)) continue;
+    const int hm = dem::hash_cell(dem::cell_of(xm, D.cell, g, 1), dem::cell_of(ym, D.cell, g, 1),
+                                  dem::cell_of(zm, D.cell, g, 1), mask);
+    if (!in_neighbourhood(dem::cell_of(xt, D.cell, g, 1), dem::cell_of(yt, D.cell, g, 1),
+                          dem::cell_of(zt, D.cell, g, 1), g, mask, hm))
       continue;
     const int at = atomicAdd(&S.count, 1);
-    if (at < kIncCap) S.key[at] = ((unsigned long long)m1 << 32) | (unsigned)m2;
+    if (at < kIncCap) S.key[at] = ((unsigned long long)t << 32) | (unsigned)m;
   }
   __syncthreads();
   const int cnt = S.count;
   if (cnt > kIncCap) {
-    if (threadIdx.x == 0) inc[e] = 0;
+    if (threadIdx.x == 0) {
+      inc[e] = 0;
+      oob[e] = 0;
+      t0n[e] = n;
+      t0alive[e] = n;
+    }
     return;
   }
   // rank sort (the keys are distinct: every pair is found once)
@@ -1421,32 +1479,33 @@ int enqueue_check(dem_batch* b, hipStream_t st, int new_step) {
   hipLaunchKernelGGL(k_b_begin, dim3((b->n_ens + 255) / 256), dim3(256), 0, st,
                      (const BEns*)l.ens, l.dev, (const BHead*)l.head, new_step,
                      (const int*)l.steps_done, l.stepped, l.active, l.pair_count, l.inc, l.oob,
-                     l.merged);
+                     l.merged, l.t0n, l.t0alive);
   DEM_CHECK_LAUNCH("k_b_begin");
   if (!b->tune.contacts) return DEM_OK;
-  // the bucket counts are 0 here: cleared at creation and by every bucket scan
-  hipLaunchKernelGGL(k_b_assign, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
-                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.active, (const double*)b->pos, l.cnt, l.bucket, l.rank, l.oob);
-  DEM_CHECK_LAUNCH("k_b_assign");
-  hipLaunchKernelGGL(k_b_bucket_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
-                     (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active, l.cnt,
-                     l.start);
-  DEM_CHECK_LAUNCH("k_b_bucket_scan");
-  hipLaunchKernelGGL(k_b_scatter, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
-                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.active, (const int*)l.start, (const int*)l.bucket,
-                     (const int*)l.rank, (const double*)b->pos, (const double*)b->radii, l.sorted,
-                     l.spos);
-  DEM_CHECK_LAUNCH("k_b_scatter");
   if (!new_step) {  // a first check is never incremental
     hipLaunchKernelGGL(k_b_inc_search, dim3(b->n_ens), dim3(256), 0, st, (const BEns*)l.ens,
                        (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active, l.inc,
-                       (const int*)l.oob, (const double*)b->pos, (const double*)b->radii,
+                       l.oob, l.t0n, l.t0alive, (const double*)b->pos, (const double*)b->radii,
                        (const int*)l.start, (const int*)l.sorted, (const double4*)l.spos,
                        l.pair_count, l.inc_pairs);
     DEM_CHECK_LAUNCH("k_b_inc_search");
   }
+  // the bucket counts are 0 here: cleared at creation and by every bucket scan
+  hipLaunchKernelGGL(k_b_assign, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.active, (const int*)l.inc, (const double*)b->pos, l.cnt,
+                     l.bucket, l.rank, l.oob);
+  DEM_CHECK_LAUNCH("k_b_assign");
+  hipLaunchKernelGGL(k_b_bucket_scan, dim3(b->n_ens), dim3(kScanWG), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const BHead*)l.head, (const int*)l.active,
+                     (const int*)l.inc, l.cnt, l.start);
+  DEM_CHECK_LAUNCH("k_b_bucket_scan");
+  hipLaunchKernelGGL(k_b_scatter, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
+                     (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
+                     (const int*)l.active, (const int*)l.inc, (const int*)l.start,
+                     (const int*)l.bucket, (const int*)l.rank, (const double*)b->pos,
+                     (const double*)b->radii, l.sorted, l.spos);
+  DEM_CHECK_LAUNCH("k_b_scatter");
   hipLaunchKernelGGL(k_b_search, dim3(gr * kSearchSplit), dim3(kRow), 0, st,
                      (const BEns*)l.ens, (const BDev*)l.dev, (const int2*)l.rows,
                      (const BHead*)l.head, (const int*)l.active, (const int*)l.inc,
@@ -1738,7 +1797,8 @@ int dem_batch_create(const dem_batch_ensemble* ens, int n_ens, const dem_batch_t
   if (!rc) {
     hipLaunchKernelGGL(k_b_init, dim3((n_ens + 255) / 256), dim3(256), 0, st,
                        (const BEns*)l.ens, l.dev, n_ens, l.active, l.stepped, l.go,
-                       l.pair_count, l.steps_done, l.time, l.inc, l.oob, l.merged);
+                       l.pair_count, l.steps_done, l.time, l.inc, l.oob, l.merged, l.t0n,
+                       l.t0alive);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = dem::set_error(DEM_ERR_HIP, "k_b_init: %s", hipGetErrorString(e));
   }
@@ -1829,11 +1889,12 @@ static int merge_systems(dem_batch_t b, int count, const int32_t* systems,
   if (blocks > 0x7fffffffLL) return dem::set_error(DEM_ERR_CAPACITY, "too many rows to merge");
   hipLaunchKernelGGL(k_b_compact_gather, dim3((unsigned)blocks), dim3(256), 0, st,
                      (const BEns*)b->l.ens, count, rows_max, (const int*)b->l.stage_i,
-                     (const double*)b->pos, (const double*)b->radii, b->l.spos);
+                     (const double*)b->pos, (const double*)b->radii, b->l.cscr, b->l.sorted,
+                     (const int*)b->l.t0n, b->l.t0alive);
   DEM_CHECK_LAUNCH("k_b_compact_gather");
   hipLaunchKernelGGL(k_b_compact_scatter, dim3((unsigned)blocks), dim3(256), 0, st,
                      (const BEns*)b->l.ens, count, rows_max, (const int*)b->l.stage_i,
-                     (const double*)b->l.stage_d, (const double4*)b->l.spos, b->pos, b->radii,
+                     (const double*)b->l.stage_d, (const double4*)b->l.cscr, b->pos, b->radii,
                      b->l.merged);
   DEM_CHECK_LAUNCH("k_b_compact_scatter");
   for (int k = 0; k < count; ++k) {
