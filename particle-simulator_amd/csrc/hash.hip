@@ -63,6 +63,7 @@ constexpr int kMaxTiles = 8192;               // so table_size and n are at most
 constexpr int kMaxRBlocks = 256;              // partial maxima of the radii
 constexpr int kStash = 8;                     // partners kept per target by the search
 constexpr int kBucketsPerWave = 8;
+constexpr int kMapCap = 256;  // candidates per group mapped to their neighbour in LDS
 
 inline int tiles(int n) { return (n + kTile - 1) / kTile; }
 
@@ -289,8 +290,12 @@ __global__ __launch_bounds__(kThreads) void k_hash_scatter(
 
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-// Distinct neighbour buckets of a cell: lane k < 27 gets neighbour k's bucket,
-// or -1 when it is outside an open box or repeats the bucket of a lower lane.
+// Neighbour buckets of a cell: lane k < 27 gets neighbour k's bucket, or -1
+// when it is outside an open box.  Two neighbours may share a bucket (a hash
+// alias, or the same cell when g <= 2); the reference then visits the bucket
+// twice and np.unique drops the repeated pairs.  The search scans every
+// neighbour's bucket and counts a pair only in the first neighbour with that
+// bucket (tested for candidates in contact only).
 __device__ __forceinline__ int neighbour_bucket(int ix, int iy, int iz, int g, int periodic,
                                                 int mask) {
   const int lane = threadIdx.x & 63;
@@ -301,13 +306,7 @@ __device__ __forceinline__ int neighbour_bucket(int ix, int iy, int iz, int g, i
     const int nz = dem::nb_coord(iz, lane % 3 - 1, g, periodic);
     if (nx >= 0 && ny >= 0 && nz >= 0) nh = hash_cell(nx, ny, nz, mask);
   }
-  bool dup = false;
-#pragma unroll
-  for (int q = 0; q < 26; ++q) {
-    const int v = readlane(nh, q);
-    dup |= (q < lane && v >= 0 && v == nh);
-  }
-  return dup ? -1 : nh;
+  return nh;
 }
 
 // Orders one wave's LDS accesses across its lanes.
@@ -324,6 +323,8 @@ struct SearchLds {
   int list[64][kStash];               // the first kStash of them
   int start[27];                      // neighbour bucket k: first slot
   int before[27];                     // candidates in neighbours < k
+  int bucket[27];                     // neighbour k's bucket (-1: outside an open box)
+  unsigned char nbr[kMapCap];         // neighbour of candidate m < kMapCap
 };
 
 // One wave per kBucketsPerWave consecutive buckets (their counts and offsets
@@ -378,39 +379,42 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
         const bool mine = ((remaining >> lane) & 1ull) && kx == gx && ky == gy && kz == gz;
         const unsigned long long members = __ballot(mine);
         remaining &= ~members;
-        // candidates: the slots of the group's distinct neighbour buckets
+        // candidates: the slots of the group's neighbour buckets, candidate
+        // m in neighbour k for before[k] <= m < before[k] + len[k]
         const int nh = neighbour_bucket(gx, gy, gz, g, periodic, mask);
         const int len = nh >= 0 ? counts[nh] : 0;
         const int first = nh >= 0 ? offsets[nh] : 0;
-        int before = 0, M = 0;
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-          const int lq = readlane(len, q);
-          before += q < lane ? lq : 0;
-          M += lq;
-        }
+        const int incl = wave_inclusive(len);  // lanes >= 27 add nothing
+        const int before = incl - len;
+        const int M = readlane(incl, 63);
         if (lane < 27) {
           S.start[lane] = first;
           S.before[lane] = before;
+          S.bucket[lane] = nh;
+          for (int t = before; t < before + len && t < kMapCap; ++t) S.nbr[t] = (unsigned char)lane;
         }
-        const int inc = before + len;  // candidates in neighbours <= lane
         wave_sync();
-        // a uniform loop: every lane is active at the readlanes (a lane read
-        // while inactive in a divergent region holds no defined value)
-        // two candidates per lane and pass, both loads in flight before the
-        // tests; a uniform loop: every lane is active at the readlanes (a lane
-        // read while inactive in a divergent region holds no defined value)
+        // two candidates per lane and pass, both loads in flight before the tests
         for (int m0 = 0; m0 < M; m0 += 128) {
-          int slot[2];
-          double4 cand[2];
+          int slot[2], nbk[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const int m = m0 + 64 * u + lane;
-            int k = 0;  // the neighbour holding candidate m: neighbours whose range ends by m
-#pragma unroll
-            for (int q = 0; q < 26; ++q) k += readlane(inc, q) <= m ? 1 : 0;
+            int k = 0;
+            if (m < kMapCap) {
+              k = S.nbr[m < M ? m : 0];
+            } else {  // the last neighbour whose range starts at or before m
+              int hi = 26;
+              while (k < hi) {
+                const int mid = (k + hi + 1) >> 1;
+                if (S.before[mid] <= m) k = mid;
+                else hi = mid - 1;
+              }
+            }
+            nbk[u] = k;
             slot[u] = m < M ? S.start[k] + (m - S.before[k]) : -1;
           }
+          double4 cand[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u)
             if (slot[u] >= 0) cand[u] = spos[slot[u]];
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
           for (int u = 0; u < 2; ++u) {
             if (slot[u] < 0) continue;
             const double4 q = cand[u];
-            int j = -1;
+            int j = -1, first = -1;
             unsigned long long mm = members;
             while (mm) {
               const int t = __ffsll((long long)mm) - 1;
@@ -426,6 +430,13 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
               if (contact_pair(S.x[t], S.y[t], S.z[t], S.r[t], q.x, q.y, q.z, q.w, L, periodic)) {
                 if (j < 0) j = sorted[slot[u]];
                 if (S.id[t] < j) {
+                  if (first < 0) {  // counted only in the first neighbour with this bucket
+                    const int h = S.bucket[nbk[u]];
+                    first = 1;
+                    for (int qk = 0; qk < nbk[u]; ++qk)
+                      if (S.bucket[qk] == h) first = 0;
+                  }
+                  if (!first) break;
                   const int p = atomicAdd(&S.cnt[t], 1);
                   if (p < kStash) S.list[t][p] = j;
                 }
