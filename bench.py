@@ -92,6 +92,75 @@ def cpu_baseline(system, fc, F_any, budget_s: float) -> dict:
     }
 
 
+def config0_cpu(steps: int = 100) -> dict:
+    """The FP64 C/OpenMP oracle stepping the N = 2000 system (forces, Oseen
+    velocities, Euler step and wrap) for `steps` steps: the CPU side of the
+    plain direct workload at the reference's CPU-runnable size."""
+    import numpy as np
+
+    import oracle
+    from dem.workloads import make_system
+
+    s = make_system(2000, seed=0, phi=0.02)
+    F = oracle.direct_forces(s.positions, s.radii, s.m_const, s.L, True)
+    V = oracle.direct_convection(s.positions, F, s.eta_const, s.L, True)
+    v0 = F / (s.stokes_factor * s.radii[:, None]) + V
+    dt = 0.1 * s.radii.min() / np.max(np.linalg.norm(v0, axis=1))
+    x = s.positions.copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        F = oracle.direct_forces(x, s.radii, s.m_const, s.L, True)
+        V = oracle.direct_convection(x, F, s.eta_const, s.L, True)
+        x = oracle.euler_step(x, F, V, s.radii, s.stokes_factor, dt, s.L, True)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    return {"ms_per_step": ms, "pair_interactions_per_s": s.n * (s.n - 1) / (ms * 1e-3),
+            "steps": steps, "cores": oracle.num_threads(), "kind": "port"}
+
+
+def config0_leg(steps: int = 100) -> dict:
+    """The plain direct workload at N = 2000 (seed 0, phi = 0.02, periodic,
+    E = 3e5 V/m): 100 steps of the device-resident solver without contact
+    detection (no coalescence), dt = 0.1 r_min / max|v(0)| from the FP32
+    path; positions copied to the host trajectory every step."""
+    import numpy as np
+    import torch
+
+    from dem.force_calculator import DirectDropletForceCalculator
+    from dem.particle_state import DropletState
+    from dem.post_processor import DropletPostProcessor
+    from dem.solution import DropletSolution
+    from dem.solver.device_solver import DeviceEulerDropletSolver
+    from dem.workloads import make_system
+
+    s = make_system(2000, seed=0, phi=0.02)
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                      eta_water=s.eta_water, E=s.E, L=s.L,
+                                      boundary_mode="periodic")
+    F, V = fc.calculate_forces_and_convection(s.positions, s.radii)
+    vmax = np.max(np.linalg.norm(F / (s.stokes_factor * s.radii[:, None]) + V, axis=1))
+    dt = 0.1 * s.radii.min() / vmax
+
+    def run(nsteps):
+        sol = DropletSolution(DropletState(s.positions.copy(), s.radii.copy(), 0.0),
+                              length=nsteps + 8)
+        solver = DeviceEulerDropletSolver(force_calculator=fc, solution=sol,
+                                          post_processor=DropletPostProcessor(sol, box_size=s.L),
+                                          collision_detector=None, verbose=False)
+        t0 = time.perf_counter()
+        solver.solve(dt, (nsteps - 0.5) * dt)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, solver
+
+    run(5)
+    el, solver = run(steps)
+    k = solver.steps_done
+    ms = el / max(k, 1) * 1e3
+    return {"workload": "N=2000, phi=0.02, periodic, seed 0, direct operator, no coalescence",
+            "steps": k, "ms_per_step": ms, "pair_interactions_per_s": s.n * (s.n - 1) / (ms * 1e-3),
+            "note": "device-resident loop, positions copied to the host trajectory every "
+                    "step; at this size a step is a handful of dependent launches"}
+
+
 def hash_pass(s, reps: int = 20, radius_scale: float = 1.6) -> dict:
     """Spatial-hash contact pass on the same positions with radii x 1.6 (a
     system with contacts, so the emit path is timed too); device-resident
@@ -450,8 +519,8 @@ def main() -> int:
                     help="also run a plain 1 GiB device copy (scale for PMC byte counters)")
     ap.add_argument("--one-sided", action="store_true",
                     help="time the one-sided direct kernels instead of the pair-once ones")
-    ap.add_argument("--legs", default="hash,step,tree,pbm,ensembles,adjoint",
-                    help="comma list of extra measurements: hash, step, tree, pbm, ensembles, "
+    ap.add_argument("--legs", default="config0,hash,step,tree,pbm,ensembles,adjoint",
+                    help="comma list of extra measurements: config0, hash, step, tree, pbm, ensembles, "
                          "adjoint")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -555,6 +624,10 @@ def main() -> int:
     if not a.no_cpu_baseline:
         base = cpu_baseline(s, fc, Fn, a.cpu_seconds)
     extras = {}
+    if "config0" in legs:
+        extras["config0"] = config0_leg()
+        if base is not None:
+            extras["config0"]["cpu_baseline"] = config0_cpu()
     if "hash" in legs:
         extras["hash"] = hash_pass(s)
     if "tree" in legs:
