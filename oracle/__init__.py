@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_brute_pairs.restype = i
         _lib.oracle_num_threads.restype = i
         _lib.oracle_set_threads.argtypes = [i]
+        _lib.oracle_set_plant.argtypes = [i]
         _lib.oracle_tree_build.argtypes = [p]
         _lib.oracle_tree_build.restype = i
         _lib.oracle_tree_propagate_forces.argtypes = [p, p]
@@ -121,6 +122,13 @@ def _ptr(a: np.ndarray) -> int:
 
 def num_threads() -> int:
     return int(lib().oracle_num_threads())
+
+
+def set_plant(plant: int) -> None:
+    """Plant a mistake in the tree walks (tests only; 0 restores them): bit 1
+    drops the trace term of the Stokeslet dipole, bit 2 the whole dipole,
+    bit 4 evaluates the force far field at the node's geometric centre."""
+    lib().oracle_set_plant(int(plant))
 
 
 def set_threads(t: int) -> None:

@@ -250,6 +250,13 @@ void oracle_tree_propagate_forces(oracle_tree* t, const double* F) {
   }
 }
 
+/* Mistakes planted by tests (0 in every other use), to show that the
+ * far-field order tests detect them: bit 1 drops the trace term of the
+ * Stokeslet dipole, bit 2 drops the whole dipole, bit 4 evaluates the force
+ * far field at the node's geometric centre instead of its R^3 centroid. */
+static int g_plant = 0;
+void oracle_set_plant(int plant) { g_plant = plant; }
+
 static void get_center(const oracle_tree* t, int i, double c[3]) {
   if (t->r3sum[i] > 0.0) {
     const double inv = 1.0 / t->r3sum[i];
@@ -311,6 +318,8 @@ void oracle_tree_forces(const oracle_tree* t, double theta, double m_const, doub
       const double s = node_size(t, node);
       double c[3];
       get_center(t, node, c);
+      if (g_plant & 4)
+        for (int a = 0; a < 3; ++a) c[a] = (t->bmin[3 * node + a] + t->bmax[3 * node + a]) * 0.5;
       const double dx = mic(c[0] - pi[0], t->L, t->periodic);
       const double dy = mic(c[1] - pi[1], t->L, t->periodic);
       const double dz = mic(c[2] - pi[2], t->L, t->periodic);
@@ -414,6 +423,8 @@ void oracle_tree_stokes(const oracle_tree* t, int mode, double theta, const doub
         double D[9];
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) D[3 * a + b] = P[3 * a + b] - cm[a] * Fv[b];
+        if (g_plant & 2)
+          for (int a = 0; a < 9; ++a) D[a] = 0.0;
         const double rx = dx, ry = dy, rz = dz;
         const double ux = D[0] * rx + D[3] * ry + D[6] * rz;
         const double uy = D[1] * rx + D[4] * ry + D[7] * rz;
@@ -421,7 +432,7 @@ void oracle_tree_stokes(const oracle_tree* t, int mode, double theta, const doub
         const double wx = D[0] * rx + D[1] * ry + D[2] * rz;
         const double wy = D[3] * rx + D[4] * ry + D[5] * rz;
         const double wz = D[6] * rx + D[7] * ry + D[8] * rz;
-        const double s_tr = D[0] + D[4] + D[8];
+        const double s_tr = (g_plant & 1) ? 0.0 : D[0] + D[4] + D[8];
         const double q = rx * wx + ry * wy + rz * wz;
         const double inv_r3 = inv_r * inv_r2, inv_r5 = inv_r3 * inv_r2;
         vx += eta * (inv_r3 * (ux - wx - s_tr * rx) + 3.0 * rx * q * inv_r5);

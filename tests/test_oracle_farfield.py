@@ -70,3 +70,28 @@ def test_cluster_is_one_accepted_node():
         np.testing.assert_allclose(t.forces(0.0, 1.0)[0], Fd[0], rtol=1e-12)
         Vd = oracle.direct_convection(pos, F, 1.0, 1.0, periodic)
         np.testing.assert_allclose(t.stokeslet(0.0, F, 1.0)[0], Vd[0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_planted_far_field_mistakes_fail_the_order_test(periodic):
+    """Mistakes planted in the oracle walks leave a first-order (or worse)
+    far-field error, which the order test above rejects; the untouched field
+    keeps its second order."""
+    hs = (0.02, 0.01, 0.005)
+
+    def ratios(plant):
+        oracle.set_plant(plant)
+        try:
+            e = np.array([far_field_errors(h, periodic) for h in hs])
+        finally:
+            oracle.set_plant(0)
+        return e[:-1] / e[1:]
+
+    second = lambda r: bool(np.all((r > 3.6) & (r < 4.5)))  # noqa: E731
+    r = ratios(1)  # trace term of the Stokeslet dipole dropped
+    assert second(r[:, 0]) and not second(r[:, 1]) and not second(r[:, 2]), r
+    r = ratios(2)  # Stokeslet dipole dropped
+    assert second(r[:, 0]) and not second(r[:, 1]) and not second(r[:, 2]), r
+    r = ratios(4)  # force monopole at the geometric node centre
+    assert not second(r[:, 0]) and second(r[:, 1]), r
+    assert second(ratios(0)), "the oracle is restored"
