@@ -121,7 +121,7 @@ struct BEns {
   long long part_off;   // doubles into the partial-sum array
   long long cnt_off;    // ints into the bucket counts (T per system)
   long long start_off;  // ints into the bucket starts (T + 1 per system)
-  long long partj_off;  // doubles into the reversed-pair partials (pair-once mode)
+  long long partj_off;  // entries into the reversed-pair partials (pair-once mode)
   int off;              // first droplet in the caller's arrays
   int roff;             // first record (regions padded to 256 records)
   int n, cap, S;        // S: source splits, or chunks of a block's partner list
@@ -160,7 +160,7 @@ struct BLayout {
   Rec* rec;
   RecC* recc;
   double* part;
-  double* part_j;  // pair-once reversed-pair partials [sum B (B/2) 3 256]
+  dem::PartJ* part_j;  // pair-once reversed-pair partials [sum B (B/2) 3 256]
   int* cnt;      // [sum T]
   int* start;    // [sum T + 1]
   int* bucket;   // [sum cap]
@@ -251,7 +251,7 @@ BLayout carve(void* ws, int n_ens, const Plan& p) {
   l.rec = c.take<Rec>(at_least1(p.rec_total));
   l.recc = c.take<RecC>(at_least1(p.rec_total));
   l.part = c.take<double>(at_least1(p.part_total));
-  l.part_j = c.take<double>(at_least1(p.partj_total));
+  l.part_j = c.take<dem::PartJ>(at_least1(p.partj_total));
   l.cnt = c.take<int>(at_least1(p.cnt_total));
   l.start = c.take<int>(at_least1(p.start_total));
   l.bucket = c.take<int>(at_least1(p.n_total));
@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(kRow) void k_b_force_reduce(const BEns* __restrict_
                                                          const BHead* __restrict__ head,
                                                          const int* __restrict__ go,
                                                          const double* __restrict__ part,
-                                                         const double* __restrict__ part_j,
+                                                         const dem::PartJ* __restrict__ part_j,
                                                          const double* __restrict__ radii,
                                                          double* __restrict__ F,
                                                          RecC* __restrict__ recc) {
@@ -1237,7 +1237,7 @@ template <bool SYM>
 __global__ __launch_bounds__(kRow) void k_b_conv_euler(
     const BEns* __restrict__ ens, const BDev* __restrict__ dev, const int2* __restrict__ rows,
     const BHead* __restrict__ head, const int* __restrict__ go, const double* __restrict__ part,
-    const double* __restrict__ part_j, const double* __restrict__ radii,
+    const dem::PartJ* __restrict__ part_j, const double* __restrict__ radii,
     const double* __restrict__ F, double* __restrict__ V,
     double* __restrict__ pos, Rec* __restrict__ rec, RecC* __restrict__ recc,
     int* __restrict__ steps_done, double* __restrict__ time) {
@@ -1289,7 +1289,7 @@ template <int TPL>
 __global__ __launch_bounds__(dem::kSymB / TPL) void k_b_force_sym(
     const BEns* __restrict__ ens, const int4* __restrict__ items, const BHead* __restrict__ head,
     const int* __restrict__ go, const Rec* __restrict__ rec, int tile, double* __restrict__ part,
-    double* __restrict__ part_j) {
+    dem::PartJ* __restrict__ part_j) {
   if ((int)blockIdx.x >= head->n_items) return;
   const int4 it = items[blockIdx.x];
   if (!go[it.x]) return;
@@ -1304,7 +1304,7 @@ template <int TPL>
 __global__ __launch_bounds__(dem::kSymB / TPL) void k_b_conv_sym(
     const BEns* __restrict__ ens, const int4* __restrict__ items, const BHead* __restrict__ head,
     const int* __restrict__ go, const RecC* __restrict__ recc, int tile, double* __restrict__ part,
-    double* __restrict__ part_j) {
+    dem::PartJ* __restrict__ part_j) {
   if ((int)blockIdx.x >= head->n_items) return;
   const int4 it = items[blockIdx.x];
   if (!go[it.x]) return;
@@ -1559,7 +1559,7 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
   DEM_CHECK_LAUNCH("k_b_force");
   hipLaunchKernelGGL(k_b_force_reduce<false>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const int*)l.go, (const double*)l.part, (const dem::PartJ*)l.part_j,
                      (const double*)b->radii, b->F, l.recc);
   DEM_CHECK_LAUNCH("k_b_force_reduce");
   if (prof) {
@@ -1570,7 +1570,7 @@ int enqueue_advance_bw(dem_batch* b, hipStream_t st, bool prof) {
   DEM_CHECK_LAUNCH("k_b_conv");
   hipLaunchKernelGGL(k_b_conv_euler<false>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const int*)l.go, (const double*)l.part, (const dem::PartJ*)l.part_j,
                      (const double*)b->radii, (const double*)b->F, b->V, b->pos, l.rec, l.recc,
                      l.steps_done, l.time);
   DEM_CHECK_LAUNCH("k_b_conv_euler");
@@ -1600,7 +1600,7 @@ int enqueue_advance_sym(dem_batch* b, hipStream_t st, bool prof) {
   DEM_CHECK_LAUNCH("k_b_force_sym");
   hipLaunchKernelGGL(k_b_force_reduce<true>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const int*)l.go, (const double*)l.part, (const dem::PartJ*)l.part_j,
                      (const double*)b->radii, b->F, l.recc);
   DEM_CHECK_LAUNCH("k_b_force_reduce");
   if (prof) {
@@ -1611,7 +1611,7 @@ int enqueue_advance_sym(dem_batch* b, hipStream_t st, bool prof) {
   DEM_CHECK_LAUNCH("k_b_conv_sym");
   hipLaunchKernelGGL(k_b_conv_euler<true>, dim3(gr), dim3(kRow), 0, st, (const BEns*)l.ens,
                      (const BDev*)l.dev, (const int2*)l.rows, (const BHead*)l.head,
-                     (const int*)l.go, (const double*)l.part, (const double*)l.part_j,
+                     (const int*)l.go, (const double*)l.part, (const dem::PartJ*)l.part_j,
                      (const double*)b->radii, (const double*)b->F, b->V, b->pos, l.rec, l.recc,
                      l.steps_done, l.time);
   DEM_CHECK_LAUNCH("k_b_conv_euler");

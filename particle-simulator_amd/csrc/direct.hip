@@ -115,7 +115,7 @@ struct Layout {
   Rec* rec;      // [npad]: records beyond n are zero (no radius, no force)
   RecC* recc;
   double* part;  // [split][6][n]
-  double* part_j;  // symmetric kernels: [B][B/2][3][kSymB]
+  dem::PartJ* part_j;  // symmetric kernels: [B][B/2][3][kSymB]
   int npad;
   size_t bytes;
 };
@@ -137,7 +137,7 @@ Layout carve(void* ws, int n) {
   l.part = c.take<double>(parts);
   if (sym_alloc(n)) {
     const size_t B = (size_t)sym_blocks(n);
-    l.part_j = c.take<double>(std::max<size_t>(B * (B / 2), 1) * 3 * kSymB);
+    l.part_j = c.take<dem::PartJ>(std::max<size_t>(B * (B / 2), 1) * 3 * kSymB);
   } else {
     l.part_j = nullptr;
   }
@@ -263,7 +263,7 @@ template <int TPL>
 __global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec, int n,
                                                            int B, int ec, int tile,
                                                            double* __restrict__ part,
-                                                           double* __restrict__ part_j) {
+                                                           dem::PartJ* __restrict__ part_j) {
   const int b = blockIdx.x, s = blockIdx.y;
   const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
   dem::force_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict
 
 // Four lanes per target (the reversed partials are most of the bytes).
 __global__ __launch_bounds__(256) void k_force_sym_reduce(
-    const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
+    const double* __restrict__ part, int S, int n, const dem::PartJ* __restrict__ part_j, int B,
     const double* __restrict__ radii, const DevScales* __restrict__ scp, double m_const,
     double* __restrict__ F, RecC* __restrict__ recc) {
   const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
@@ -350,14 +350,14 @@ template <int TPL>
 __global__ __launch_bounds__(kSymB / TPL) void k_conv_sym(const RecC* __restrict__ rec, int n,
                                                           int B, int ec, int tile,
                                                           double* __restrict__ part,
-                                                          double* __restrict__ part_j) {
+                                                          dem::PartJ* __restrict__ part_j) {
   const int b = blockIdx.x, s = blockIdx.y;
   const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
   dem::conv_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
 __global__ __launch_bounds__(256) void k_conv_sym_reduce(
-    const double* __restrict__ part, int S, int n, const double* __restrict__ part_j, int B,
+    const double* __restrict__ part, int S, int n, const dem::PartJ* __restrict__ part_j, int B,
     const DevScales* __restrict__ scp, double eta_const, const double* __restrict__ F,
     double* __restrict__ V) {
   const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
@@ -584,7 +584,7 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
                                        l.part_j));
     DEM_CHECK_LAUNCH("k_force_sym");
     hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
-                       (const double*)l.part, S, n, (const double*)l.part_j, B, radii,
+                       (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B, radii,
                        (const DevScales*)l.sc, ph->m_const, F, l.recc);
     DEM_CHECK_LAUNCH("k_force_sym_reduce");
     return DEM_OK;
@@ -623,7 +623,7 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
                                        l.part, l.part_j));
     DEM_CHECK_LAUNCH("k_conv_sym");
     hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
-                       (const double*)l.part, S, n, (const double*)l.part_j, B,
+                       (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B,
                        (const DevScales*)l.sc, ph->eta_const, F, V);
     DEM_CHECK_LAUNCH("k_conv_sym_reduce");
     return DEM_OK;

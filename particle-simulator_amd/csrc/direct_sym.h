@@ -10,6 +10,10 @@
 
 namespace dem {
 
+// Reversed-pair partial sums as stored: each is a sum of FP32 slot
+// accumulators, kept in FP32 (half the bytes of the round trip to the reduce).
+using PartJ = float;
+
 constexpr int kSymB = 256;           // droplets per block
 constexpr int kSymLds = kSymB + 64;  // source slots lane + k, k < 256, without a wrap
 constexpr int kSymTplForce = 2;      // targets per thread, force / Oseen bodies
@@ -36,9 +40,9 @@ __host__ __device__ inline int sym_entries_max(int B) { return 1 + B / 2; }
 // instruction's lanes touch distinct slots, and steps run in groups
 // k = kk + 64u (u = 0..3) whose four slot windows [k, k + 63] are disjoint, so
 // only consecutive groups share slots and wave_sync() orders them.  Per
-// (b, c) the waves' accumulators are summed in fixed order into float64
-// part_j[b][e-1][3][256]; the target side keeps float64 accumulators as the
-// one-sided kernel does.
+// (b, c) the waves' accumulators are summed in fixed order (in float64) and
+// stored as FP32 in part_j[b][e-1][3][256]; the reduce adds them in float64.
+// The target side keeps float64 accumulators as the one-sided kernel does.
 
 // Orders one wave's LDS accesses across its lanes: the compiler keeps program
 // order across it, and the LDS executes a wave's instructions in order.
@@ -66,7 +70,7 @@ __device__ __forceinline__ void sym_zero(float (*acc)[kSymLds], int lane) {
 // Reversed-pair sums of the partner block's sources t + T h, all W waves.
 template <int T, int W>
 __device__ __forceinline__ void sym_fold(const float (*acc)[3][kSymLds], int t,
-                                         double* __restrict__ dst) {
+                                         PartJ* __restrict__ dst) {
   for (int slot = t; slot < kSymB; slot += T) {
     for (int a = 0; a < 3; ++a) {
       double v = 0.0;
@@ -74,7 +78,7 @@ __device__ __forceinline__ void sym_fold(const float (*acc)[3][kSymLds], int t,
         v += (double)acc[w][a][slot];
         if (slot < 64) v += (double)acc[w][a][slot + kSymB];
       }
-      dst[a * kSymB + slot] = v;
+      dst[a * kSymB + slot] = (PartJ)v;
     }
   }
 }
@@ -116,7 +120,7 @@ template <int TPL>
 __device__ __forceinline__ void force_sym_block(const Rec* __restrict__ rec, int n, int B, int b,
                                                 int s, int e0, int e1, int tile,
                                                 double* __restrict__ part,
-                                                double* __restrict__ part_j) {
+                                                PartJ* __restrict__ part_j) {
   constexpr int T = kSymB / TPL, W = T / 64;
   __shared__ uint32_t sx[kSymLds], sy[kSymLds], sz[kSymLds];
   __shared__ float sr[kSymLds], sr3[kSymLds];  // radius in fixed units, r^3 in work units
@@ -236,7 +240,7 @@ template <int TPL>
 __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int n, int B, int b,
                                                int s, int e0, int e1, int tile,
                                                double* __restrict__ part,
-                                               double* __restrict__ part_j) {
+                                               PartJ* __restrict__ part_j) {
   constexpr int T = kSymB / TPL, W = T / 64;
   __shared__ unsigned long long sx[kSymLds], sy[kSymLds], sz[kSymLds];
   __shared__ float sfx[kSymLds], sfy[kSymLds], sfz[kSymLds];
@@ -332,7 +336,7 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
 // Reversed-pair sums of target i from the blocks b = bi - e (mod B) that
 // have block bi as their e-th partner (e <= P_b), over e = k + 1, k + 5, ...
 // (one of four interleaved streams, k = 0..3), in ascending e.
-__device__ __forceinline__ void sym_reversed(const double* __restrict__ part_j, int B, int i, int k,
+__device__ __forceinline__ void sym_reversed(const PartJ* __restrict__ part_j, int B, int i, int k,
                                              double* r) {
   const int bi = i / kSymB, il = i % kSymB, Pmax = B / 2;
   r[0] = r[1] = r[2] = 0.0;
@@ -340,7 +344,7 @@ __device__ __forceinline__ void sym_reversed(const double* __restrict__ part_j, 
   for (int e = k + 1; e <= Pmax; e += 4) {
     const int b = bi >= e ? bi - e : bi - e + B;
     if (e > sym_partners(b, B)) continue;
-    const double* q = part_j + ((size_t)b * Pmax + (e - 1)) * 3 * kSymB + il;
+    const PartJ* q = part_j + ((size_t)b * Pmax + (e - 1)) * 3 * kSymB + il;
     r[0] += q[0];
     r[1] += q[kSymB];
     r[2] += q[2 * kSymB];
@@ -352,7 +356,7 @@ __device__ __forceinline__ void sym_reversed(const double* __restrict__ part_j, 
 // sym_reversed.  The four-lane reduce kernels form the same sums in the same
 // order, so every caller gets identical bits.
 __device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, int n,
-                                        const double* __restrict__ part_j, int B, int i,
+                                        const PartJ* __restrict__ part_j, int B, int i,
                                         double* sum) {
   for (int s = 0; s < S; ++s)
     for (int a = 0; a < 3; ++a) sum[a] += part[(size_t)(3 * s + a) * n + i];
@@ -364,7 +368,7 @@ __device__ __forceinline__ void sym_sum(const double* __restrict__ part, int S, 
 // sym_sum with the four streams on four consecutive lanes (lane k = lane & 3);
 // the result is valid on lane k = 0.
 __device__ __forceinline__ void sym_sum4(const double* __restrict__ part, int S, int n,
-                                         const double* __restrict__ part_j, int B, int i,
+                                         const PartJ* __restrict__ part_j, int B, int i,
                                          double* sum) {
   const int k = threadIdx.x & 3;
   double r[3];
