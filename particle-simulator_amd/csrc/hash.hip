@@ -379,6 +379,16 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
         const bool mine = ((remaining >> lane) & 1ull) && kx == gx && ky == gy && kz == gz;
         const unsigned long long members = __ballot(mine);
         remaining &= ~members;
+        // A group whose cell lies at least two cells from the periodic seam
+        // on every axis (g >= 7) tests contacts without the minimum image: a
+        // droplet in contact with it lies within two cells on the ring
+        // (|d| <= r_i + r_j <= cell per axis), so on the same side of the
+        // seam, at |d_a| <= cell < L/2 where the image rule changes nothing;
+        // any other candidate is farther apart unwrapped than wrapped, hence
+        // out of contact either way.  Same decisions, fewer instructions.
+        const bool inner = periodic && g >= 7 && gx >= 2 && gx <= g - 3 && gy >= 2 &&
+                           gy <= g - 3 && gz >= 2 && gz <= g - 3;
+        const int per_g = inner ? 0 : periodic;
         // candidates: the slots of the group's neighbour buckets, candidate
         // m in neighbour k for before[k] <= m < before[k] + len[k]
         const int nh = neighbour_bucket(gx, gy, gz, g, periodic, mask);
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
             while (mm) {
               const int t = __ffsll((long long)mm) - 1;
               mm &= mm - 1;
-              if (contact_pair(S.x[t], S.y[t], S.z[t], S.r[t], q.x, q.y, q.z, q.w, L, periodic)) {
+              if (contact_pair(S.x[t], S.y[t], S.z[t], S.r[t], q.x, q.y, q.z, q.w, L, per_g)) {
                 if (j < 0) j = sorted[slot[u]];
                 if (S.id[t] < j) {
                   if (first < 0) {  // counted only in the first neighbour with this bucket
