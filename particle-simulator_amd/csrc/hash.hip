@@ -69,6 +69,7 @@ inline int tiles(int n) { return (n + kTile - 1) / kTile; }
 
 struct Layout {
   HashParams* hp;
+  int* oob;           // 1: a coordinate lies outside [0, L] (periodic boxes)
   double* rmax_part;  // [kMaxRBlocks]
   int* bucket;        // [n] bucket of each droplet
   int* counts;        // [T] droplets per bucket
@@ -91,6 +92,7 @@ Layout carve(void* ws, int n, int T) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   const size_t tt = (size_t)T;
   l.hp = c.take<HashParams>(1);
+  l.oob = c.take<int>(1);
   l.rmax_part = c.take<double>(kMaxRBlocks);
   l.bucket = c.take<int>(nn);
   l.counts = c.take<int>(tt);
@@ -166,7 +168,8 @@ __device__ __forceinline__ int tile_prefix(const int* __restrict__ tile_sums, in
 __global__ __launch_bounds__(kThreads) void k_hash_prep(const double* __restrict__ r, int n, int nrb,
                                                         int* __restrict__ counts,
                                                         int* __restrict__ fill, int T,
-                                                        double* __restrict__ rmax_part) {
+                                                        double* __restrict__ rmax_part, int* __restrict__ oob) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *oob = 0;
   for (int h = blockIdx.x * kThreads + threadIdx.x; h < T; h += gridDim.x * kThreads) {
     counts[h] = 0;
     fill[h] = 0;
@@ -191,7 +194,8 @@ __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restri
                                                           const double* __restrict__ rmax_part,
                                                           int nrb, HashParams* __restrict__ hpo,
                                                           int* __restrict__ bucket,
-                                                          int* __restrict__ counts) {
+                                                          int* __restrict__ counts,
+                                                          int* __restrict__ oob) {
   __shared__ HashParams shp;
   if (threadIdx.x < 64) {
     double m = -INFINITY;
@@ -212,6 +216,12 @@ __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restri
   __syncthreads();
   const HashParams hp = shp;
   const int i = blockIdx.x * kThreads + threadIdx.x;
+  bool out = false;  // (NaN counts as outside)
+  if (periodic && i < n) {
+    const double* p = pos + 3 * (size_t)i;
+    out = !(p[0] >= 0.0 && p[0] <= L && p[1] >= 0.0 && p[1] <= L && p[2] >= 0.0 && p[2] <= L);
+  }
+  if (__syncthreads_or(out) && threadIdx.x == 0) atomicOr(oob, 1);
   if (i >= n || hp.skip) return;
   const int ix = cell_of(pos[3 * (size_t)i], hp.cell, hp.g, periodic);
   const int iy = cell_of(pos[3 * (size_t)i + 1], hp.cell, hp.g, periodic);
@@ -335,7 +345,7 @@ struct SearchLds {
 // of the group.  Cross-lane values move by readlane (scalar) where the source
 // lane is uniform.
 __global__ __launch_bounds__(kThreads) void k_hash_search(
-    int T, double L, int periodic, int mask, const HashParams* __restrict__ hpp,
+    int T, double L, int periodic, int mask, const HashParams* __restrict__ hpp, const int* __restrict__ oob,
     const int* __restrict__ counts, const int* __restrict__ offsets,
     const int* __restrict__ sorted, const double4* __restrict__ spos, int* __restrict__ pair_cnt,
     int* __restrict__ stash) {
@@ -345,6 +355,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
   const HashParams hp = *hpp;
   if (hp.skip) return;
   const int g = hp.g;
+  const bool in_box = *oob == 0;
   const int b0 = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * kBucketsPerWave;
   int my_c = 0, my_o = 0;
   if (lane < kBucketsPerWave && b0 + lane < T) {
@@ -379,15 +390,17 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
         const bool mine = ((remaining >> lane) & 1ull) && kx == gx && ky == gy && kz == gz;
         const unsigned long long members = __ballot(mine);
         remaining &= ~members;
-        // A group whose cell lies at least two cells from the periodic seam
-        // on every axis (g >= 7) tests contacts without the minimum image: a
-        // droplet in contact with it lies within two cells on the ring
-        // (|d| <= r_i + r_j <= cell per axis), so on the same side of the
-        // seam, at |d_a| <= cell < L/2 where the image rule changes nothing;
-        // any other candidate is farther apart unwrapped than wrapped, hence
+        // With every droplet in [0, L]^3, a group whose cell index lies in
+        // [2, g - 4] on every axis (g >= 7) tests contacts without the
+        // minimum image.  A droplet in contact with it is within r_i + r_j
+        // <= cell (up to rounding) per axis after the image rule; the group's
+        // droplets lie in [2 cell, (g - 3) cell] (up to rounding) and the box
+        // edge L > (g - 1) cell, so that droplet is on the same side of the
+        // seam, at |d_a| <= cell < L/2, where the rule changes nothing.  Any
+        // other candidate is at least as far apart unwrapped as wrapped, so
         // out of contact either way.  Same decisions, fewer instructions.
-        const bool inner = periodic && g >= 7 && gx >= 2 && gx <= g - 3 && gy >= 2 &&
-                           gy <= g - 3 && gz >= 2 && gz <= g - 3;
+        const bool inner = periodic && in_box && g >= 7 && gx >= 2 && gx <= g - 4 &&
+                           gy >= 2 && gy <= g - 4 && gz >= 2 && gz <= g - 4;
         const int per_g = inner ? 0 : periodic;
         // candidates: the slots of the group's neighbour buckets, candidate
         // m in neighbour k for before[k] <= m < before[k] + len[k]
@@ -600,10 +613,10 @@ int run_hash(const Layout& l, const double* pos, const double* radii, int n, dou
   if (prep_blocks > 2048) prep_blocks = 2048;
   if (prep_blocks < nrb) prep_blocks = nrb;
   hipLaunchKernelGGL(k_hash_prep, dim3(prep_blocks), dim3(kThreads), 0, st, radii, n, nrb,
-                     l.counts, l.fill, T, l.rmax_part);
+                     l.counts, l.fill, T, l.rmax_part, l.oob);
   DEM_CHECK_LAUNCH("k_hash_prep");
   hipLaunchKernelGGL(k_hash_assign, dim3(nb), dim3(kThreads), 0, st, pos, n, per, mask, L,
-                     (const double*)l.rmax_part, nrb, l.hp, l.bucket, l.counts);
+                     (const double*)l.rmax_part, nrb, l.hp, l.bucket, l.counts, l.oob);
   DEM_CHECK_LAUNCH("k_hash_assign");
   const int ntT = tiles(T);
   hipLaunchKernelGGL(k_tile_scan, dim3(ntT), dim3(kThreads), 0, st, (const int*)l.counts, l.local,
@@ -619,7 +632,8 @@ int run_hash(const Layout& l, const double* pos, const double* radii, int n, dou
   const int waves = (T + kBucketsPerWave - 1) / kBucketsPerWave;
   const int se_blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
   hipLaunchKernelGGL(k_hash_search, dim3(se_blocks), dim3(kThreads), 0, st, T, L, per, mask,
-                     (const HashParams*)l.hp, (const int*)l.counts, (const int*)l.offsets,
+                     (const HashParams*)l.hp, (const int*)l.oob, (const int*)l.counts,
+                     (const int*)l.offsets,
                      (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt, l.stash);
   DEM_CHECK_LAUNCH("k_hash_search");
   const int ntn = tiles(n);

@@ -80,6 +80,31 @@ def test_dense_random_sets_bit_exact(gpu, periodic, seed):
     assert len(ref) > 50
 
 
+def test_positions_outside_the_box_bit_exact(gpu):
+    """Periodic pass with 30 % of the droplets shifted by +-L along one axis
+    (unwrapped positions, as a caller may pass them).  L / cell = 16 exactly,
+    so a shifted droplet keeps its cell and bucket and the reference's single
+    image correction (|d| > L/2: d -= sign(d) L) brings it into contact with
+    its neighbours; the search's shortcut for groups away from the seam
+    (no minimum image) is valid only for positions inside the box and must
+    stand aside here."""
+    rng = np.random.default_rng(11)
+    n, L = 3000, 1.0
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(0.005, 0.03125, size=n)
+    r[0] = 0.03125  # cell = 2 r_max = 1/16
+    sel = rng.random(n) < 0.3
+    axis = rng.integers(0, 3, size=n)
+    sign = rng.choice([-1.0, 1.0], size=n)
+    pos[sel, axis[sel]] += sign[sel] * L
+    det = SpatialHashCollisionDetector(num_particles=n, L=L, boundary_mode="periodic")
+    is_coll, pairs = det.detect(pos, r)
+    ok, ref = oracle.hash_pairs(pos, r, L, True, _next_power_of_2(2 * n))
+    assert is_coll is ok and np.array_equal(pairs, ref) and len(ref) > 100
+    # the shifted droplets take part in contacts
+    assert np.any(sel[ref[:, 0]] | sel[ref[:, 1]])
+
+
 @pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("capacity", [16, 200])
 def test_aliased_buckets_bit_exact(gpu, periodic, capacity):
