@@ -131,3 +131,6 @@ def test_batch_workspace_and_argument_checks():
     assert rc == _native.DEM_ERR_WORKSPACE
     assert out.value is None
     assert lib.dem_batch_destroy(None) == 0
+    # the incremental re-check switch on a null handle
+    rc = lib.dem_batch_set_incremental(None, 1)
+    assert rc == _native.DEM_ERR_INVALID and "null handle" in _native.last_error()
