@@ -152,6 +152,19 @@ int dem_direct_set_tuning(int split, int tile);
  * the previous mode.  No counterpart in the reference (a tuning switch). */
 int dem_direct_set_symmetric(int on);
 
+/* Separations of the pair-once Oseen sum in mode 1 of
+ * dem_direct_set_symmetric: on (default) takes them from 32-bit high words of
+ * the positions -- exact in an open box; in a periodic box the pairs that lie
+ * within one unit of half the box on some axis (where the 32-bit difference
+ * can name the other periodic image) are found by an exact-match lookup of
+ * the high words during dem_direct_forces_and_convection and their terms
+ * replaced by the 64-bit ones, and a call where such a word repeats falls back
+ * to 64-bit separations; dem_direct_convection alone keeps 64-bit separations
+ * in a periodic box.  off: 64-bit separations, the kernels of mode 2.
+ * Returns the previous setting.  No counterpart in the reference (a tuning
+ * switch of this build's kernels). */
+int dem_direct_set_image32(int on);
+
 /* ------------------------------------------------------------------------ */
 /* Spatial-hash contact detection                                            */
 /* ------------------------------------------------------------------------ */

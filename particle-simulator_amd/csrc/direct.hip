@@ -29,6 +29,7 @@
 //   * Inside a split the FP32 sum runs over `tile` sources and is then added to
 //     a float64 accumulator (blocked summation: the FP32 rounding error grows
 //     with the tile length, not with N).
+#include <limits.h>
 #include <math.h>
 
 #include <algorithm>
@@ -50,6 +51,8 @@ int g_split = 0;  // 0: heuristic
 int g_tile = 0;   // 0: heuristic
 int g_sym = 1;    // 0: one-sided only; 1: pair-once for kSymMin <= n <= kSymMax without an
                   // explicit split; 2: pair-once for every n, g_split giving the chunk count
+int g_img32 = 1;  // mode 1: the pair-once Oseen sum on 32-bit separations (boundary pairs
+                  // corrected); 0: on 64-bit separations as in mode 2
 
 // Symmetric (pair-once) evaluation: blocks of kSymB droplets; block b pairs
 // with the next sym_partners(b, B) blocks cyclically, which covers every
@@ -116,9 +119,21 @@ struct Layout {
   RecC* recc;
   double* part;  // [split][6][n]
   dem::PartJ* part_j;  // symmetric kernels: [B][B/2][3][kSymB]
+  // boundary pairs of the 32-bit Oseen separations (pair-once, periodic)
+  unsigned long long* img_slot;  // [3][1 << img_log2m] high-word hash tables
+  int* img_partner;              // [n][3]: partner across half the ring, -1 none
+  int* img_flag;                 // nonzero: a duplicated high word has a partner
+  int img_log2m;
   int npad;
   size_t bytes;
 };
+
+// Slots per axis of the boundary-pair tables: a power of two >= 4 n.
+inline int img_log2m(int n) {
+  int m = 10;
+  while (m < 30 && (1ll << m) < 4ll * n) ++m;
+  return m;
+}
 
 Layout carve(void* ws, int n) {
   dem::Carver c(ws);
@@ -140,6 +155,16 @@ Layout carve(void* ws, int n) {
     l.part_j = c.take<dem::PartJ>(std::max<size_t>(B * (B / 2), 1) * 3 * kSymB);
   } else {
     l.part_j = nullptr;
+  }
+  l.img_log2m = img_log2m(nn);
+  if (sym_fits(n)) {
+    l.img_slot = c.take<unsigned long long>((size_t)3 << l.img_log2m);
+    l.img_partner = c.take<int>((size_t)3 * nn);
+    l.img_flag = c.take<int>(1);
+  } else {
+    l.img_slot = nullptr;
+    l.img_partner = nullptr;
+    l.img_flag = nullptr;
   }
   l.bytes = dem::align_up(c.used, 256);
   return l;
@@ -197,13 +222,21 @@ __global__ void k_setup(const double* __restrict__ bbox, int nblk, double L, int
   *sc = dem::open_scales(lo, ext, m_const);
 }
 
-// Quantise positions, scale radii, and (optionally) load caller forces as sources.
+// Quantise positions, scale radii, and (optionally) load caller forces as
+// sources; clear the boundary-pair tables (img_slot, img_words words) and flag
+// when the call uses them.
 __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
                                               const double* __restrict__ radii,
                                               const double* __restrict__ forces, int n, int npad,
                                               const DevScales* __restrict__ scp,
-                                              Rec* __restrict__ rec, RecC* __restrict__ recc) {
+                                              Rec* __restrict__ rec, RecC* __restrict__ recc,
+                                              unsigned long long* __restrict__ img_slot,
+                                              size_t img_words, int* __restrict__ img_flag) {
   int i = blockIdx.x * 256 + threadIdx.x;
+  if (img_slot) {
+    for (size_t u = i; u < img_words; u += (size_t)gridDim.x * 256) img_slot[u] = 0ull;
+    if (i == 0) *img_flag = 0;
+  }
   if (i >= n) {
     if (i < npad) {  // padding: no radius, no force, at the origin
       rec[i] = Rec{};
@@ -217,6 +250,127 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
   dem::pack_one(pos + 3 * (size_t)i, radii[i], forces ? forces + 3 * (size_t)i : nullptr, sc, o, c);
   rec[i] = o;
   recc[i] = c;
+}
+
+// ---------------------------------------------------------------- boundary pairs
+//
+// The pair-once Oseen kernel may take separations from the 32-bit high words
+// h = q >> 32 of the 64-bit ring positions q.  Then c = h_j - h_i (mod 2^32)
+// is hi(q_j - q_i) plus the borrow of the low words, and its sign -- the
+// periodic image -- differs from the 64-bit difference's only when
+// c == 2^31 exactly on some axis, where hi(q_j - q_i) may be 2^31 - 1 (the
+// pair lies within one unit of half the box).  Those pairs are found by an
+// exact-match join: per axis, an open-addressing table keyed by h (each slot
+// (h << 32) | (i + 1), 0 empty, bit 31 marking a key that several droplets
+// share) is probed for h ^ 2^31.  A droplet has at most one partner per axis
+// unless a key repeats; a repeated key with a partner makes the call fall back
+// to 64-bit separations (img_flag).  The reduce then replaces, for each
+// partner, the term the kernel added with the 64-bit one.
+
+constexpr unsigned long long kImgMulti = 1ull << 31;
+
+__device__ __forceinline__ uint32_t img_hash(uint32_t key, int log2m) {
+  return (key * 0x9E3779B1u) >> (32 - log2m);
+}
+
+__device__ __forceinline__ void img_insert(unsigned long long* __restrict__ tab, int log2m,
+                                           uint32_t key, int i) {
+  const uint32_t mask = (1u << log2m) - 1u;
+  const unsigned long long val = ((unsigned long long)key << 32) | (unsigned long long)(i + 1);
+  uint32_t slot = img_hash(key, log2m);
+  for (uint32_t it = 0; it <= mask; ++it) {
+    const unsigned long long old = atomicCAS(tab + slot, 0ull, val);
+    if (old == 0ull) return;
+    if ((uint32_t)(old >> 32) == key) {
+      atomicOr(tab + slot, kImgMulti);
+      return;
+    }
+    slot = (slot + 1u) & mask;
+  }
+}
+
+// Droplet whose high word is key ^ 2^31: its index, -1 for none, -2 when
+// several droplets share that word.
+__device__ __forceinline__ int img_probe(const unsigned long long* __restrict__ tab, int log2m,
+                                         uint32_t key) {
+  const uint32_t mask = (1u << log2m) - 1u, want = key ^ 0x80000000u;
+  uint32_t slot = img_hash(want, log2m);
+  for (uint32_t it = 0; it <= mask; ++it) {
+    const unsigned long long v = tab[slot];
+    if (v == 0ull) return -1;
+    if ((uint32_t)(v >> 32) == want) return (v & kImgMulti) ? -2 : (int)(v & 0x7fffffffull) - 1;
+    slot = (slot + 1u) & mask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ uint32_t rec_hi(const RecC& r, int a) {
+  return a == 0 ? r.xh : (a == 1 ? r.yh : r.zh);
+}
+__device__ __forceinline__ uint32_t rec_lo(const RecC& r, int a) {
+  return a == 0 ? r.xl : (a == 1 ? r.yl : r.zl);
+}
+
+// Oseen term F/|d| + d (d.F)/|d|^3 in float64 (fixed units, as the pair
+// kernels accumulate it); nothing for d == 0.
+__device__ __forceinline__ void oseen_term64(const double* d, const double* f, double* out) {
+#pragma clang fp contract(off)
+  const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (!(r2 > 0.0)) {
+    out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  const double ir = 1.0 / sqrt(r2);
+  const double t = (d[0] * f[0] + d[1] * f[1] + d[2] * f[2]) * (ir * ir);
+  for (int a = 0; a < 3; ++a) out[a] = ir * (f[a] + t * d[a]);
+}
+
+// Correction of droplet i's Oseen sum for its boundary partners (at most one
+// per axis, taken in ascending index): the 64-bit term minus the term the
+// 32-bit pair kernel added.  The kernel evaluates each pair from its target:
+// droplet i is the target when j lies in its own block or in one of its
+// block's partner blocks, else j is, and i received G(d_ji) F_j.
+__device__ __forceinline__ void img_correct(const RecC* __restrict__ rec, int n, int B, int i,
+                                            const int* __restrict__ partner, double* m) {
+  // the three partners in ascending order, INT_MAX for none
+  int js[3];
+  for (int a = 0; a < 3; ++a) {
+    const int j = partner[3 * (size_t)i + a];
+    js[a] = (j < 0 || j >= n || j == i) ? INT_MAX : j;
+  }
+  auto order = [](int& u, int& v) {
+    const int lo = min(u, v), hi = max(u, v);
+    u = lo;
+    v = hi;
+  };
+  order(js[0], js[1]);
+  order(js[1], js[2]);
+  order(js[0], js[1]);
+  if (js[0] == INT_MAX) return;
+  const RecC ri = rec[i];
+  const int bi = i / kSymB;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int j = js[k];
+    if (j == INT_MAX || (k > 0 && j == js[k - 1])) continue;  // none, or found on two axes
+    const RecC rj = rec[j];
+    const int bj = j / kSymB;
+    const int e = (bj - bi + B) % B;
+    const bool i_target = bi == bj || (e >= 1 && e <= sym_partners(bi, B));
+    double d32[3], d64[3];
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t hi = rec_hi(ri, a), hj = rec_hi(rj, a);
+      d32[a] = (double)dem::diff32(i_target ? hj : hi, i_target ? hi : hj);
+      const unsigned long long qi = ((unsigned long long)hi << 32) | rec_lo(ri, a);
+      const unsigned long long qj = ((unsigned long long)hj << 32) | rec_lo(rj, a);
+      d64[a] = (double)dem::diff64n(qj, 0ull - qi);
+    }
+    const double f[3] = {(double)rj.fx, (double)rj.fy, (double)rj.fz};
+    double t64[3], t32[3];
+    oseen_term64(d64, f, t64);
+    oseen_term64(d32, f, t32);
+    for (int a = 0; a < 3; ++a) m[a] += t64[a] - t32[a];
+  }
 }
 
 // ---------------------------------------------------------------- phase 1
@@ -259,26 +413,57 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
 
 // ---------------------------------------------------------------- phase 1, pair-once
 
+// Boundary-pair tables of one call (null img_slot: not used).
+struct ImgArgs {
+  unsigned long long* slot;
+  int* partner;
+  int* flag;
+  int log2m;
+};
+
 template <int TPL>
-__global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec, int n,
+__global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec,
+                                                           const RecC* __restrict__ recc, int n,
                                                            int B, int ec, int tile,
                                                            double* __restrict__ part,
-                                                           dem::PartJ* __restrict__ part_j) {
+                                                           dem::PartJ* __restrict__ part_j,
+                                                           ImgArgs img) {
   const int b = blockIdx.x, s = blockIdx.y;
+  if (img.slot) {
+    // the S workgroups of block b share its 3 x 256 (droplet, axis) keys
+    for (int u = s + (int)gridDim.y * (int)threadIdx.x; u < 3 * kSymB;
+         u += (int)gridDim.y * (int)blockDim.x) {
+      const int i = b * kSymB + u / 3, a = u % 3;
+      if (i < n)
+        img_insert(img.slot + ((size_t)a << img.log2m), img.log2m, rec_hi(recc[i], a), i);
+    }
+  }
   const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
   dem::force_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
-// Four lanes per target (the reversed partials are most of the bytes).
+// Four lanes per target (the reversed partials are most of the bytes); with
+// boundary-pair tables, lanes 1..3 then look up the target's partner across
+// half the ring on axes 0..2.
 __global__ __launch_bounds__(256) void k_force_sym_reduce(
     const double* __restrict__ part, int S, int n, const dem::PartJ* __restrict__ part_j, int B,
     const double* __restrict__ radii, const DevScales* __restrict__ scp, double m_const,
-    double* __restrict__ F, RecC* __restrict__ recc) {
+    double* __restrict__ F, RecC* __restrict__ recc, ImgArgs img) {
   const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
   if (i >= n) return;  // the four lanes of a target leave together
   double sum[3] = {0.0, 0.0, 0.0};
   dem::sym_sum4(part, S, n, part_j, B, i, sum);
-  if (threadIdx.x & 3) return;
+  const int k = threadIdx.x & 3;
+  if (k) {
+    if (img.slot) {
+      const int a = k - 1;
+      const int j = img_probe(img.slot + ((size_t)a << img.log2m), img.log2m,
+                              rec_hi(recc[i], a));
+      img.partner[3 * (size_t)i + a] = j < 0 ? -1 : j;
+      if (j == -2) atomicOr(img.flag, 1);
+    }
+    return;
+  }
   const DevScales sc = *scp;
   double Fi[3];
   float fr[3];
@@ -346,26 +531,39 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
   for (int a = 0; a < 3; ++a) V[3 * (size_t)i + a] = Vi[a];
 }
 
-template <int TPL>
+// Separations of the pair-once Oseen sum: 64-bit, 32-bit (open boxes, where
+// they are exact), or 32-bit unless the boundary-pair lookup fell back.
+enum ConvPos { kPos64 = 0, kPos32 = 1, kPos32Checked = 2 };
+
+// POS32: 32-bit separations in a 32-bit LDS layout (more workgroups per
+// compute unit).  With a flag, the kernel runs only when (*flag != 0) equals
+// run_if: the 32-bit kernel while the boundary-pair lookup holds, the 64-bit
+// one as its fallback (both are launched; the other leaves at once).
+template <int TPL, bool POS32>
 __global__ __launch_bounds__(kSymB / TPL) void k_conv_sym(const RecC* __restrict__ rec, int n,
                                                           int B, int ec, int tile,
                                                           double* __restrict__ part,
-                                                          dem::PartJ* __restrict__ part_j) {
+                                                          dem::PartJ* __restrict__ part_j,
+                                                          const int* __restrict__ flag,
+                                                          int run_if) {
+  if (flag && ((*flag != 0) != (run_if != 0))) return;
+  __shared__ dem::ConvSymLds<kSymB / TPL / 64, POS32> L;
   const int b = blockIdx.x, s = blockIdx.y;
   const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
-  dem::conv_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
+  dem::conv_sym_block<TPL, POS32>(L, rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
 __global__ __launch_bounds__(256) void k_conv_sym_reduce(
     const double* __restrict__ part, int S, int n, const dem::PartJ* __restrict__ part_j, int B,
     const DevScales* __restrict__ scp, double eta_const, const double* __restrict__ F,
-    double* __restrict__ V) {
+    double* __restrict__ V, const RecC* __restrict__ recc, int mode, ImgArgs img) {
   const int i = blockIdx.x * 64 + (threadIdx.x >> 2);
   if (i >= n) return;  // the four lanes of a target leave together
   double m[3] = {0.0, 0.0, 0.0};
   const double c[3] = {0.0, 0.0, 0.0};
   dem::sym_sum4(part, S, n, part_j, B, i, m);
   if (threadIdx.x & 3) return;
+  if (mode == kPos32Checked && *img.flag == 0) img_correct(recc, n, B, i, img.partner, m);
   const DevScales sc = *scp;
   const dem::CorrScalars cs{};
   double Vi[3];
@@ -519,7 +717,7 @@ int check_corr(const dem_correction* c) {
 
 // Set-up + pack (shared by all FP32 entry points).
 int prepare(const Layout& l, const double* pos, const double* radii, const double* forces, int n,
-            const dem_phys* ph, hipStream_t st) {
+            const dem_phys* ph, hipStream_t st, bool img = false) {
   int nblk = 1;
   if (!ph->periodic) {
     nblk = (n + 255) / 256;
@@ -532,7 +730,8 @@ int prepare(const Layout& l, const double* pos, const double* radii, const doubl
                      ph->m_const, l.sc);
   DEM_CHECK_LAUNCH("k_setup");
   hipLaunchKernelGGL(k_pack, dim3((l.npad + 255) / 256), dim3(256), 0, st, pos, radii, forces, n,
-                     l.npad, l.sc, l.rec, l.recc);
+                     l.npad, l.sc, l.rec, l.recc, img ? l.img_slot : nullptr,
+                     img ? ((size_t)3 << l.img_log2m) : (size_t)0, l.img_flag);
   DEM_CHECK_LAUNCH("k_pack");
   return DEM_OK;
 }
@@ -569,8 +768,26 @@ int resident_slots(int* cache, const void* kernel, int threads) {
   return cache[dev];
 }
 
+// The boundary-pair lookup runs with the pair-once kernels of mode 1 in a
+// periodic box when one call evaluates both phases.
+bool img_applies(const Layout& l, int n, const dem_phys* ph) {
+  return g_sym == 1 && g_img32 && use_sym(n) && ph->periodic && l.img_slot != nullptr;
+}
+
+ImgArgs img_args(const Layout& l, bool on) {
+  if (!on) return ImgArgs{nullptr, nullptr, nullptr, 0};
+  return ImgArgs{l.img_slot, l.img_partner, l.img_flag, l.img_log2m};
+}
+
+// Separations of the pair-once Oseen kernel for this call.
+int conv_mode(const dem_phys* ph, bool img) {
+  if (g_sym != 1 || !g_img32) return kPos64;
+  if (!ph->periodic) return kPos32;
+  return img ? kPos32Checked : kPos64;
+}
+
 int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, double* F,
-               hipStream_t st) {
+               hipStream_t st, ImgArgs img = ImgArgs{nullptr, nullptr, nullptr, 0}) {
   if (use_sym(n)) {
     static int slots[kMaxDevices];
     const int B = sym_blocks(n);
@@ -580,12 +797,12 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
     DEM_PROF_LAUNCH("k_force_sym", st,
                     hipLaunchKernelGGL(k_force_sym<kSymTplForce>, dim3(B, S),
                                        dim3(kSymB / kSymTplForce), 0, st,
-                                       (const Rec*)l.rec, n, B, ec, choose_tile_force(), l.part,
-                                       l.part_j));
+                                       (const Rec*)l.rec, (const RecC*)l.recc, n, B, ec,
+                                       choose_tile_force(), l.part, l.part_j, img));
     DEM_CHECK_LAUNCH("k_force_sym");
     hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B, radii,
-                       (const DevScales*)l.sc, ph->m_const, F, l.recc);
+                       (const DevScales*)l.sc, ph->m_const, F, l.recc, img);
     DEM_CHECK_LAUNCH("k_force_sym_reduce");
     return DEM_OK;
   }
@@ -603,7 +820,8 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
 }
 
 int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* corr,
-             const double* F, double* V, hipStream_t st) {
+             const double* F, double* V, hipStream_t st, int mode = kPos64,
+             ImgArgs img = ImgArgs{nullptr, nullptr, nullptr, 0}) {
   const int S = choose_split(n);
   const int jchunk = (n + S - 1) / S;
   dim3 grid((n + WG - 1) / WG, S);
@@ -611,20 +829,30 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
   dem::CorrScalars cs{};
   const bool on = corr && corr->enabled;
   if (!on && use_sym(n)) {
-    static int slots[kMaxDevices];
+    static int slots32[kMaxDevices], slots64[kMaxDevices];
     const int B = sym_blocks(n);
-    const int S = sym_split(n, resident_slots(slots, (const void*)k_conv_sym<kSymTplConv>,
-                                              kSymB / kSymTplConv));
+    const bool p32 = mode != kPos64;
+    const int S = p32 ? sym_split(n, resident_slots(slots32, (const void*)k_conv_sym<kSymTplConv, true>,
+                                                    kSymB / kSymTplConv))
+                      : sym_split(n, resident_slots(slots64, (const void*)k_conv_sym<kSymTplConv, false>,
+                                                    kSymB / kSymTplConv));
     const int ec = (1 + B / 2 + S - 1) / S;
-    DEM_PROF_LAUNCH("k_conv_sym", st,
-                    hipLaunchKernelGGL(k_conv_sym<kSymTplConv>, dim3(B, S),
-                                       dim3(kSymB / kSymTplConv), 0, st,
-                                       (const RecC*)l.recc, n, B, ec, choose_tile_conv(),
-                                       l.part, l.part_j));
+    const int* flag = mode == kPos32Checked ? (const int*)img.flag : nullptr;
+    if (p32)
+      DEM_PROF_LAUNCH("k_conv_sym", st,
+                      hipLaunchKernelGGL((k_conv_sym<kSymTplConv, true>), dim3(B, S),
+                                         dim3(kSymB / kSymTplConv), 0, st, (const RecC*)l.recc, n,
+                                         B, ec, choose_tile_conv(), l.part, l.part_j, flag, 0));
+    if (!p32 || flag)
+      DEM_PROF_LAUNCH(p32 ? "k_conv_sym_fallback" : "k_conv_sym", st,
+                      hipLaunchKernelGGL((k_conv_sym<kSymTplConv, false>), dim3(B, S),
+                                         dim3(kSymB / kSymTplConv), 0, st, (const RecC*)l.recc, n,
+                                         B, ec, choose_tile_conv(), l.part, l.part_j, flag, 1));
     DEM_CHECK_LAUNCH("k_conv_sym");
     hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B,
-                       (const DevScales*)l.sc, ph->eta_const, F, V);
+                       (const DevScales*)l.sc, ph->eta_const, F, V, (const RecC*)l.recc, mode,
+                       img);
     DEM_CHECK_LAUNCH("k_conv_sym_reduce");
     return DEM_OK;
   }
@@ -667,6 +895,12 @@ int dem_direct_set_symmetric(int on) {
   return prev;
 }
 
+int dem_direct_set_image32(int on) {
+  const int prev = g_img32;
+  g_img32 = on ? 1 : 0;
+  return prev;
+}
+
 int dem_direct_set_tuning(int split, int tile) {
   int prev = (g_split << 16) | (g_tile & 0xffff);
   g_split = split > 0 ? split : 0;
@@ -701,7 +935,7 @@ int dem_direct_convection(const double* pos, const double* radii, const double* 
     return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
   hipStream_t st = dem::as_stream(stream);
   if ((rc = prepare(l, pos, radii, F, n, ph, st))) return rc;
-  return run_conv(l, n, ph, corr, F, V, st);
+  return run_conv(l, n, ph, corr, F, V, st, conv_mode(ph, false));
 }
 
 int dem_direct_forces_and_convection(const double* pos, const double* radii, int n,
@@ -716,9 +950,11 @@ int dem_direct_forces_and_convection(const double* pos, const double* radii, int
   if (!ws || ws_bytes < l.bytes)
     return dem::set_error(DEM_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, l.bytes);
   hipStream_t st = dem::as_stream(stream);
-  if ((rc = prepare(l, pos, radii, nullptr, n, ph, st))) return rc;
-  if ((rc = run_forces(l, radii, n, ph, F, st))) return rc;
-  return run_conv(l, n, ph, corr, F, V, st);
+  const bool img = !(corr && corr->enabled) && img_applies(l, n, ph);
+  const ImgArgs ia = img_args(l, img);
+  if ((rc = prepare(l, pos, radii, nullptr, n, ph, st, img))) return rc;
+  if ((rc = run_forces(l, radii, n, ph, F, st, ia))) return rc;
+  return run_conv(l, n, ph, corr, F, V, st, conv_mode(ph, img), ia);
 }
 
 int dem_direct_forces_and_convection_f64(const double* pos, const double* radii, int n,

@@ -5,6 +5,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "direct_core.h"
 #include "pair_math.h"
 
@@ -234,45 +236,78 @@ __device__ __forceinline__ float oseen_pair_sym(float dx, float dy, float dz, fl
   return ir;
 }
 
+// LDS of one pair-once Oseen workgroup of W waves: source positions (64-bit
+// ring words, or with POS32 their 32-bit high words), source forces and the
+// waves' reversed-pair slot accumulators.
+template <int W, bool POS32>
+struct ConvSymLds {
+  typename std::conditional<POS32, uint32_t, unsigned long long>::type pos[3][kSymLds];
+  float f[3][kSymLds];
+  float acc[W][3][kSymLds];
+};
+
 // One workgroup of the pair-once Oseen sum (same arguments and partial
-// layout as force_sym_block).
-template <int TPL>
-__device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int n, int B, int b,
+// layout as force_sym_block) in the caller's LDS.  POS32 takes separations
+// from the 32-bit high words of the positions (floor of the 64-bit ring
+// value): c = h_j - h_i is the 64-bit difference's high word plus the low
+// words' borrow, so it is exact to one unit except that c == 2^31 (exactly
+// half the ring) may stand for +2^31 - 1, the other periodic image.  Callers
+// that take POS32 in a periodic box correct those pairs afterwards; in an
+// open box the high words stay below 2^31 and c never wraps.
+template <int TPL, bool POS32>
+__device__ __forceinline__ void conv_sym_block(ConvSymLds<kSymB / TPL / 64, POS32>& L,
+                                               const RecC* __restrict__ rec, int n, int B, int b,
                                                int s, int e0, int e1, int tile,
                                                double* __restrict__ part,
                                                PartJ* __restrict__ part_j) {
   constexpr int T = kSymB / TPL, W = T / 64;
-  __shared__ unsigned long long sx[kSymLds], sy[kSymLds], sz[kSymLds];
-  __shared__ float sfx[kSymLds], sfy[kSymLds], sfz[kSymLds];
-  __shared__ float acc[W][3][kSymLds];
+  using PosW = typename std::conditional<POS32, uint32_t, unsigned long long>::type;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  unsigned long long nq[TPL][3];
+  unsigned long long nq[TPL][3];  // negated 64-bit target positions
+  uint32_t hq[TPL][3];            // 32-bit target positions
   float fi[TPL][3];
   double ai[TPL][3];
   for (int h = 0; h < TPL; ++h) {
     const RecC m = rec[(size_t)b * kSymB + t + T * h];
-    nq[h][0] = 0ull - (((unsigned long long)m.xh << 32) | m.xl);
-    nq[h][1] = 0ull - (((unsigned long long)m.yh << 32) | m.yl);
-    nq[h][2] = 0ull - (((unsigned long long)m.zh << 32) | m.zl);
+    if (POS32) {
+      hq[h][0] = m.xh;
+      hq[h][1] = m.yh;
+      hq[h][2] = m.zh;
+    } else {
+      nq[h][0] = 0ull - (((unsigned long long)m.xh << 32) | m.xl);
+      nq[h][1] = 0ull - (((unsigned long long)m.yh << 32) | m.yl);
+      nq[h][2] = 0ull - (((unsigned long long)m.zh << 32) | m.zl);
+    }
     fi[h][0] = m.fx;
     fi[h][1] = m.fy;
     fi[h][2] = m.fz;
     ai[h][0] = ai[h][1] = ai[h][2] = 0.0;
   }
+  // separation of target h from the source in LDS slot q, axis a
+  auto sep = [&](int h, int a, int q) -> float {
+    if (POS32) return dem::diff32((uint32_t)L.pos[a][q], hq[h][a]);
+    return dem::diff64n((unsigned long long)L.pos[a][q], nq[h][a]);
+  };
   const int gt = max(1, tile / 4);
   for (int e = e0; e < e1; ++e) {
     const int c = e == 0 ? b : (b + e) % B;
     __syncthreads();
     for (int sl = t; sl < kSymB; sl += T) {
       const RecC rc = rec[(size_t)c * kSymB + sl];
-      sym_stage(sx, ((unsigned long long)rc.xh << 32) | rc.xl, sl);
-      sym_stage(sy, ((unsigned long long)rc.yh << 32) | rc.yl, sl);
-      sym_stage(sz, ((unsigned long long)rc.zh << 32) | rc.zl, sl);
-      sym_stage(sfx, rc.fx, sl);
-      sym_stage(sfy, rc.fy, sl);
-      sym_stage(sfz, rc.fz, sl);
+      if (POS32) {
+        sym_stage(L.pos[0], (PosW)rc.xh, sl);
+        sym_stage(L.pos[1], (PosW)rc.yh, sl);
+        sym_stage(L.pos[2], (PosW)rc.zh, sl);
+      } else {
+        sym_stage(L.pos[0], (PosW)(((unsigned long long)rc.xh << 32) | rc.xl), sl);
+        sym_stage(L.pos[1], (PosW)(((unsigned long long)rc.yh << 32) | rc.yl), sl);
+        sym_stage(L.pos[2], (PosW)(((unsigned long long)rc.zh << 32) | rc.zl), sl);
+      }
+      sym_stage(L.f[0], rc.fx, sl);
+      sym_stage(L.f[1], rc.fy, sl);
+      sym_stage(L.f[2], rc.fz, sl);
     }
-    sym_zero(acc[w], lane);
+    sym_zero(L.acc[w], lane);
     __syncthreads();
     if (e == 0) {
       for (int kt = 0; kt < kSymB; kt += tile) {
@@ -281,12 +316,10 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
 #pragma unroll 2
         for (int k = kt; k < ke; ++k) {
           const int q = lane + k;
-          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
-          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
+          const float Fx = L.f[0][q], Fy = L.f[1][q], Fz = L.f[2][q];
 #pragma unroll
           for (int h = 0; h < TPL; ++h) {
-            const float dx = dem::diff64n(xj, nq[h][0]), dy = dem::diff64n(yj, nq[h][1]),
-                        dz = dem::diff64n(zj, nq[h][2]);
+            const float dx = sep(h, 0, q), dy = sep(h, 1, q), dz = sep(h, 2, q);
             dem::oseen_pair(dx, dy, dz, Fx, Fy, Fz, dx * dx + dy * dy + dz * dz > 0.f,  // j != i
                             v[h][0], v[h][1], v[h][2]);
           }
@@ -303,20 +336,18 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int q = lane + kk + 64 * u;
-          const unsigned long long xj = sx[q], yj = sy[q], zj = sz[q];
-          const float Fx = sfx[q], Fy = sfy[q], Fz = sfz[q];
-          float o[3] = {acc[w][0][q], acc[w][1][q], acc[w][2][q]};
+          const float Fx = L.f[0][q], Fy = L.f[1][q], Fz = L.f[2][q];
+          float o[3] = {L.acc[w][0][q], L.acc[w][1][q], L.acc[w][2][q]};
 #pragma unroll
           for (int h = 0; h < TPL; ++h) {
             float oh[3];
-            const float ir = oseen_pair_sym(dem::diff64n(xj, nq[h][0]), dem::diff64n(yj, nq[h][1]),
-                                            dem::diff64n(zj, nq[h][2]), Fx, Fy, Fz, fi[h], v[h],
-                                            oh);
+            const float ir = oseen_pair_sym(sep(h, 0, q), sep(h, 1, q), sep(h, 2, q), Fx, Fy, Fz,
+                                            fi[h], v[h], oh);
             for (int a = 0; a < 3; ++a) o[a] = fmaf(ir, oh[a], o[a]);
           }
-          acc[w][0][q] = o[0];
-          acc[w][1][q] = o[1];
-          acc[w][2][q] = o[2];
+          L.acc[w][0][q] = o[0];
+          L.acc[w][1][q] = o[1];
+          L.acc[w][2][q] = o[2];
         }
         wave_sync();
       }
@@ -324,13 +355,23 @@ __device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int
         for (int a = 0; a < 3; ++a) ai[h][a] += (double)v[h][a];
     }
     __syncthreads();
-    sym_fold<T, W>(acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
+    sym_fold<T, W>(L.acc, t, part_j + ((size_t)b * (B / 2) + (e - 1)) * 3 * kSymB);
   }
   for (int h = 0; h < TPL; ++h) {
     const int i = b * kSymB + t + T * h;
     if (i < n)
       for (int a = 0; a < 3; ++a) part[(size_t)(3 * s + a) * n + i] = ai[h][a];
   }
+}
+
+// The 64-bit form in its own LDS (the batched kernels).
+template <int TPL>
+__device__ __forceinline__ void conv_sym_block(const RecC* __restrict__ rec, int n, int B, int b,
+                                               int s, int e0, int e1, int tile,
+                                               double* __restrict__ part,
+                                               PartJ* __restrict__ part_j) {
+  __shared__ ConvSymLds<kSymB / TPL / 64, false> L;
+  conv_sym_block<TPL, false>(L, rec, n, B, b, s, e0, e1, tile, part, part_j);
 }
 
 // Reversed-pair sums of target i from the blocks b = bi - e (mod B) that

@@ -122,6 +122,7 @@ SIGNATURES: dict[str, tuple] = {
     "dem_direct_f64_targets": (_I, [_P, _P, _I, _P, _P, _P, _I, _P, _P, _P, _P]),
     "dem_direct_set_tuning": (_I, [_I, _I]),
     "dem_direct_set_symmetric": (_I, [_I]),
+    "dem_direct_set_image32": (_I, [_I]),
     "dem_octree_set_exact_mac": (_I, [_I]),
     "dem_hash_workspace_bytes": (_S, [_I, _I]),
     "dem_hash_pairs": (_I, [_P, _P, _I, _D, _I, _I, _P, _I, _P, _P, _S, _P]),

@@ -255,3 +255,118 @@ def test_forced_pair_once_small_n(gpu, n, split):
     ev = summary(V.cpu().numpy(), V64.cpu().numpy(), FLOOR)
     print(f"\n[pair-once mode 2 N={n} split={split}] force {ef}\n  conv {ev}")
     assert _ok_forces(ef) and _ok_velocities(ev)
+
+
+# --- 32-bit Oseen separations: pairs at half the box (dem_direct_set_image32)
+
+
+def _planted_half_box_system(dup: bool = False):
+    """N = 10240 (pair-once range), periodic, box edge 2^-10 m so that the
+    fixed-point ring words q = (x / L) 2^64 are exact.  Pairs are planted
+    across half the box: droplet j sits at q_j = q_i + 2^63 -+ 2^11 on one axis,
+    so the 32-bit high words differ by exactly 2^31.  With the borrow (-) the
+    true separation is just below L/2 and the 32-bit difference names the other
+    image; without it (+) both agree.  One pair is planted on two axes.  dup:
+    a third droplet takes droplet 10's x, so the high word that droplet 5000
+    looks up is shared and the call must fall back to 64-bit separations."""
+    s = make_system(10240, seed=3, phi=0.02)
+    L = 2.0**-10
+    u = s.positions / s.L
+    u -= np.floor(u)
+    radii = s.radii * (L / s.L)
+    plants = [(10, 5000, (0,), True), (20, 6000, (1,), True), (30, 7000, (2,), True),
+              (40, 8000, (0,), False), (60, 9000, (0, 2), True)]
+    for k, (i, j, axes, borrow) in enumerate(plants):
+        for a in axes:
+            qi = 2**62 + (k + 1) * 2**40 + (a + 1) * 2**20
+            qj = qi + 2**63 + (-(2**11) if borrow else 2**11)
+            u[i, a], u[j, a] = qi / 2.0**64, qj / 2.0**64
+            assert int(u[i, a] * 2.0**64) == qi and int(u[j, a] * 2.0**64) == qj
+    if dup:
+        u[50, 0] = u[10, 0]
+    return s, u * L, radii, L, plants
+
+
+def _oseen_flip(pos, F, eta, L, i, j, axes):
+    """Relative change of v_i if the pair (i, j) took the other image on
+    `axes` (float64)."""
+    d = pos[j] - pos[i]
+    d -= L * np.round(d / L)
+    df = d.copy()
+    df[list(axes)] *= -1.0
+
+    def G(dd):
+        r = np.linalg.norm(dd)
+        return eta * (F[j] / r + dd * np.dot(dd, F[j]) / r**3)
+
+    return G(df) - G(d)
+
+
+def test_image32_boundary_pairs_corrected(gpu):
+    """The default pair-once Oseen sum (32-bit separations + boundary-pair
+    correction) against the FP64 kernel and against 64-bit separations: the
+    contract holds, forces are untouched (bit-identical), and every planted
+    droplet whose velocity a wrong image would move by >= 1e-3 is within 5 % of
+    that move and no worse than the 64-bit evaluation."""
+    from dem import _device, _native
+
+    s, pos, radii, L, plants = _planted_half_box_system()
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                      eta_water=s.eta_water, E=s.E, L=L, boundary_mode="periodic")
+    pd, rd = _device.to_device(pos), _device.to_device(radii)
+    F, V = fc.forces_and_convection_device(pd, rd)
+    prev = _native.call("dem_direct_set_image32", 0)
+    try:
+        F1, V1 = fc.forces_and_convection_device(pd, rd)
+    finally:
+        _native.call("dem_direct_set_image32", prev)
+    F64, V64 = fc.forces_and_convection_f64_device(pd, rd)
+    F, V, F1, V1, F64, V64 = (t.cpu().numpy() for t in (F, V, F1, V1, F64, V64))
+    assert np.array_equal(F, F1)
+    ev, ev1 = summary(V, V64, FLOOR), summary(V1, V64, FLOOR)
+    print(f"\n[planted half-box pairs] image32 {ev}\n  64-bit {ev1}")
+    assert _ok_forces(summary(F, F64, FLOOR)) and _ok_velocities(ev) and _ok_velocities(ev1)
+    from ._errors import rel_err
+
+    e, e1 = rel_err(V, V64, FLOOR), rel_err(V1, V64, FLOOR)
+    rms = float(np.sqrt(np.mean(np.sum(V64**2, axis=1))))
+    checked = 0
+    for i, j, axes, _ in plants:
+        for p, q in ((i, j), (j, i)):
+            flip = np.linalg.norm(_oseen_flip(pos, F64, fc.eta_const, L, p, q, axes))
+            rel = flip / max(np.linalg.norm(V64[p]), FLOOR * rms)
+            print(f"  droplet {p}: wrong-image move {rel:.2e}, error {e[p]:.2e} (64-bit {e1[p]:.2e})")
+            assert e[p] <= 2.0 * e1[p] + 1e-6
+            if rel >= 1e-3:
+                assert e[p] <= 0.05 * rel
+                checked += 1
+    assert checked >= 5
+
+
+def test_image32_duplicate_word_falls_back(gpu):
+    """A high word shared by two droplets, one of which has a partner across
+    half the box, makes the call take 64-bit separations: the result equals the
+    64-bit evaluation bit for bit."""
+    from dem import _device, _native
+
+    s, pos, radii, L, _ = _planted_half_box_system(dup=True)
+    fc = DirectDropletForceCalculator(num_particles=s.n, eps_oil=s.eps_oil, eta_oil=s.eta_oil,
+                                      eta_water=s.eta_water, E=s.E, L=L, boundary_mode="periodic")
+    pd, rd = _device.to_device(pos), _device.to_device(radii)
+    F, V = fc.forces_and_convection_device(pd, rd)
+    prev = _native.call("dem_direct_set_image32", 0)
+    try:
+        F1, V1 = fc.forces_and_convection_device(pd, rd)
+    finally:
+        _native.call("dem_direct_set_image32", prev)
+    assert np.array_equal(F.cpu().numpy(), F1.cpu().numpy())
+    assert np.array_equal(V.cpu().numpy(), V1.cpu().numpy())
+    # and the same positions without the duplicate take the 32-bit path
+    s2, pos2, radii2, _, _ = _planted_half_box_system(dup=False)
+    F2, V2 = fc.forces_and_convection_device(_device.to_device(pos2), _device.to_device(radii2))
+    _native.call("dem_direct_set_image32", 0)
+    try:
+        F3, V3 = fc.forces_and_convection_device(_device.to_device(pos2), _device.to_device(radii2))
+    finally:
+        _native.call("dem_direct_set_image32", prev)
+    assert not np.array_equal(V2.cpu().numpy(), V3.cpu().numpy())
