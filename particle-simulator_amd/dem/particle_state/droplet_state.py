@@ -162,6 +162,14 @@ class DropletState(ParticleState):
     def copy(self):
         return DropletState(self.positions.copy(), self.radii.copy(), self.time)
 
+    @classmethod
+    def adopt(cls, positions, radii, time=0):
+        """A state holding ``positions`` and ``radii`` themselves (no copy), for
+        arrays that nothing else refers to."""
+        st = cls.__new__(cls)
+        st.positions, st.radii, st.time = positions, radii, time
+        return st
+
     def save(self, filename):
         np.savez(filename, positions=self.positions, radii=self.radii, time=self.time)
 

@@ -14,6 +14,11 @@ so its callers and files work unchanged.
 Recording: rows [0, current_step] of ``times`` / ``trajectories`` are
 recorded; the buffers double when full (``length`` doubles with them, and is
 what a saved chain stores), and ``compact`` trims them to the recorded rows.
+Unrecorded rows of ``times`` hold the segment's start time; unrecorded rows
+of ``trajectories`` are not initialised: the reference zero-fills a new
+segment's buffer (milliseconds per segment at 65536 droplets) but leaves the
+rows a doubling adds uninitialised, and every accessor reads recorded rows
+only.
 
 The merge (``generate_next_solution``, reference :208-287) is kept bit-exact
 with the reference: pairs visited in numpy's default ``argsort`` order of the
@@ -150,10 +155,10 @@ class DropletSolution(Solution):
         self.collided_droplets = []
         self.resulting_droplet = []
         self.last_update_time = time.time()
-        # unrecorded rows hold the start time / zeros until written
+        # unrecorded rows: the start time in ``times``, uninitialised trajectories
         self.length = length
         self.times = np.zeros((length, 1)) + state.time
-        self.trajectories = np.zeros((length, self.num_particles, 3))
+        self.trajectories = np.empty((length, self.num_particles, 3))
         self.current_step = -1
         self.save_step(state.time, state.positions)
 
@@ -249,12 +254,16 @@ class DropletSolution(Solution):
             self.trajectories[self.current_step], self.radii, self.collided_droplets)
         return self.next_solution(new_positions, new_radii, visit)
 
-    def next_solution(self, new_positions, new_radii, collided):
+    def next_solution(self, new_positions, new_radii, collided, adopt=False):
         """Open the next segment from already merged arrays (``merge_collided``'s
         outputs, or the same values compacted on the device); ``collided`` is
-        the pair list in visiting order, kept as this segment's record."""
+        the pair list in visiting order, kept as this segment's record.  With
+        ``adopt`` the new initial state holds the arrays themselves instead of
+        copies (arrays that nothing else refers to)."""
         self.collided_droplets = collided
-        start = DropletState(new_positions, new_radii, self.times[self.current_step])
+        t0 = self.times[self.current_step]
+        start = (DropletState.adopt(new_positions, new_radii, t0) if adopt
+                 else DropletState(new_positions, new_radii, t0))
         nxt = DropletSolution(
             initial_droplet_state=start,
             real_time_visualization=self.real_time_visualization,

@@ -130,9 +130,11 @@ class DeviceEulerDropletSolver(EulerDropletSolver):
         _native.call("dem_merge_compact", _device.ptr(self._pos), _device.ptr(self._rad), n,
                      _device.ptr(rem_d), len(rem), _device.ptr(merged_d), m, _device.ptr(out_p),
                      _device.ptr(out_r), _device.stream())
-        new_p_h = np.array(self._staged(out_p))
+        # one host copy of the merged state out of the pinned staging buffer,
+        # adopted by the new segment's initial state
         new_r_h = np.array(self._staged(out_r))
-        return sol.next_solution(new_p_h, new_r_h, visit), (out_p, out_r)
+        new_p_h = np.array(self._staged(out_p))
+        return sol.next_solution(new_p_h, new_r_h, visit, adopt=True), (out_p, out_r)
 
     def _staged(self, d):
         """Device float64 tensor -> numpy view of a pinned staging buffer (one

@@ -204,3 +204,27 @@ def test_merge_plan_rows_matches_the_reference_merge_bit_for_bit():
         got_p = np.vstack([pos[keep], m[:, :3]])
         got_r = np.concatenate([rad[keep], m[:, 3]])
         assert np.array_equal(got_p, ref_p) and np.array_equal(got_r, ref_r), trial
+
+
+def test_get_state_rows_interpolation_and_chain():
+    """get_state (reference droplet_solution.py:423-484) on compacted
+    segments: a recorded time returns its row, a time between two rows
+    interpolates linearly, and a time in a later segment of the chain is
+    found there."""
+    rng = np.random.default_rng(4)
+    n = 5
+    p0, p1, p2 = (rng.random((n, 3)) for _ in range(3))
+    r = np.full(n, 1e-6)
+    sol = DropletSolution(DropletState(p0.copy(), r.copy(), 0.0), length=2)
+    sol.save_step(1.0, p1)
+    sol.save_step(2.0, p2)  # doubles the buffers
+    sol.compact()
+    assert np.array_equal(sol.get_state(1.0).positions, p1)
+    assert np.array_equal(sol.get_state(2.0).positions, p2)
+    np.testing.assert_allclose(sol.get_state(1.25).positions, p1 + 0.25 * (p2 - p1), rtol=0,
+                               atol=1e-15)
+    nxt = sol.next_solution(p2[1:], r[1:], np.array([[0, 1]]))
+    nxt.save_step(3.0, p0[1:])
+    nxt.compact()
+    got = sol.get_state(2.5).positions
+    np.testing.assert_allclose(got, p2[1:] + 0.5 * (p0[1:] - p2[1:]), rtol=0, atol=1e-15)
