@@ -36,12 +36,18 @@
 
 #include "dem_common.h"
 #include "direct_core.h"
+#include "direct_pairs.h"
 #include "direct_sym.h"
+#include "image32.h"
 #include "profile.h"
 
 namespace {
 
 using dem::DevScales;
+using dem::ImgArgs;
+using dem::img_correct;
+using dem::img_probe;
+using dem::rec_hi;
 using dem::Rec;
 using dem::RecC;
 
@@ -252,127 +258,6 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ pos,
   recc[i] = c;
 }
 
-// ---------------------------------------------------------------- boundary pairs
-//
-// The pair-once Oseen kernel may take separations from the 32-bit high words
-// h = q >> 32 of the 64-bit ring positions q.  Then c = h_j - h_i (mod 2^32)
-// is hi(q_j - q_i) plus the borrow of the low words, and its sign -- the
-// periodic image -- differs from the 64-bit difference's only when
-// c == 2^31 exactly on some axis, where hi(q_j - q_i) may be 2^31 - 1 (the
-// pair lies within one unit of half the box).  Those pairs are found by an
-// exact-match join: per axis, an open-addressing table keyed by h (each slot
-// (h << 32) | (i + 1), 0 empty, bit 31 marking a key that several droplets
-// share) is probed for h ^ 2^31.  A droplet has at most one partner per axis
-// unless a key repeats; a repeated key with a partner makes the call fall back
-// to 64-bit separations (img_flag).  The reduce then replaces, for each
-// partner, the term the kernel added with the 64-bit one.
-
-constexpr unsigned long long kImgMulti = 1ull << 31;
-
-__device__ __forceinline__ uint32_t img_hash(uint32_t key, int log2m) {
-  return (key * 0x9E3779B1u) >> (32 - log2m);
-}
-
-__device__ __forceinline__ void img_insert(unsigned long long* __restrict__ tab, int log2m,
-                                           uint32_t key, int i) {
-  const uint32_t mask = (1u << log2m) - 1u;
-  const unsigned long long val = ((unsigned long long)key << 32) | (unsigned long long)(i + 1);
-  uint32_t slot = img_hash(key, log2m);
-  for (uint32_t it = 0; it <= mask; ++it) {
-    const unsigned long long old = atomicCAS(tab + slot, 0ull, val);
-    if (old == 0ull) return;
-    if ((uint32_t)(old >> 32) == key) {
-      atomicOr(tab + slot, kImgMulti);
-      return;
-    }
-    slot = (slot + 1u) & mask;
-  }
-}
-
-// Droplet whose high word is key ^ 2^31: its index, -1 for none, -2 when
-// several droplets share that word.
-__device__ __forceinline__ int img_probe(const unsigned long long* __restrict__ tab, int log2m,
-                                         uint32_t key) {
-  const uint32_t mask = (1u << log2m) - 1u, want = key ^ 0x80000000u;
-  uint32_t slot = img_hash(want, log2m);
-  for (uint32_t it = 0; it <= mask; ++it) {
-    const unsigned long long v = tab[slot];
-    if (v == 0ull) return -1;
-    if ((uint32_t)(v >> 32) == want) return (v & kImgMulti) ? -2 : (int)(v & 0x7fffffffull) - 1;
-    slot = (slot + 1u) & mask;
-  }
-  return -1;
-}
-
-__device__ __forceinline__ uint32_t rec_hi(const RecC& r, int a) {
-  return a == 0 ? r.xh : (a == 1 ? r.yh : r.zh);
-}
-__device__ __forceinline__ uint32_t rec_lo(const RecC& r, int a) {
-  return a == 0 ? r.xl : (a == 1 ? r.yl : r.zl);
-}
-
-// Oseen term F/|d| + d (d.F)/|d|^3 in float64 (fixed units, as the pair
-// kernels accumulate it); nothing for d == 0.
-__device__ __forceinline__ void oseen_term64(const double* d, const double* f, double* out) {
-#pragma clang fp contract(off)
-  const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  if (!(r2 > 0.0)) {
-    out[0] = out[1] = out[2] = 0.0;
-    return;
-  }
-  const double ir = 1.0 / sqrt(r2);
-  const double t = (d[0] * f[0] + d[1] * f[1] + d[2] * f[2]) * (ir * ir);
-  for (int a = 0; a < 3; ++a) out[a] = ir * (f[a] + t * d[a]);
-}
-
-// Correction of droplet i's Oseen sum for its boundary partners (at most one
-// per axis, taken in ascending index): the 64-bit term minus the term the
-// 32-bit pair kernel added.  The kernel evaluates each pair from its target:
-// droplet i is the target when j lies in its own block or in one of its
-// block's partner blocks, else j is, and i received G(d_ji) F_j.
-__device__ __forceinline__ void img_correct(const RecC* __restrict__ rec, int n, int B, int i,
-                                            const int* __restrict__ partner, double* m) {
-  // the three partners in ascending order, INT_MAX for none
-  int js[3];
-  for (int a = 0; a < 3; ++a) {
-    const int j = partner[3 * (size_t)i + a];
-    js[a] = (j < 0 || j >= n || j == i) ? INT_MAX : j;
-  }
-  auto order = [](int& u, int& v) {
-    const int lo = min(u, v), hi = max(u, v);
-    u = lo;
-    v = hi;
-  };
-  order(js[0], js[1]);
-  order(js[1], js[2]);
-  order(js[0], js[1]);
-  if (js[0] == INT_MAX) return;
-  const RecC ri = rec[i];
-  const int bi = i / kSymB;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int j = js[k];
-    if (j == INT_MAX || (k > 0 && j == js[k - 1])) continue;  // none, or found on two axes
-    const RecC rj = rec[j];
-    const int bj = j / kSymB;
-    const int e = (bj - bi + B) % B;
-    const bool i_target = bi == bj || (e >= 1 && e <= sym_partners(bi, B));
-    double d32[3], d64[3];
-    for (int a = 0; a < 3; ++a) {
-      const uint32_t hi = rec_hi(ri, a), hj = rec_hi(rj, a);
-      d32[a] = (double)dem::diff32(i_target ? hj : hi, i_target ? hi : hj);
-      const unsigned long long qi = ((unsigned long long)hi << 32) | rec_lo(ri, a);
-      const unsigned long long qj = ((unsigned long long)hj << 32) | rec_lo(rj, a);
-      d64[a] = (double)dem::diff64n(qj, 0ull - qi);
-    }
-    const double f[3] = {(double)rj.fx, (double)rj.fy, (double)rj.fz};
-    double t64[3], t32[3];
-    oseen_term64(d64, f, t64);
-    oseen_term64(d32, f, t32);
-    for (int a = 0; a < 3; ++a) m[a] += t64[a] - t32[a];
-  }
-}
-
 // ---------------------------------------------------------------- phase 1
 
 __global__ __launch_bounds__(WG) void k_force_tile(const Rec* __restrict__ rec, int n, int jchunk,
@@ -412,35 +297,6 @@ __global__ __launch_bounds__(256) void k_force_reduce(const double* __restrict__
 }
 
 // ---------------------------------------------------------------- phase 1, pair-once
-
-// Boundary-pair tables of one call (null img_slot: not used).
-struct ImgArgs {
-  unsigned long long* slot;
-  int* partner;
-  int* flag;
-  int log2m;
-};
-
-template <int TPL>
-__global__ __launch_bounds__(kSymB / TPL) void k_force_sym(const Rec* __restrict__ rec,
-                                                           const RecC* __restrict__ recc, int n,
-                                                           int B, int ec, int tile,
-                                                           double* __restrict__ part,
-                                                           dem::PartJ* __restrict__ part_j,
-                                                           ImgArgs img) {
-  const int b = blockIdx.x, s = blockIdx.y;
-  if (img.slot) {
-    // the S workgroups of block b share its 3 x 256 (droplet, axis) keys
-    for (int u = s + (int)gridDim.y * (int)threadIdx.x; u < 3 * kSymB;
-         u += (int)gridDim.y * (int)blockDim.x) {
-      const int i = b * kSymB + u / 3, a = u % 3;
-      if (i < n)
-        img_insert(img.slot + ((size_t)a << img.log2m), img.log2m, rec_hi(recc[i], a), i);
-    }
-  }
-  const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
-  dem::force_sym_block<TPL>(rec, n, B, b, s, e0, e1, tile, part, part_j);
-}
 
 // Four lanes per target (the reversed partials are most of the bytes); with
 // boundary-pair tables, lanes 1..3 then look up the target's partner across
@@ -534,24 +390,6 @@ __global__ __launch_bounds__(256) void k_conv_reduce(const double* __restrict__ 
 // Separations of the pair-once Oseen sum: 64-bit, 32-bit (open boxes, where
 // they are exact), or 32-bit unless the boundary-pair lookup fell back.
 enum ConvPos { kPos64 = 0, kPos32 = 1, kPos32Checked = 2 };
-
-// POS32: 32-bit separations in a 32-bit LDS layout (more workgroups per
-// compute unit).  With a flag, the kernel runs only when (*flag != 0) equals
-// run_if: the 32-bit kernel while the boundary-pair lookup holds, the 64-bit
-// one as its fallback (both are launched; the other leaves at once).
-template <int TPL, bool POS32>
-__global__ __launch_bounds__(kSymB / TPL) void k_conv_sym(const RecC* __restrict__ rec, int n,
-                                                          int B, int ec, int tile,
-                                                          double* __restrict__ part,
-                                                          dem::PartJ* __restrict__ part_j,
-                                                          const int* __restrict__ flag,
-                                                          int run_if) {
-  if (flag && ((*flag != 0) != (run_if != 0))) return;
-  __shared__ dem::ConvSymLds<kSymB / TPL / 64, POS32> L;
-  const int b = blockIdx.x, s = blockIdx.y;
-  const int e0 = s * ec, e1 = min(1 + sym_partners(b, B), e0 + ec);
-  dem::conv_sym_block<TPL, POS32>(L, rec, n, B, b, s, e0, e1, tile, part, part_j);
-}
 
 __global__ __launch_bounds__(256) void k_conv_sym_reduce(
     const double* __restrict__ part, int S, int n, const dem::PartJ* __restrict__ part_j, int B,
@@ -791,14 +629,12 @@ int run_forces(const Layout& l, const double* radii, int n, const dem_phys* ph, 
   if (use_sym(n)) {
     static int slots[kMaxDevices];
     const int B = sym_blocks(n);
-    const int S = sym_split(n, resident_slots(slots, (const void*)k_force_sym<kSymTplForce>,
+    const int S = sym_split(n, resident_slots(slots, dem::force_sym_kernel(),
                                               kSymB / kSymTplForce));
     const int ec = (1 + B / 2 + S - 1) / S;
     DEM_PROF_LAUNCH("k_force_sym", st,
-                    hipLaunchKernelGGL(k_force_sym<kSymTplForce>, dim3(B, S),
-                                       dim3(kSymB / kSymTplForce), 0, st,
-                                       (const Rec*)l.rec, (const RecC*)l.recc, n, B, ec,
-                                       choose_tile_force(), l.part, l.part_j, img));
+                    dem::launch_force_sym(dim3(B, S), st, (const Rec*)l.rec, (const RecC*)l.recc,
+                                          n, B, ec, choose_tile_force(), l.part, l.part_j, img));
     DEM_CHECK_LAUNCH("k_force_sym");
     hipLaunchKernelGGL(k_force_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B, radii,
@@ -832,22 +668,20 @@ int run_conv(const Layout& l, int n, const dem_phys* ph, const dem_correction* c
     static int slots32[kMaxDevices], slots64[kMaxDevices];
     const int B = sym_blocks(n);
     const bool p32 = mode != kPos64;
-    const int S = p32 ? sym_split(n, resident_slots(slots32, (const void*)k_conv_sym<kSymTplConv, true>,
+    const int S = p32 ? sym_split(n, resident_slots(slots32, dem::conv_sym_kernel(true),
                                                     kSymB / kSymTplConv))
-                      : sym_split(n, resident_slots(slots64, (const void*)k_conv_sym<kSymTplConv, false>,
+                      : sym_split(n, resident_slots(slots64, dem::conv_sym_kernel(false),
                                                     kSymB / kSymTplConv));
     const int ec = (1 + B / 2 + S - 1) / S;
     const int* flag = mode == kPos32Checked ? (const int*)img.flag : nullptr;
     if (p32)
       DEM_PROF_LAUNCH("k_conv_sym", st,
-                      hipLaunchKernelGGL((k_conv_sym<kSymTplConv, true>), dim3(B, S),
-                                         dim3(kSymB / kSymTplConv), 0, st, (const RecC*)l.recc, n,
-                                         B, ec, choose_tile_conv(), l.part, l.part_j, flag, 0));
+                      dem::launch_conv_sym(true, dim3(B, S), st, (const RecC*)l.recc, n, B, ec,
+                                           choose_tile_conv(), l.part, l.part_j, flag, 0));
     if (!p32 || flag)
       DEM_PROF_LAUNCH(p32 ? "k_conv_sym_fallback" : "k_conv_sym", st,
-                      hipLaunchKernelGGL((k_conv_sym<kSymTplConv, false>), dim3(B, S),
-                                         dim3(kSymB / kSymTplConv), 0, st, (const RecC*)l.recc, n,
-                                         B, ec, choose_tile_conv(), l.part, l.part_j, flag, 1));
+                      dem::launch_conv_sym(false, dim3(B, S), st, (const RecC*)l.recc, n, B, ec,
+                                           choose_tile_conv(), l.part, l.part_j, flag, 1));
     DEM_CHECK_LAUNCH("k_conv_sym");
     hipLaunchKernelGGL(k_conv_sym_reduce, dim3((n + 63) / 64), dim3(256), 0, st,
                        (const double*)l.part, S, n, (const dem::PartJ*)l.part_j, B,
