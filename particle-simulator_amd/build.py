@@ -35,11 +35,14 @@ HIP_FLAGS = [
 ]
 
 
-# Per-file additions to HIP_FLAGS.  The pair-once direct kernels take the
-# ILP-first instruction scheduler: under the default one 96 of the Oseen
-# loop's 258 VALU instructions wait on the instruction before them.
+# Per-file additions to HIP_FLAGS.  The pair-once direct kernels, single-
+# system and batched, take the ILP-first instruction scheduler: under the
+# default one 96 of the 32-bit Oseen loop's 258 VALU instructions wait on the
+# instruction before them (batched 64-bit loop: 181 of 615), which costs
+# issue slots even at four waves per SIMD.
 FILE_FLAGS = {
     "direct_pairs": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "batch": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
 }
 
 
