@@ -46,8 +46,8 @@ EXEC_CONV_SYM = 24.5
 # the same program); profiles/direct_force_oseen_n65536.md
 TRAFFIC_N65536 = {"k_force_tile": (15384.9 * 2.0 + 12288.0) * 1024,
                   "k_conv_tile": (23247.7 * 2.0 + 12288.0) * 1024,
-                  "k_force_sym": (21778.0 * 2.0 + 124556.7) * 1024,
-                  "k_conv_sym": (49514.9 * 2.0 + 113280.0) * 1024}
+                  "k_force_sym": (17396.9 * 2.0 + 121481.7) * 1024,
+                  "k_conv_sym": (52468.8 * 2.0 + 107136.0) * 1024}
 
 
 def _errors(x, ref, floor=1e-3) -> dict:
