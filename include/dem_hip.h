@@ -195,7 +195,7 @@ int dem_hash_pairs(const double* pos, const double* radii, int n, double L,
  * GPU, held until the process ends.  Same results as dem_hash_pairs.  Sizes
  * whose workspace (dem_hash_workspace_bytes(n, table_size)) exceeds the
  * scratch are rejected with DEM_ERR_CAPACITY (n = 65536 with table_size
- * 131072 needs about 7 MB; n up to ~310000 fits with the reference's table
+ * 131072 needs about 7 MB; n up to ~300000 fits with the reference's table
  * size).  Calls that overlap on one GPU share the scratch and race: issue
  * them on one stream or serialise them.
  * No counterpart in the reference: for callers that cannot allocate device

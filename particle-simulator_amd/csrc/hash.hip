@@ -17,16 +17,18 @@
 //
 // MI355X design, seven launches and no host synchronisation:
 //   prep     clear the bucket counters, partial maxima of the radii;
-//   assign   cell and bucket of every droplet, bucket counts (atomics);
-//   scan     per-tile exclusive scan of the bucket counts (4096-entry tiles,
-//            one launch; consumers add the prefix of the tile sums themselves);
-//   scatter  bucket offsets, droplets in bucket order with a bucket-ordered
-//            copy of (x, y, z, r);
-//   search   one wave per run of buckets: the droplets of a bucket that share
-//            a cell share their neighbour buckets, so the wave loads those
-//            buckets' candidates once (contiguous slots) and tests them
-//            against every target of the cell; each target's count and its
-//            first kStash partners (sorted by j) are kept;
+//   assign   cell and bucket of every droplet, bucket counts and each
+//            droplet's rank in its bucket (atomics);
+//   scan     per-tile exclusive scan of the bucket counts (one launch, tiles
+//            of 256..4096 entries; consumers add the prefix of the tile sums
+//            themselves);
+//   scatter  bucket offsets, droplets in bucket order (offset + rank) with a
+//            bucket-ordered copy of (x, y, z, r);
+//   search   one one-wave workgroup per run of buckets: the droplets of a
+//            bucket that share a cell share their neighbour buckets, so the
+//            wave loads those buckets' candidates once (contiguous slots) and
+//            tests them against every target of the cell; each target's
+//            count and its first kStash partners (sorted by j) are kept;
 //   scan     per-tile scan of the per-droplet pair counts;
 //   emit     each droplet's pairs at its offset (its stash, or a rescan of its
 //            neighbourhood when it has more partners than the stash holds);
@@ -57,26 +59,38 @@ struct HashParams {
 };
 
 constexpr int kThreads = 256;
-constexpr int kTileItems = 16;
-constexpr int kTile = kThreads * kTileItems;  // entries per scan tile
+constexpr int kTileItems = 16;                // most entries per thread and scan tile
+constexpr int kTile = kThreads * kTileItems;  // largest scan tile
 constexpr int kMaxTiles = 8192;               // so table_size and n are at most 2^25
 constexpr int kMaxRBlocks = 256;              // partial maxima of the radii
 constexpr int kStash = 8;                     // partners kept per target by the search
-constexpr int kBucketsPerWave = 8;
+constexpr int kBucketsPerWave = 4;
 constexpr int kMapCap = 256;  // candidates per group mapped to their neighbour in LDS
+constexpr int kCandPerLane = 3;  // candidate loads in flight per lane and pass
+// One wave per search workgroup: a wave that finds its buckets empty frees
+// its LDS at once instead of holding it until the block's slowest wave ends.
+constexpr int kSearchThreads = 64;
 
-inline int tiles(int n) { return (n + kTile - 1) / kTile; }
+// Scan tile of n entries: the smallest power-of-two multiple of kThreads, at
+// most kTile, whose tile count fits kMaxTiles (as log2; small scans then
+// spread over many workgroups).
+inline int tile_log2(int n) {
+  int lg = 8;  // kThreads entries
+  while (lg < 12 && ((n + (1 << lg) - 1) >> lg) > kMaxTiles) ++lg;
+  return lg;
+}
+inline int tiles(int n) { return (n + (1 << tile_log2(n)) - 1) >> tile_log2(n); }
 
 struct Layout {
   HashParams* hp;
   int* oob;           // 1: a coordinate lies outside [0, L] (periodic boxes)
   double* rmax_part;  // [kMaxRBlocks]
   int* bucket;        // [n] bucket of each droplet
+  int* rank;          // [n] its place among its bucket's droplets
   int* counts;        // [T] droplets per bucket
   int* offsets;       // [T] first slot of each bucket
-  int* fill;          // [T] slots taken so far (scatter); with local[T] after it, reused
-  int* local;         // [T] as the search's stash: kStash partners per droplet
-  int* stash;         // (= fill; at least kStash n ints)
+  int* local;         // [T] per-tile scan of the counts; reused as the search's
+  int* stash;         // stash (= local): kStash partners per droplet
   int* btile;         // [tiles(T)] tile sums of the bucket counts
   int* sorted;        // [n] droplet of each slot, bucket order
   double4* spos;      // [n] (x, y, z, r) of each slot
@@ -95,12 +109,12 @@ Layout carve(void* ws, int n, int T) {
   l.oob = c.take<int>(1);
   l.rmax_part = c.take<double>(kMaxRBlocks);
   l.bucket = c.take<int>(nn);
+  l.rank = c.take<int>(nn);
   l.counts = c.take<int>(tt);
   l.offsets = c.take<int>(tt);
-  const size_t shared = 2 * tt > kStash * nn ? 2 * tt : kStash * nn;
-  l.fill = c.take<int>(shared);
-  l.local = l.fill + tt;
-  l.stash = l.fill;
+  const size_t shared = tt > kStash * nn ? tt : kStash * nn;
+  l.local = c.take<int>(shared);
+  l.stash = l.local;
   l.btile = c.take<int>((size_t)tiles(T));
   l.sorted = c.take<int>(nn);
   l.spos = c.take<double4>(nn);
@@ -163,17 +177,13 @@ __device__ __forceinline__ int tile_prefix(const int* __restrict__ tile_sums, in
 
 // ---------------------------------------------------------------- kernels
 
-// Clears the bucket counters and slot cursors; blocks b < nrb also write the
-// largest radius of droplets b*256 + k*nrb*256 (fmax: exact, order-free).
+// Clears the bucket counters; blocks b < nrb also write the largest radius
+// of droplets b*256 + k*nrb*256 (fmax: exact, order-free).
 __global__ __launch_bounds__(kThreads) void k_hash_prep(const double* __restrict__ r, int n, int nrb,
-                                                        int* __restrict__ counts,
-                                                        int* __restrict__ fill, int T,
+                                                        int* __restrict__ counts, int T,
                                                         double* __restrict__ rmax_part, int* __restrict__ oob) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *oob = 0;
-  for (int h = blockIdx.x * kThreads + threadIdx.x; h < T; h += gridDim.x * kThreads) {
-    counts[h] = 0;
-    fill[h] = 0;
-  }
+  for (int h = blockIdx.x * kThreads + threadIdx.x; h < T; h += gridDim.x * kThreads) counts[h] = 0;
   if ((int)blockIdx.x >= nrb) return;
   double m = -INFINITY;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += nrb * kThreads) m = fmax(m, r[i]);
@@ -188,12 +198,14 @@ __global__ __launch_bounds__(kThreads) void k_hash_prep(const double* __restrict
 }
 
 // Cell size and grid from the largest radius (every block derives them; block
-// 0 stores them), then the bucket of every droplet and the bucket counts.
+// 0 stores them), then the bucket of every droplet, the bucket counts and
+// each droplet's rank in its bucket (the counter's value before its add).
 __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restrict__ pos, int n,
                                                           int periodic, int mask, double L,
                                                           const double* __restrict__ rmax_part,
                                                           int nrb, HashParams* __restrict__ hpo,
                                                           int* __restrict__ bucket,
+                                                          int* __restrict__ rank,
                                                           int* __restrict__ counts,
                                                           int* __restrict__ oob) {
   __shared__ HashParams shp;
@@ -228,62 +240,66 @@ __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restri
   const int iz = cell_of(pos[3 * (size_t)i + 2], hp.cell, hp.g, periodic);
   const int h = hash_cell(ix, iy, iz, mask);
   bucket[i] = h;
-  atomicAdd(&counts[h], 1);
+  rank[i] = atomicAdd(&counts[h], 1);
 }
 
-// Exclusive scan of in[0, n) within tiles of kTile entries (out) and the tile
-// sums; the global prefix of entry k is out[k] + (sum of tile sums before k's tile).
+// Exclusive scan of in[0, n) within tiles of 2^lg entries (out) and the tile
+// sums; the global prefix of entry k is out[k] + (sum of tile sums before k's
+// tile).  2^lg = kThreads * items, items in {1, 2, ..., kTileItems}.
 __global__ __launch_bounds__(kThreads) void k_tile_scan(const int* __restrict__ in,
                                                         int* __restrict__ out,
-                                                        int* __restrict__ tile_sums, int n) {
-  // padded LDS: entry e at e + e / 32 (a thread's 16 consecutive entries
-  // then fall in distinct banks for neighbouring threads)
+                                                        int* __restrict__ tile_sums, int n, int lg) {
+  // padded LDS: entry e at e + e / 32 (a thread's consecutive entries then
+  // fall in distinct banks for neighbouring threads)
   __shared__ int s[kTile + kTile / 32];
   __shared__ int wsum[kThreads / 64];
-  const int base = blockIdx.x * kTile, t = threadIdx.x;
+  const int items = 1 << (lg - 8);
+  const int base = blockIdx.x << lg, t = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
     const int e = k * kThreads + t;
-    s[e + (e >> 5)] = base + e < n ? in[base + e] : 0;
+    if (k < items) s[e + (e >> 5)] = base + e < n ? in[base + e] : 0;
   }
   __syncthreads();
   int v[kTileItems];
   int sum = 0;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
-    const int e = t * kTileItems + k;
-    v[k] = s[e + (e >> 5)];
+    const int e = t * items + k;
+    v[k] = k < items ? s[e + (e >> 5)] : 0;
     sum += v[k];
   }
   int total;
   int run = block_exclusive(sum, wsum, &total);
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
-    const int e = t * kTileItems + k;
-    s[e + (e >> 5)] = run;
+    const int e = t * items + k;
+    if (k < items) s[e + (e >> 5)] = run;
     run += v[k];
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
     const int e = k * kThreads + t;
-    if (base + e < n) out[base + e] = s[e + (e >> 5)];
+    if (k < items && base + e < n) out[base + e] = s[e + (e >> 5)];
   }
   if (t == 0) tile_sums[blockIdx.x] = total;
 }
 
-// Bucket offsets (all T buckets) and the droplets in bucket order.  A cleared
-// grid (no buckets) leaves every droplet without pairs.
+// Bucket offsets (all T buckets) and the droplets in bucket order (slot =
+// bucket offset + rank).  A cleared grid (no buckets) leaves every droplet
+// without pairs.
 __global__ __launch_bounds__(kThreads) void k_hash_scatter(
     int n, int T, const HashParams* __restrict__ hpp, const int* __restrict__ bucket,
-    const double* __restrict__ pos, const double* __restrict__ r, const int* __restrict__ local,
-    const int* __restrict__ btile, int ntile, int* __restrict__ offsets, int* __restrict__ fill,
-    int* __restrict__ sorted, double4* __restrict__ spos, int* __restrict__ pair_cnt) {
+    const int* __restrict__ rank, const double* __restrict__ pos, const double* __restrict__ r,
+    const int* __restrict__ local, const int* __restrict__ btile, int ntile, int lg,
+    int* __restrict__ offsets, int* __restrict__ sorted, double4* __restrict__ spos,
+    int* __restrict__ pair_cnt) {
   extern __shared__ int pre[];
   tile_prefix(btile, ntile, pre);
   const int stride = gridDim.x * kThreads;
   const int g0 = blockIdx.x * kThreads + threadIdx.x;
-  for (int h = g0; h < T; h += stride) offsets[h] = local[h] + pre[h / kTile];
+  for (int h = g0; h < T; h += stride) offsets[h] = local[h] + pre[h >> lg];
   const bool skip = hpp->skip;
   for (int i = g0; i < n; i += stride) {
     if (skip) {
@@ -291,7 +307,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_scatter(
       continue;
     }
     const int h = bucket[i];
-    const int slot = local[h] + pre[h / kTile] + atomicAdd(&fill[h], 1);
+    const int slot = local[h] + pre[h >> lg] + rank[i];
     sorted[slot] = i;
     spos[slot] = make_double4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2],
                               r[i]);
@@ -340,23 +356,24 @@ struct SearchLds {
 // One wave per kBucketsPerWave consecutive buckets (their counts and offsets
 // probed by one load per lane).  Targets of a bucket are taken 64 at a time
 // and grouped by cell (aliased cells of one bucket have different
-// neighbourhoods); a group's candidates are the slots of its distinct
-// neighbour buckets, spread over the lanes and tested against every target
-// of the group.  Cross-lane values move by readlane (scalar) where the source
+// neighbourhoods); a group's candidates are the slots of its neighbour
+// buckets, kCandPerLane per lane with all loads in flight, and each lane
+// tests its candidates against every target of the group (targets in the
+// outer loop).  Cross-lane values move by readlane (scalar) where the source
 // lane is uniform.
-__global__ __launch_bounds__(kThreads) void k_hash_search(
+__global__ __launch_bounds__(kSearchThreads) void k_hash_search(
     int T, double L, int periodic, int mask, const HashParams* __restrict__ hpp, const int* __restrict__ oob,
     const int* __restrict__ counts, const int* __restrict__ offsets,
     const int* __restrict__ sorted, const double4* __restrict__ spos, int* __restrict__ pair_cnt,
     int* __restrict__ stash) {
-  __shared__ SearchLds lds_all[kThreads / 64];
+  __shared__ SearchLds lds_all[kSearchThreads / 64];
   SearchLds& S = lds_all[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const HashParams hp = *hpp;
   if (hp.skip) return;
   const int g = hp.g;
   const bool in_box = *oob == 0;
-  const int b0 = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * kBucketsPerWave;
+  const int b0 = (blockIdx.x * (kSearchThreads / 64) + (threadIdx.x >> 6)) * kBucketsPerWave;
   int my_c = 0, my_o = 0;
   if (lane < kBucketsPerWave && b0 + lane < T) {
     my_c = counts[b0 + lane];
@@ -417,11 +434,12 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
           for (int t = before; t < before + len && t < kMapCap; ++t) S.nbr[t] = (unsigned char)lane;
         }
         wave_sync();
-        // two candidates per lane and pass, both loads in flight before the tests
-        for (int m0 = 0; m0 < M; m0 += 128) {
-          int slot[2], nbk[2];
+        // kCandPerLane candidates per lane and pass, all loads in flight
+        // before the tests
+        for (int m0 = 0; m0 < M; m0 += 64 * kCandPerLane) {
+          int slot[kCandPerLane], nbk[kCandPerLane];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
+          for (int u = 0; u < kCandPerLane; ++u) {
             const int m = m0 + 64 * u + lane;
             int k = 0;
             if (m < kMapCap) {
@@ -437,33 +455,36 @@ __global__ __launch_bounds__(kThreads) void k_hash_search(
             nbk[u] = k;
             slot[u] = m < M ? S.start[k] + (m - S.before[k]) : -1;
           }
-          double4 cand[2];
+          double4 cand[kCandPerLane];
 #pragma unroll
-          for (int u = 0; u < 2; ++u)
+          for (int u = 0; u < kCandPerLane; ++u)
             if (slot[u] >= 0) cand[u] = spos[slot[u]];
+          // targets outer, candidates inner: a target's values are read once
+          // and the lane's candidate tests are independent of each other
+          int j[kCandPerLane], first[kCandPerLane];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            if (slot[u] < 0) continue;
-            const double4 q = cand[u];
-            int j = -1, first = -1;
-            unsigned long long mm = members;
-            while (mm) {
-              const int t = __ffsll((long long)mm) - 1;
-              mm &= mm - 1;
-              if (contact_pair(S.x[t], S.y[t], S.z[t], S.r[t], q.x, q.y, q.z, q.w, L, per_g)) {
-                if (j < 0) j = sorted[slot[u]];
-                if (S.id[t] < j) {
-                  if (first < 0) {  // counted only in the first neighbour with this bucket
-                    const int h = S.bucket[nbk[u]];
-                    first = 1;
-                    for (int qk = 0; qk < nbk[u]; ++qk)
-                      if (S.bucket[qk] == h) first = 0;
-                  }
-                  if (!first) break;
-                  const int p = atomicAdd(&S.cnt[t], 1);
-                  if (p < kStash) S.list[t][p] = j;
-                }
+          for (int u = 0; u < kCandPerLane; ++u) j[u] = first[u] = -1;
+          unsigned long long mm = members;
+          while (mm) {
+            const int t = __ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            const double tx = S.x[t], ty = S.y[t], tz = S.z[t], tr = S.r[t];
+#pragma unroll
+            for (int u = 0; u < kCandPerLane; ++u) {
+              if (slot[u] < 0 || first[u] == 0) continue;
+              const double4 q = cand[u];
+              if (!contact_pair(tx, ty, tz, tr, q.x, q.y, q.z, q.w, L, per_g)) continue;
+              if (j[u] < 0) j[u] = sorted[slot[u]];
+              if (S.id[t] >= j[u]) continue;
+              if (first[u] < 0) {  // counted only in the first neighbour with this bucket
+                const int h = S.bucket[nbk[u]];
+                first[u] = 1;
+                for (int qk = 0; qk < nbk[u]; ++qk)
+                  if (S.bucket[qk] == h) first[u] = 0;
+                if (!first[u]) continue;  // and no later target of the group counts it
               }
+              const int p = atomicAdd(&S.cnt[t], 1);
+              if (p < kStash) S.list[t][p] = j[u];
             }
           }
         }
@@ -550,7 +571,7 @@ __device__ void rescan_droplet(int i, int off, int cnt, const double* __restrict
 // *count_out (block 0).
 __global__ __launch_bounds__(kThreads) void k_hash_emit(
     int n, const int* __restrict__ pair_cnt, const int* __restrict__ pscan,
-    const int* __restrict__ ptile, int ntile, const int* __restrict__ stash,
+    const int* __restrict__ ptile, int ntile, int lg, const int* __restrict__ stash,
     const double* __restrict__ pos, const double* __restrict__ r, double L, int periodic,
     int mask, const HashParams* __restrict__ hpp, const int* __restrict__ counts,
     const int* __restrict__ offsets, const int* __restrict__ sorted,
@@ -562,7 +583,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_emit(
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const int cnt = pair_cnt[i];
     if (cnt == 0) continue;
-    const int off = pscan[i] + pre[i / kTile];
+    const int off = pscan[i] + pre[i >> lg];
     if (cnt <= kStash) {
       for (int a = 0; a < cnt; ++a) {
         const int at = off + a;
@@ -613,36 +634,36 @@ int run_hash(const Layout& l, const double* pos, const double* radii, int n, dou
   if (prep_blocks > 2048) prep_blocks = 2048;
   if (prep_blocks < nrb) prep_blocks = nrb;
   hipLaunchKernelGGL(k_hash_prep, dim3(prep_blocks), dim3(kThreads), 0, st, radii, n, nrb,
-                     l.counts, l.fill, T, l.rmax_part, l.oob);
+                     l.counts, T, l.rmax_part, l.oob);
   DEM_CHECK_LAUNCH("k_hash_prep");
   hipLaunchKernelGGL(k_hash_assign, dim3(nb), dim3(kThreads), 0, st, pos, n, per, mask, L,
-                     (const double*)l.rmax_part, nrb, l.hp, l.bucket, l.counts, l.oob);
+                     (const double*)l.rmax_part, nrb, l.hp, l.bucket, l.rank, l.counts, l.oob);
   DEM_CHECK_LAUNCH("k_hash_assign");
-  const int ntT = tiles(T);
+  const int lgT = tile_log2(T), ntT = tiles(T);
   hipLaunchKernelGGL(k_tile_scan, dim3(ntT), dim3(kThreads), 0, st, (const int*)l.counts, l.local,
-                     l.btile, T);
+                     l.btile, T, lgT);
   DEM_CHECK_LAUNCH("k_tile_scan<buckets>");
   int sc_blocks = ((n > T ? n : T) + kThreads - 1) / kThreads;
   if (sc_blocks > 4096) sc_blocks = 4096;
   hipLaunchKernelGGL(k_hash_scatter, dim3(sc_blocks), dim3(kThreads), ntT * sizeof(int), st, n, T,
-                     (const HashParams*)l.hp, (const int*)l.bucket, pos, radii,
-                     (const int*)l.local, (const int*)l.btile, ntT, l.offsets, l.fill, l.sorted,
-                     l.spos, l.pair_cnt);
+                     (const HashParams*)l.hp, (const int*)l.bucket, (const int*)l.rank, pos,
+                     radii, (const int*)l.local, (const int*)l.btile, ntT, lgT, l.offsets,
+                     l.sorted, l.spos, l.pair_cnt);
   DEM_CHECK_LAUNCH("k_hash_scatter");
   const int waves = (T + kBucketsPerWave - 1) / kBucketsPerWave;
-  const int se_blocks = (waves + kThreads / 64 - 1) / (kThreads / 64);
-  hipLaunchKernelGGL(k_hash_search, dim3(se_blocks), dim3(kThreads), 0, st, T, L, per, mask,
+  const int se_blocks = (waves + kSearchThreads / 64 - 1) / (kSearchThreads / 64);
+  hipLaunchKernelGGL(k_hash_search, dim3(se_blocks), dim3(kSearchThreads), 0, st, T, L, per, mask,
                      (const HashParams*)l.hp, (const int*)l.oob, (const int*)l.counts,
                      (const int*)l.offsets,
                      (const int*)l.sorted, (const double4*)l.spos, l.pair_cnt, l.stash);
   DEM_CHECK_LAUNCH("k_hash_search");
-  const int ntn = tiles(n);
+  const int lgn = tile_log2(n), ntn = tiles(n);
   hipLaunchKernelGGL(k_tile_scan, dim3(ntn), dim3(kThreads), 0, st, (const int*)l.pair_cnt,
-                     l.pscan, l.ptile, n);
+                     l.pscan, l.ptile, n, lgn);
   DEM_CHECK_LAUNCH("k_tile_scan<pairs>");
   int em_blocks = nb < 2048 ? nb : 2048;
   hipLaunchKernelGGL(k_hash_emit, dim3(em_blocks), dim3(kThreads), ntn * sizeof(int), st, n,
-                     (const int*)l.pair_cnt, (const int*)l.pscan, (const int*)l.ptile, ntn,
+                     (const int*)l.pair_cnt, (const int*)l.pscan, (const int*)l.ptile, ntn, lgn,
                      (const int*)l.stash, pos, radii, L, per, mask, (const HashParams*)l.hp,
                      (const int*)l.counts, (const int*)l.offsets, (const int*)l.sorted,
                      (const double4*)l.spos, pairs, pair_capacity, count_out);

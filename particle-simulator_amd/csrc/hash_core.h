@@ -61,6 +61,10 @@ __device__ __forceinline__ bool contact_pair(double xi, double yi, double zi, do
   // so its correctly rounded value (within 2^-53 relative) is above rs too:
   // the same answer as the test below.
   if (d2 > rs * rs * (1.0 + 0x1p-40)) return false;
+  // Near pairs are rare: the empty volatile statement keeps the square root
+  // behind a branch that waves skip, where the compiler would otherwise
+  // compute it (a quarter-rate v_rsq_f64 and its refinement) for every pair.
+  __asm__ volatile("");
   const double dist = sqrt(d2);
   return dist <= rs;
 }

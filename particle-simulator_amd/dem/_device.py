@@ -58,6 +58,41 @@ def empty(shape, dtype="float64"):
     return t.empty(shape, dtype=getattr(t, dtype), device=device())
 
 
+_hip = None
+
+
+def _hip_runtime() -> ctypes.CDLL:
+    """torch's HIP runtime (the library libdemhip links against)."""
+    global _hip
+    if _hip is None:
+        import os
+
+        _hip = ctypes.CDLL(os.path.join(os.path.dirname(torch().__file__), "lib", "libamdhip64.so"))
+    return _hip
+
+
+class HostWord:
+    """One int32 in pinned host memory that kernels write through its device
+    address; read() waits for the current stream and returns the value.  A
+    count read this way needs no device-to-host copy."""
+
+    def __init__(self):
+        t = torch()
+        device()
+        self._h = t.zeros(1, dtype=t.int32, pin_memory=True)
+        self._view = self._h.numpy()
+        d = ctypes.c_void_p()
+        rc = _hip_runtime().hipHostGetDevicePointer(
+            ctypes.byref(d), ctypes.c_void_p(self._h.data_ptr()), 0)
+        if rc != 0 or not d.value:
+            raise RuntimeError(f"hipHostGetDevicePointer failed (status {rc})")
+        self.dev = ctypes.c_void_p(d.value)
+
+    def read(self) -> int:
+        torch().cuda.current_stream().synchronize()
+        return int(self._view[0])
+
+
 class Workspace:
     """A grow-only device scratch buffer (uint8)."""
 

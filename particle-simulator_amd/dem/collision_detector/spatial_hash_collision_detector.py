@@ -54,7 +54,7 @@ class SpatialHashCollisionDetector(CollisionDetector):
         lib = _native.load()
         ws = self._ws.get(lib.dem_hash_workspace_bytes(n, self._hash_table_size))
         if self._count is None:
-            self._count = t.zeros(1, dtype=t.int32, device=_device.device())
+            self._count = _device.HostWord()  # the kernels write the total here
         cap = max(1024, n)
         if self._pairs is None or self._pairs.shape[0] < cap:
             self._pairs = t.empty((cap, 2), dtype=t.int32, device=_device.device())
@@ -62,9 +62,9 @@ class SpatialHashCollisionDetector(CollisionDetector):
             cap = int(self._pairs.shape[0])
             _native.call("dem_hash_pairs", _device.ptr(pos_d), _device.ptr(radii_d), n,
                          ctypes.c_double(float(L)), 1 if periodic else 0, self._hash_table_size,
-                         _device.ptr(self._pairs), cap, _device.ptr(self._count),
+                         _device.ptr(self._pairs), cap, self._count.dev,
                          _device.ptr(ws), ws.numel(), _device.stream())
-            k = int(self._count.item())
+            k = self._count.read()
             if k <= cap:
                 return k, self._pairs[:k]
             self._pairs = t.empty((_next_power_of_2(k), 2), dtype=t.int32,
