@@ -161,7 +161,8 @@ __device__ __forceinline__ int tile_prefix(const int* __restrict__ tile_sums, in
   const int per = (nt + kThreads - 1) / kThreads;
   const int b = threadIdx.x * per;
   int sum = 0;
-  for (int k = 0; k < per; ++k)
+#pragma unroll 4
+  for (int k = 0; k < per; ++k)  // (unrolled: the loads go out together)
     if (b + k < nt) sum += tile_sums[b + k];
   int total;
   int run = block_exclusive(sum, wsum, &total);
@@ -208,12 +209,24 @@ __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restri
                                                           int* __restrict__ rank,
                                                           int* __restrict__ counts,
                                                           int* __restrict__ oob) {
+  static_assert(kMaxRBlocks <= kThreads, "one partial maximum per thread");
   __shared__ HashParams shp;
-  if (threadIdx.x < 64) {
-    double m = -INFINITY;
-    for (int b = threadIdx.x; b < nrb; b += 64) m = fmax(m, rmax_part[b]);
+  __shared__ double wm[kThreads / 64];
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  // the droplet's position loads go out together with the partial maxima's
+  double px = 0.0, py = 0.0, pz = 0.0;
+  if (i < n) {
+    px = pos[3 * (size_t)i];
+    py = pos[3 * (size_t)i + 1];
+    pz = pos[3 * (size_t)i + 2];
+  }
+  {
+    double m = (int)threadIdx.x < nrb ? rmax_part[threadIdx.x] : -INFINITY;
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
     if (threadIdx.x == 0) {
+      for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, wm[w]);
       HashParams p;
       p.cell = 2.0 * m;
       p.skip = !(p.cell > 0.0);
@@ -227,17 +240,14 @@ __global__ __launch_bounds__(kThreads) void k_hash_assign(const double* __restri
   }
   __syncthreads();
   const HashParams hp = shp;
-  const int i = blockIdx.x * kThreads + threadIdx.x;
   bool out = false;  // (NaN counts as outside)
-  if (periodic && i < n) {
-    const double* p = pos + 3 * (size_t)i;
-    out = !(p[0] >= 0.0 && p[0] <= L && p[1] >= 0.0 && p[1] <= L && p[2] >= 0.0 && p[2] <= L);
-  }
+  if (periodic && i < n)
+    out = !(px >= 0.0 && px <= L && py >= 0.0 && py <= L && pz >= 0.0 && pz <= L);
   if (__syncthreads_or(out) && threadIdx.x == 0) atomicOr(oob, 1);
   if (i >= n || hp.skip) return;
-  const int ix = cell_of(pos[3 * (size_t)i], hp.cell, hp.g, periodic);
-  const int iy = cell_of(pos[3 * (size_t)i + 1], hp.cell, hp.g, periodic);
-  const int iz = cell_of(pos[3 * (size_t)i + 2], hp.cell, hp.g, periodic);
+  const int ix = cell_of(px, hp.cell, hp.g, periodic);
+  const int iy = cell_of(py, hp.cell, hp.g, periodic);
+  const int iz = cell_of(pz, hp.cell, hp.g, periodic);
   const int h = hash_cell(ix, iy, iz, mask);
   bucket[i] = h;
   rank[i] = atomicAdd(&counts[h], 1);
