@@ -160,6 +160,49 @@ def test_n65536_scale_bit_exact(gpu):
     print(f"\n[N=65536] g={g} L/cell={s.L / (2 * r.max()):.3f} pairs={len(ref)}")
 
 
+def _hash_through_abi(pos, r, L, periodic, table_size):
+    """dem_hash_pairs with an explicit table size, pairs as a host array."""
+    import ctypes
+
+    t = _device.torch()
+    n = pos.shape[0]
+    pd, rd = _device.to_device(pos), _device.to_device(r)
+    lib = _native.load()
+    ws = t.empty(lib.dem_hash_workspace_bytes(n, table_size), dtype=t.uint8, device="cuda")
+    cap = 4 * n
+    pairs = t.empty((cap, 2), dtype=t.int32, device="cuda")
+    cnt = t.zeros(1, dtype=t.int32, device="cuda")
+    _native.call("dem_hash_pairs", _device.ptr(pd), _device.ptr(rd), n, ctypes.c_double(L),
+                 int(periodic), table_size, _device.ptr(pairs), cap, _device.ptr(cnt),
+                 _device.ptr(ws), ws.numel(), _device.stream())
+    k = int(cnt.item())
+    assert k <= cap
+    return pairs[:k].cpu().numpy()
+
+
+@pytest.mark.parametrize("n,log2_table,periodic", [
+    (300_000, 23, True),     # bucket-count scan in 1024-entry tiles
+    (300_000, 25, False),    # 4096-entry tiles (the largest)
+    (2_200_000, 23, True),   # pair-count scan in 512-entry tiles
+])
+def test_scan_tile_sizes_bit_exact(gpu, n, log2_table, periodic):
+    """The counts are scanned in tiles sized to the input (256 entries up to
+    2^21 entries, then 512, 1024, ... 4096): large tables and n > 2^21 take
+    the wider tiles.  Uniform droplets with radii for ~0.03-0.04 contacts
+    each; the oracle uses the same table size."""
+    rng = np.random.default_rng(n + log2_table)
+    L = 1e-3
+    pos = rng.uniform(0.0, L, size=(n, 3))
+    rmax = 2e-6 if n < 1_000_000 else 1e-6
+    r = rng.uniform(0.5, 1.0, size=n) * rmax
+    T = 1 << log2_table
+    got = _hash_through_abi(pos, r, L, periodic, T)
+    ok, ref = oracle.hash_pairs(pos, r, L, periodic, T)
+    assert ok and len(ref) > 1000
+    assert np.array_equal(got, ref)
+    print(f"\n[n={n} T=2^{log2_table}] pairs={len(ref)}")
+
+
 def test_unsynced_capacity_growth(gpu):
     """More pairs than the initial buffer: the operator grows it, nothing is lost."""
     rng = np.random.default_rng(9)
