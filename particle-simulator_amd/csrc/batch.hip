@@ -64,6 +64,7 @@ using dem::RecC;
 
 constexpr int kRow = 256;       // droplets per row
 constexpr int kSearchSplit = 4;  // lanes per droplet in the contact count pass
+constexpr int kCandUnroll = 2;   // candidate loads in flight per lane in the count pass
 constexpr int kScanWG = 1024;   // threads of a per-system scan workgroup
 constexpr int kZeroBlocks = 1024;
 constexpr int kIncCap = 1024;   // pairs an incremental re-check holds per system
@@ -808,11 +809,18 @@ __device__ int count_slot(const BEns& E, const BDev& D, const int* __restrict__ 
   int cnt = 0;
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
-    for (int q = s0[u]; q < s1[u]; ++q) {
-      const double4 c = spos[q];
-      if (dem::contact_pair(me.x, me.y, me.z, me.w, c.x, c.y, c.z, c.w, E.L, 1) &&
-          i < sorted[q] && nb.first(sub + SPLIT * u, hk[u]))
-        ++cnt;
+    // kCandUnroll candidates per round, their loads in flight together
+    for (int q = s0[u]; q < s1[u]; q += kCandUnroll) {
+      double4 c[kCandUnroll];
+#pragma unroll
+      for (int t = 0; t < kCandUnroll; ++t)
+        if (q + t < s1[u]) c[t] = spos[q + t];
+#pragma unroll
+      for (int t = 0; t < kCandUnroll; ++t)
+        if (q + t < s1[u] &&
+            dem::contact_pair(me.x, me.y, me.z, me.w, c[t].x, c[t].y, c[t].z, c[t].w, E.L, 1) &&
+            i < sorted[q + t] && nb.first(sub + SPLIT * u, hk[u]))
+          ++cnt;
     }
   }
   return cnt;
