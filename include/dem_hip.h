@@ -179,9 +179,12 @@ size_t dem_hash_workspace_bytes(int n, int table_size);
  * cell = 2*max(r), g = max(1, ceil(L/cell)), periodic wrap or open clamp,
  * hash (ix*73856093 ^ iy*19349663 ^ iz*83492791) & (table_size-1).
  * Pairs are written unique and lexicographically sorted to
- * pairs[0 .. 2*min(count, pair_capacity)); *count_out (device int32) receives
- * the number of distinct pairs (may exceed pair_capacity: the caller then
- * retries with a larger buffer).  periodic: 1 periodic, 0 open. */
+ * pairs[0 .. 2*min(count, pair_capacity)); *count_out receives the number
+ * of distinct pairs (may exceed pair_capacity: the caller then retries with a
+ * larger buffer).  count_out is a device address: device memory, or pinned
+ * host memory through its device address (hipHostGetDevicePointer), which the
+ * host reads after synchronising the stream without a copy.  periodic: 1
+ * periodic, 0 open. */
 int dem_hash_pairs(const double* pos, const double* radii, int n, double L,
                    int periodic, int table_size, int32_t* pairs,
                    int pair_capacity, int32_t* count_out, void* ws,
