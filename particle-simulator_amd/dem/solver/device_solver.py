@@ -19,9 +19,13 @@ steps.
 * forces + velocities: the operator's ``*_device`` entry point;
 * Euler update and wrap: ``dem_euler_step`` / ``dem_euler_step_total``,
   bit-exact with the reference's numpy update;
-* ``solution.save_step`` receives a host copy only on saved steps; the
-  segment's ``initial_droplet_state.positions``, which the host backend
-  advances in place, receives the state when the segment ends.
+* ``solution.save_step`` receives a host copy only on saved steps.  The
+  copy is queued at the save and the row is written after the next step's
+  kernels are launched (or before anything reads the solution: a merge, a
+  new segment, the end of ``solve``), so the host copy runs while the GPU
+  computes; with live plots on, rows are written at once.  The segment's
+  ``initial_droplet_state.positions``, which the host backend advances in
+  place, receives the state when the segment ends.
 
 Given identical operator outputs, trajectories are bit-identical to
 ``EulerDropletSolver``'s.
@@ -43,6 +47,8 @@ class DeviceEulerDropletSolver(EulerDropletSolver):
                          save_interval, pbm_coupling)
         self.verbose = verbose
         self._pin = None      # pinned staging for device -> host copies
+        self._ring = None     # pinned slot of a queued save
+        self._pending = None  # (t, slot view, event, solution) of a row not yet written
         self._carried = None  # device state of a segment opened by a device merge
         self._pos = self._rad = None
         self._saved_is_state = False  # the last saved row equals the device state
@@ -50,6 +56,7 @@ class DeviceEulerDropletSolver(EulerDropletSolver):
     # ------------------------------------------------ device state backend
 
     def _open_segment(self) -> None:
+        self._finish_save()
         sol = self.solution
         if self._carried is not None:
             self._pos, self._rad = self._carried
@@ -72,6 +79,7 @@ class DeviceEulerDropletSolver(EulerDropletSolver):
         return pairs_d.cpu().numpy() if k > 0 else None
 
     def _merge(self, pairs) -> None:
+        self._finish_save()
         sol = self.solution
         sol.is_collision = True
         sol.collided_droplets = pairs
@@ -98,12 +106,39 @@ class DeviceEulerDropletSolver(EulerDropletSolver):
                          _device.ptr(self._rad), n, float(self.stokes_factor), float(dt),
                          float(fc.L), periodic, _device.stream())
         self._saved_is_state = False
+        self._finish_save()  # the last saved row, written while the step computes
 
     def _positions_to_save(self):
         self._saved_is_state = True
         return self._staged(self._pos)
 
+    def _save(self, t: float) -> None:
+        self._finish_save()
+        if self.solution.real_time_visualization:  # live plots read the row at once
+            self.solution.save_step(t, self._positions_to_save())
+            return
+        torch = _device.torch()
+        k = self._pos.numel()
+        if self._ring is None or self._ring.numel() < k:
+            self._ring = torch.empty(max(k, 1 << 16), dtype=torch.float64, pin_memory=True)
+        v = self._ring[:k].view(self._pos.shape)
+        v.copy_(self._pos, non_blocking=True)  # ordered before the next step's kernels
+        ev = torch.cuda.Event()
+        ev.record()
+        self._pending = (t, v, ev, self.solution)
+        self._saved_is_state = True
+
+    def _finish_save(self) -> None:
+        """Writes a queued row into the solution it was saved from."""
+        if self._pending is None:
+            return
+        t, v, ev, sol = self._pending
+        self._pending = None
+        ev.synchronize()
+        sol.save_step(t, v.numpy())
+
     def _close(self) -> None:
+        self._finish_save()
         if self._pos is not None and self._carried is None:
             self._hand_back(self._staged(self._pos))
 

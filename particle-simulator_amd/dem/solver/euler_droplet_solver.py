@@ -121,6 +121,10 @@ class EulerDropletSolver(Solver):
     def _positions_to_save(self):
         return self._x
 
+    def _save(self, t: float) -> None:
+        """A saved step: the positions' row in the solution."""
+        self.solution.save_step(t, self._positions_to_save())
+
     def _close(self) -> None:
         """End of solve(): nothing is held outside the solution."""
 
@@ -157,7 +161,7 @@ class EulerDropletSolver(Solver):
             t += dt
             self.steps_done += 1
             if self.save_interval <= 1 or step % self.save_interval == 0 or step == n_steps - 1:
-                self.solution.save_step(t, self._positions_to_save())
+                self._save(t)
             if self.solution.should_update_visualization():
                 self.post_processor.live_plot()
         return t, "done"
