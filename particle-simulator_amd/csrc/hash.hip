@@ -66,7 +66,7 @@ constexpr int kMaxRBlocks = 256;              // partial maxima of the radii
 constexpr int kStash = 8;                     // partners kept per target by the search
 constexpr int kBucketsPerWave = 4;
 constexpr int kMapCap = 256;  // candidates per group mapped to their neighbour in LDS
-constexpr int kCandPerLane = 3;  // candidate loads in flight per lane and pass
+constexpr int kCandPerLane = 2;  // candidate loads in flight per lane and pass
 // One wave per search workgroup: a wave that finds its buckets empty frees
 // its LDS at once instead of holding it until the block's slowest wave ends.
 constexpr int kSearchThreads = 64;
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_hash_search(
         // kCandPerLane candidates per lane and pass, all loads in flight
         // before the tests
         for (int m0 = 0; m0 < M; m0 += 64 * kCandPerLane) {
-          int slot[kCandPerLane], nbk[kCandPerLane];
+          int slot[kCandPerLane];
 #pragma unroll
           for (int u = 0; u < kCandPerLane; ++u) {
             const int m = m0 + 64 * u + lane;
@@ -462,7 +462,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_hash_search(
                 else hi = mid - 1;
               }
             }
-            nbk[u] = k;
             slot[u] = m < M ? S.start[k] + (m - S.before[k]) : -1;
           }
           double4 cand[kCandPerLane];
@@ -470,10 +469,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_hash_search(
           for (int u = 0; u < kCandPerLane; ++u)
             if (slot[u] >= 0) cand[u] = spos[slot[u]];
           // targets outer, candidates inner: a target's values are read once
-          // and the lane's candidate tests are independent of each other
-          int j[kCandPerLane], first[kCandPerLane];
-#pragma unroll
-          for (int u = 0; u < kCandPerLane; ++u) j[u] = first[u] = -1;
+          // and the lane's candidate tests are independent of each other.
+          // Contacts are rare: their bookkeeping (the candidate's index, its
+          // neighbour and the first-neighbour rule) is recomputed per contact,
+          // which keeps the loop at six waves per SIMD.
           unsigned long long mm = members;
           while (mm) {
             const int t = __ffsll((long long)mm) - 1;
@@ -481,20 +480,31 @@ __global__ __launch_bounds__(kSearchThreads) void k_hash_search(
             const double tx = S.x[t], ty = S.y[t], tz = S.z[t], tr = S.r[t];
 #pragma unroll
             for (int u = 0; u < kCandPerLane; ++u) {
-              if (slot[u] < 0 || first[u] == 0) continue;
+              if (slot[u] < 0) continue;
               const double4 q = cand[u];
               if (!contact_pair(tx, ty, tz, tr, q.x, q.y, q.z, q.w, L, per_g)) continue;
-              if (j[u] < 0) j[u] = sorted[slot[u]];
-              if (S.id[t] >= j[u]) continue;
-              if (first[u] < 0) {  // counted only in the first neighbour with this bucket
-                const int h = S.bucket[nbk[u]];
-                first[u] = 1;
-                for (int qk = 0; qk < nbk[u]; ++qk)
-                  if (S.bucket[qk] == h) first[u] = 0;
-                if (!first[u]) continue;  // and no later target of the group counts it
+              const int j = sorted[slot[u]];
+              if (S.id[t] >= j) continue;
+              // counted only in the first neighbour holding this bucket
+              const int m = m0 + 64 * u + lane;
+              int k = 0;
+              if (m < kMapCap) {
+                k = S.nbr[m];
+              } else {
+                int hi = 26;
+                while (k < hi) {
+                  const int mid = (k + hi + 1) >> 1;
+                  if (S.before[mid] <= m) k = mid;
+                  else hi = mid - 1;
+                }
               }
+              const int h = S.bucket[k];
+              bool first = true;
+              for (int qk = 0; qk < k; ++qk)
+                if (S.bucket[qk] == h) first = false;
+              if (!first) continue;
               const int p = atomicAdd(&S.cnt[t], 1);
-              if (p < kStash) S.list[t][p] = j[u];
+              if (p < kStash) S.list[t][p] = j;
             }
           }
         }
