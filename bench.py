@@ -186,7 +186,9 @@ def hash_pass(s, reps: int = 20, radius_scale: float = 1.6) -> dict:
     return {"ms_per_pass": ms, "pairs": int(k), "radius_scale": radius_scale,
             "algorithmic_bytes": nbytes,
             "achieved_GBps": nbytes / (ms * 1e-3) / 1e9, "peak_GBps": 8000.0,
-            "note": "includes the host read-back of the pair count (drop-in semantics)"}
+            "note": "includes the host read of the pair count (drop-in semantics): the emit "
+                    "kernel stores it in pinned host memory as soon as the pair counts are "
+                    "scanned and the host polls it; the pairs are complete in stream order"}
 
 
 def step_leg(steps: int = 20) -> dict:

@@ -182,8 +182,10 @@ size_t dem_hash_workspace_bytes(int n, int table_size);
  * pairs[0 .. 2*min(count, pair_capacity)); *count_out receives the number
  * of distinct pairs (may exceed pair_capacity: the caller then retries with a
  * larger buffer).  count_out is a device address: device memory, or pinned
- * host memory through its device address (hipHostGetDevicePointer), which the
- * host reads after synchronising the stream without a copy.  periodic: 1
+ * host memory through its device address (hipHostGetDevicePointer).  The
+ * count is stored (at system scope) as soon as it is known, before the last
+ * kernel has written all pairs: a host may poll a pinned count word instead
+ * of synchronising, and reads the pairs in stream order.  periodic: 1
  * periodic, 0 open. */
 int dem_hash_pairs(const double* pos, const double* radii, int n, double L,
                    int periodic, int table_size, int32_t* pairs,

@@ -599,7 +599,12 @@ __global__ __launch_bounds__(kThreads) void k_hash_emit(
     int* __restrict__ count_out) {
   extern __shared__ int pre[];
   const int total = tile_prefix(ptile, ntile, pre);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *count_out = total;
+  // The total is final once the pair counts are scanned: workgroup 0 stores
+  // it at once at system scope (to pinned host memory it goes straight out),
+  // so a host polling the word learns it while the pairs are still being
+  // written — readers of the pairs come after this kernel in stream order.
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(count_out, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const int cnt = pair_cnt[i];
     if (cnt == 0) continue;
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_emit(
 }
 
 __global__ void k_zero_count(int* p) {
-  if (threadIdx.x == 0) *p = 0;
+  if (threadIdx.x == 0) __hip_atomic_store(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 int check_hash_args(const double* pos, const double* radii, int n, int table_size,

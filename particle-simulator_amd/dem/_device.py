@@ -73,8 +73,12 @@ def _hip_runtime() -> ctypes.CDLL:
 
 class HostWord:
     """One int32 in pinned host memory that kernels write through its device
-    address; read() waits for the current stream and returns the value.  A
-    count read this way needs no device-to-host copy."""
+    address.  arm() puts a sentinel (-1) there before the launch; read()
+    polls until the kernels have published a value and returns it, with no
+    device-to-host copy and no wait for the stream to drain (work the
+    kernels still do is ordered before anything later on the stream); a
+    word that does not arrive within a second falls back to a stream
+    synchronisation."""
 
     def __init__(self):
         t = torch()
@@ -88,9 +92,22 @@ class HostWord:
             raise RuntimeError(f"hipHostGetDevicePointer failed (status {rc})")
         self.dev = ctypes.c_void_p(d.value)
 
+    def arm(self) -> None:
+        self._view[0] = -1
+
     def read(self) -> int:
-        torch().cuda.current_stream().synchronize()
-        return int(self._view[0])
+        import time
+
+        v = self._view
+        deadline = None
+        while v[0] == -1:
+            now = time.perf_counter()
+            if deadline is None:
+                deadline = now + 1.0
+            elif now > deadline:
+                torch().cuda.current_stream().synchronize()
+                break
+        return int(v[0])
 
 
 class Workspace:

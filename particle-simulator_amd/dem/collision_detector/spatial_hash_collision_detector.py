@@ -60,6 +60,7 @@ class SpatialHashCollisionDetector(CollisionDetector):
             self._pairs = t.empty((cap, 2), dtype=t.int32, device=_device.device())
         while True:
             cap = int(self._pairs.shape[0])
+            self._count.arm()
             _native.call("dem_hash_pairs", _device.ptr(pos_d), _device.ptr(radii_d), n,
                          ctypes.c_double(float(L)), 1 if periodic else 0, self._hash_table_size,
                          _device.ptr(self._pairs), cap, self._count.dev,
