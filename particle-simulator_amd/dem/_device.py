@@ -106,6 +106,8 @@ class HostWord:
                 deadline = now + 1.0
             elif now > deadline:
                 torch().cuda.current_stream().synchronize()
+                if v[0] == -1:
+                    raise RuntimeError("dem: the kernels did not write the count word")
                 break
         return int(v[0])
 
